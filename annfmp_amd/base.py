@@ -1,0 +1,182 @@
+"""ANNFieldPropKDTreeHIP: the openmp backend's interface on MI355X.
+
+Mirrors ``ANNFieldPropKDTreeOpenMP`` (annfmp/openmp/base.py:16-219): same
+constructor parameters, ``fit(image_a, image_b) -> nn_indices`` (int32,
+``(N,)``, ``nn_indices[y * n_cols + x]`` = original index of the matched B
+patch), ``patches_a`` / ``patches_b`` and the ``_timers`` keys.
+
+The PCA basis is fitted exactly as the reference does (scikit-learn
+``extract_patches_2d`` samples of ``n_subset // 2`` patches from A with
+``random_state=seed`` and from B with ``seed + 1``, then ``PCA.fit``,
+openmp/base.py:168-191).  Everything after the fit runs on the GPU through
+``annf_field_dev``: patch extraction fused with the projection, the tree over
+B, row 0, the propagation sweep and best-of-K.  Patches are never
+materialised on the host unless ``patches_a`` / ``patches_b`` is read.
+
+Extra keyword arguments (not in the reference): ``device`` (GPU ordinal) and
+``projection`` ("fp32": FP32 accumulation; "fp64": FP64 accumulation rounded
+once to float32, which reproduces the reference's FP64 transform up to its
+last-bit rounding).  Extra outputs after ``fit``: ``nn_distances`` (full-patch
+L2 of each chosen match, float32) and ``knn_indices`` (the K candidates of
+every patch before best-of-K, original indices).
+"""
+from __future__ import annotations
+
+import collections
+import time
+
+import numpy
+
+from . import device as dev
+from . import _lib
+from .kdtree import tree_depth_for
+
+
+class ANNFieldPropKDTreeHIP:
+    def __init__(self, n_neighbors=3, psize=8, dim_reduced=5, n_subset=1000, leaf_size=64, n_jobs=-1,
+                 propagate=True, select_best_nn=True, float_type="float", verbose=1, seed=0, device=0,
+                 projection="fp32"):
+        self.n_neighbors = n_neighbors
+        self.psize = psize
+        self.dim_reduced = dim_reduced
+        self.n_subset = n_subset
+        self.leaf_size = leaf_size
+        self.n_jobs = n_jobs
+        self.propagate = propagate
+        self.select_best_nn = select_best_nn
+        self.float_type = float_type
+        self.verbose = verbose
+        self.seed = seed
+        self.device = device
+        self.projection = projection
+        self._handle = None
+
+    # ------------------------------------------------------------------
+    def fit(self, image_a, image_b):
+        numpy.random.seed(self.seed)
+        self._get_numpy_float_type()
+        self.image_a = image_a
+        self.image_b = image_b
+        self._timers = collections.OrderedDict()
+        self._n_cols = self.image_a.shape[1] - self.psize + 1
+        self._n_rows = self.image_a.shape[0] - self.psize + 1
+
+        tstart = time.time()
+        self._pca_model = self._fit_pca_model(self.image_a, self.image_b)
+        self._timers["pca_fit"] = time.time() - tstart
+
+        # since there is just one leaf visit, this must hold (openmp/base.py:113)
+        assert self.leaf_size > self.n_neighbors
+
+        n_b = (image_b.shape[0] - self.psize + 1) * (image_b.shape[1] - self.psize + 1)
+        depth = tree_depth_for(n_b, self.leaf_size)
+        res = field_on_device(image_a, image_b, self.psize, self._pca_model.components_, self._pca_model.mean_,
+                              depth, self.n_neighbors, self.propagate, self.select_best_nn,
+                              projection=self.projection, device=self.device, handle=self._get_handle())
+        nn_indices = res["nn"]
+        self.nn_distances = res["dist"]
+        self.knn_indices = res["knn"]
+        st = res["stages_ms"]
+        self._timers["extracting_patches"] = 0.0   # fused into the projection
+        self._timers["pca_apply"] = (st[0] + st[1]) / 1000.0
+        self._timers["kdtree_fit"] = st[2] / 1000.0
+        self._timers["remaining_rows"] = sum(st[3:]) / 1000.0
+        self._timers["device_total"] = res["wall_s"]
+        if self.verbose > 0:
+            print("-----------------------------------------")
+            print("--------------- RUNTIMES ----------------")
+            print("-----------------------------------------")
+            print("Extraction of patches:\t{}".format(self._timers["extracting_patches"]))
+            print("PCA fit:\t\t{}".format(self._timers["pca_fit"]))
+            print("PCA apply:\t\t{}".format(self._timers["pca_apply"]))
+            print("K-D Tree (fitting):\t{}".format(self._timers["kdtree_fit"]))
+            print("Remaining rows:\t\t{}".format(self._timers["remaining_rows"]))
+            print("Total runtime:\t\t{}".format(sum(self._timers[k] for k in
+                                                     ("extracting_patches", "pca_fit", "pca_apply",
+                                                      "kdtree_fit", "remaining_rows"))))
+            print("-----------------------------------------")
+        return nn_indices
+
+    def _get_handle(self):
+        if self._handle is None:
+            self._handle = _lib.Handle(self.device)
+        return self._handle
+
+    def _create_patches(self, img):
+        from sklearn.feature_extraction import image
+        n_channels = img.shape[-1]
+        patches = image.extract_patches_2d(img, (self.psize, self.psize))
+        return patches.reshape((-1, self.psize * self.psize * n_channels))
+
+    def _fit_pca_model(self, image_a, image_b):
+        """openmp/base.py:168-191 (scikit-learn, FP64)."""
+        from sklearn.feature_extraction import image
+        from sklearn.decomposition import PCA
+        subset_a = image.extract_patches_2d(image_a, (self.psize, self.psize), max_patches=int(self.n_subset / 2),
+                                            random_state=self.seed)
+        subset_b = image.extract_patches_2d(image_b, (self.psize, self.psize), max_patches=int(self.n_subset / 2),
+                                            random_state=self.seed + 1)
+        subset_a = subset_a.reshape((subset_a.shape[0], -1))
+        subset_b = subset_b.reshape((subset_b.shape[0], -1))
+        both_subsets = numpy.concatenate((subset_a, subset_b), axis=0)
+        model = PCA(n_components=self.dim_reduced)
+        model.fit(both_subsets)
+        return model
+
+    def _get_numpy_float_type(self):
+        if self.float_type == "float":
+            return numpy.float32
+        elif self.float_type == "double":
+            raise Exception("float_type 'double' is not available: the HIP backend computes in float32")
+        raise Exception("Unknown float type: {}".format(self.float_type))
+
+    @property
+    def patches_a(self):
+        return self._create_patches(self.image_a)
+
+    @property
+    def patches_b(self):
+        return self._create_patches(self.image_b)
+
+
+def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_neighbors, propagate=True,
+                    select_best_nn=True, projection="fp32", device=0, handle=None, return_reduced=False,
+                    timing=True):
+    """One call of annf_field_dev on host images; returns numpy results.
+
+    components: (d, p*p*3) basis, mean: (p*p*3,) -- sklearn PCA layout."""
+    torch = dev.torch()
+    h = handle or _lib.Handle(device)
+    L = _lib.lib()
+    cuda = dev.cuda_device(device)
+    ia = torch.from_numpy(numpy.ascontiguousarray(image_a, dtype=numpy.uint8)).to(cuda)
+    ib = torch.from_numpy(numpy.ascontiguousarray(image_b, dtype=numpy.uint8)).to(cuda)
+    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=numpy.float32)).to(cuda)
+    mu = torch.from_numpy(numpy.ascontiguousarray(mean, dtype=numpy.float64)).to(cuda)
+    d = comp.shape[0]
+    ha, wa = image_a.shape[:2]
+    hb, wb = image_b.shape[:2]
+    n_a = (ha - psize + 1) * (wa - psize + 1)
+    n_b = (hb - psize + 1) * (wb - psize + 1)
+    K = int(n_neighbors)
+    nn = torch.empty(n_a, dtype=torch.int32, device=cuda)
+    dist = torch.empty(n_a, dtype=torch.float32, device=cuda)
+    knn = torch.empty((n_a, K), dtype=torch.int32, device=cuda)
+    ra = torch.empty((n_a, d), dtype=torch.float32, device=cuda) if return_reduced else None
+    rb = torch.empty((n_b, d), dtype=torch.float32, device=cuda) if return_reduced else None
+    torch.cuda.synchronize(cuda)
+    _lib.check("annf_set_timing", L.annf_set_timing(h.h, int(bool(timing))))
+    t0 = time.time()
+    _lib.check("annf_field_dev", L.annf_field_dev(
+        h.h, dev.dptr(ia), ha, wa, dev.dptr(ib), hb, wb, int(psize), dev.dptr(comp), dev.dptr(mu), int(d),
+        1 if projection == "fp64" else 0, int(tree_depth), K, int(bool(propagate)), int(bool(select_best_nn)),
+        dev.dptr(nn), dev.dptr(dist), dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
+    wall = time.time() - t0
+    st = numpy.zeros(7, numpy.float32)
+    L.annf_stage_times(h.h, _lib.ptr(st, _lib._fp), 7)
+    out = {"nn": nn.cpu().numpy(), "dist": dist.cpu().numpy(), "knn": knn.cpu().numpy(),
+           "stages_ms": st.tolist(), "wall_s": wall}
+    if return_reduced:
+        out["red_a"] = ra.cpu().numpy()
+        out["red_b"] = rb.cpu().numpy()
+    return out

@@ -1,0 +1,606 @@
+// C ABI of libannf_hip.so (declared in include/annf_hip.h): handles, device
+// workspaces and the host-side sequencing of the kernels in annf_kernels.hip.
+#include "annf_internal.h"
+#include "../../include/annf_hip.h"
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace annf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(ANNF_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),  \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+#define RCHK(expr)                 \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != ANNF_OK) return r_; \
+    } while (0)
+
+// A growable device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    int ensure(size_t bytes) {
+        if (bytes <= cap && p) return ANNF_OK;
+        if (p) { (void)hipFree(p); p = nullptr; cap = 0; }
+        size_t want = bytes ? bytes : 16;
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();
+            p = nullptr;
+            return fail(ANNF_ENOMEM, "hipMalloc(%zu) failed", want);
+        }
+        cap = want;
+        return ANNF_OK;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+constexpr int kStages = 7;
+
+}  // namespace
+
+struct annf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // init_extern parameters (global.h:18-27)
+    bool inited = false;
+    int n_neighbors = 10, tree_depth = 0, max_leaves = 2147483640, num_threads = 1;
+    int splitting_type = ANNF_SPLIT_LONGEST, verbosity = 0;
+    // tree
+    bool has_tree = false;
+    TreeView tree;
+    // tree storage
+    DevBuf pts_a, pts_b, orig_a, orig_b, axis, split, leaves, pli;
+    // build scratch
+    DevBuf keys, eqoff, box, nax, prefix, rank, hist, cnt, tot, bnd;
+    // query workspaces
+    DevBuf qbuf, natleaf, bcount, boffset, bcursor, order, p1d, p1p, p1l, propleaf;
+    DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
+    DevBuf basis32, mean32, basis64, mean64;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[kStages + 1] = {};
+    bool ev_used[kStages + 1] = {};
+    float stage_ms[kStages] = {};
+    bool have_stages = false;
+
+    void mark(int i) {
+        if (!timing) return;
+        if (hipEventRecord(ev[i], stream) == hipSuccess) ev_used[i] = true;
+    }
+    void reset_marks() {
+        for (int i = 0; i <= kStages; ++i) ev_used[i] = false;
+        have_stages = false;
+    }
+    void collect_marks() {
+        if (!timing) return;
+        for (int i = 0; i < kStages; ++i) {
+            stage_ms[i] = 0.0f;
+            float ms = 0.0f;
+            if (ev_used[i] && ev_used[i + 1] && hipEventElapsedTime(&ms, ev[i], ev[i + 1]) == hipSuccess)
+                stage_ms[i] = ms;
+        }
+        have_stages = true;
+    }
+    // host copies of the level bounds (kept alive for the async upload)
+    std::vector<int> h_bnd, h_off, h_cpn;
+
+    void release_all() {
+        DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
+                          &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &qbuf, &natleaf, &bcount, &boffset,
+                          &bcursor, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
+                          &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64};
+        for (DevBuf *b : list) b->release();
+        has_tree = false;
+    }
+};
+
+namespace {
+
+int sync_stream(annf_ctx *c) {
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return ANNF_OK;
+}
+
+int set_device(annf_ctx *c) {
+    HIPCHK(hipSetDevice(c->device));
+    return ANNF_OK;
+}
+
+// Segment bounds of every level (kdtree.c:114-115: left child gets c/2).
+void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off, std::vector<int> &cpn,
+                  int &max_leaf) {
+    bnd.clear();
+    off.assign(depth + 1, 0);
+    cpn.assign(depth + 1, 1);
+    std::vector<int> cur = {0, n}, nxt;
+    for (int l = 0; l <= depth; ++l) {
+        off[l] = (int)bnd.size();
+        bnd.insert(bnd.end(), cur.begin(), cur.end());
+        int maxc = 0;
+        for (size_t j = 0; j + 1 < cur.size(); ++j) maxc = std::max(maxc, cur[j + 1] - cur[j]);
+        cpn[l] = std::max(1, (maxc + 2047) / 2048);
+        if (l == depth) {
+            max_leaf = maxc;
+            break;
+        }
+        nxt.clear();
+        for (size_t j = 0; j + 1 < cur.size(); ++j) {
+            nxt.push_back(cur[j]);
+            nxt.push_back(cur[j] + (cur[j + 1] - cur[j]) / 2);
+        }
+        nxt.push_back(n);
+        cur.swap(nxt);
+    }
+}
+
+// Builds the tree from n x dpad padded points already in c->pts_a.
+int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
+    std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn;
+    int max_leaf = 0;
+    level_bounds(n, depth, bnd, off, cpn, max_leaf);
+    const size_t nodes_max = depth > 0 ? ((size_t)1 << (depth - 1)) : 1;
+    size_t chunks_max = 1;
+    for (int l = 0; l < depth; ++l) chunks_max = std::max(chunks_max, ((size_t)1 << l) * cpn[l]);
+    const size_t nn = depth > 0 ? ((size_t)1 << depth) - 1 : 1;
+    RCHK(c->pts_b.ensure((size_t)n * dpad * sizeof(float)));
+    RCHK(c->orig_a.ensure((size_t)n * sizeof(int)));
+    RCHK(c->orig_b.ensure((size_t)n * sizeof(int)));
+    RCHK(c->axis.ensure(nn * sizeof(int)));
+    RCHK(c->split.ensure(nn * sizeof(float)));
+    RCHK(c->leaves.ensure(((size_t)2 << depth) * sizeof(int)));
+    RCHK(c->pli.ensure((size_t)n * sizeof(int)));
+    RCHK(c->keys.ensure((size_t)n * sizeof(uint32_t)));
+    RCHK(c->eqoff.ensure((size_t)n * sizeof(int)));
+    RCHK(c->box.ensure(nodes_max * dpad * 2 * sizeof(uint32_t)));
+    RCHK(c->nax.ensure(nodes_max * sizeof(int)));
+    RCHK(c->prefix.ensure(nodes_max * sizeof(uint32_t)));
+    RCHK(c->rank.ensure(nodes_max * sizeof(int)));
+    RCHK(c->hist.ensure(nodes_max * 256 * sizeof(uint32_t)));
+    RCHK(c->cnt.ensure(chunks_max * sizeof(int2)));
+    RCHK(c->tot.ensure(nodes_max * sizeof(int2)));
+    RCHK(c->bnd.ensure(bnd.size() * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(c->bnd.p, bnd.data(), bnd.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(iota(c->orig_a.as<int>(), n, c->stream));
+    BuildScratch s;
+    s.pts_a = c->pts_a.as<float>();
+    s.pts_b = c->pts_b.as<float>();
+    s.orig_a = c->orig_a.as<int>();
+    s.orig_b = c->orig_b.as<int>();
+    s.keys = c->keys.as<uint32_t>();
+    s.eqoff = c->eqoff.as<int>();
+    s.box = c->box.as<uint32_t>();
+    s.nax = c->nax.as<int>();
+    s.prefix = c->prefix.as<uint32_t>();
+    s.rank = c->rank.as<int>();
+    s.hist = c->hist.as<uint32_t>();
+    s.cnt = c->cnt.as<int2>();
+    s.tot = c->tot.as<int2>();
+    s.bnd = c->bnd.as<int>();
+    TreeView t;
+    HIPCHK(build_tree(n, d, dpad, depth, c->splitting_type, bnd.data(), off.data(), cpn.data(), s,
+                      c->axis.as<int>(), c->split.as<float>(), c->leaves.as<int>(), c->pli.as<int>(), t,
+                      c->stream));
+    t.max_leaf = max_leaf;
+    c->tree = t;
+    c->has_tree = true;
+    return ANNF_OK;
+}
+
+int check_tree_params(annf_ctx *c, int n, int d, int depth) {
+    if (!c->inited) return fail(ANNF_EINVAL, "annf_init must be called before fitting");
+    if (n < 1) return fail(ANNF_EINVAL, "empty training set");
+    if (d < 1 || d > ANNF_MAX_D) return fail(ANNF_EUNSUPPORTED, "dimension %d outside [1, %d]", d, ANNF_MAX_D);
+    if (depth < 0 || depth > 30) return fail(ANNF_EINVAL, "tree depth %d outside [0, 30]", depth);
+    if ((n >> depth) < 1)
+        return fail(ANNF_EINVAL, "tree depth %d leaves empty leaves for %d points", depth, n);
+    return ANNF_OK;
+}
+
+// Rows pipeline on padded queries in c->qbuf (n_a x dpad).
+int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate, int select_best,
+             const PatchGeom &ga, const PatchGeom &gb, int *final_dev, float *dist_dev, int *knn_dev) {
+    const TreeView &t = c->tree;
+    const int nleaves = 1 << t.depth;
+    const size_t nk = (size_t)n_a * K;
+    RCHK(c->fl_pos.ensure(nk * sizeof(int)));
+    RCHK(c->fl_d.ensure((size_t)n_cols * K * sizeof(float)));
+    RCHK(c->fl_l.ensure((size_t)n_cols * K * sizeof(int)));
+    int *propleaf = nullptr;
+    if (propagate) {
+        RCHK(c->propleaf.ensure((size_t)t.n * sizeof(int)));
+        propleaf = c->propleaf.as<int>();
+        HIPCHK(prop_leaves(t, n_a, n_cols, propleaf, c->stream));
+    }
+    c->mark(3);
+    if (n_rows > 1) {
+        RCHK(c->natleaf.ensure((size_t)n_a * sizeof(int)));
+        RCHK(c->bcount.ensure((size_t)nleaves * sizeof(int)));
+        RCHK(c->boffset.ensure(((size_t)nleaves + 1) * sizeof(int)));
+        RCHK(c->bcursor.ensure((size_t)nleaves * sizeof(int)));
+        RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
+        HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
+        HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), c->stream));
+        HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
+                              c->bcursor.as<int>(), c->order.as<int>(), c->stream));
+        if (propagate) {
+            RCHK(c->p1d.ensure(nk * sizeof(float)));
+            RCHK(c->p1p.ensure(nk * sizeof(int)));
+            RCHK(c->p1l.ensure(nk * sizeof(int)));
+            HIPCHK(leaf_scan(t, c->qbuf.as<float>(), c->order.as<int>(), c->boffset.as<int>(), K, propleaf,
+                             c->p1d.as<float>(), c->p1p.as<int>(), c->p1l.as<int>(), c->stream));
+        } else {
+            // without propagation the natural-leaf list is the row's list
+            RCHK(c->p1d.ensure(nk * sizeof(float)));
+            HIPCHK(leaf_scan(t, c->qbuf.as<float>(), c->order.as<int>(), c->boffset.as<int>(), K, nullptr,
+                             c->p1d.as<float>(), c->fl_pos.as<int>(), nullptr, c->stream));
+        }
+    }
+    c->mark(4);
+    HIPCHK(exact_search(t, c->qbuf.as<float>(), n_cols, K, 2147483640LL, propleaf, c->fl_d.as<float>(),
+                        c->fl_pos.as<int>(), c->fl_l.as<int>(), c->stream));
+    c->mark(5);
+    if (propagate && n_rows > 1) {
+        HIPCHK(row_sweep(t, c->qbuf.as<float>(), n_rows, n_cols, K, propleaf, c->p1d.as<float>(),
+                         c->p1p.as<int>(), c->p1l.as<int>(), c->fl_pos.as<int>(), c->fl_l.as<int>(), c->stream));
+    }
+    c->mark(6);
+    HIPCHK(best_of_k(ga, gb, c->fl_pos.as<int>(), t.orig, n_a, K, select_best, final_dev, dist_dev, knn_dev,
+                     c->stream));
+    c->mark(7);
+    return ANNF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *annf_last_error(void) { return g_err.c_str(); }
+
+int annf_create(annf_ctx **out, int device) {
+    if (!out) return fail(ANNF_EINVAL, "null output handle");
+    *out = nullptr;
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) return fail(ANNF_EINVAL, "device %d not present (%d devices)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    annf_ctx *c = new annf_ctx();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(ANNF_EHIP, "hipStreamCreateWithFlags failed");
+    }
+    for (int i = 0; i <= kStages; ++i) {
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            delete c;
+            return fail(ANNF_EHIP, "hipEventCreate failed");
+        }
+    }
+    *out = c;
+    return ANNF_OK;
+}
+
+int annf_destroy(annf_ctx *c) {
+    if (!c) return ANNF_OK;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->release_all();
+    for (int i = 0; i <= kStages; ++i)
+        if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return ANNF_OK;
+}
+
+void *annf_stream(annf_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int annf_init(annf_ctx *c, int n_neighbors, int tree_depth, int max_leaves, int num_threads, int splitting_type,
+              int verbosity_level) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    // helper.c:30-52
+    if (n_neighbors < 1 || n_neighbors > 1000) return fail(ANNF_EINVAL, "k must be > 0 and <= 1000");
+    if (tree_depth < 0 || tree_depth > 1000) return fail(ANNF_EINVAL, "tree_depth must positive and <= 1000");
+    if (num_threads < 1) return fail(ANNF_EINVAL, "num_threads must be > 0");
+    if (verbosity_level < 0 || verbosity_level > 1) return fail(ANNF_EINVAL, "verbosity_level must either 0 or 1");
+    // kdtree.c:419-423
+    if (splitting_type != ANNF_SPLIT_CYCLIC && splitting_type != ANNF_SPLIT_LONGEST)
+        return fail(ANNF_EINVAL, "Unknown splitting type!");
+    if (max_leaves < 0) return fail(ANNF_EINVAL, "max_leaves must be a positive integer");
+    c->n_neighbors = n_neighbors;
+    c->tree_depth = tree_depth;
+    c->max_leaves = max_leaves;
+    c->num_threads = num_threads;
+    c->splitting_type = splitting_type;
+    c->verbosity = verbosity_level;
+    c->inited = true;
+    return ANNF_OK;
+}
+
+int annf_fit_dev(annf_ctx *c, const float *X, int n, int d) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    RCHK(set_device(c));
+    RCHK(check_tree_params(c, n, d, c->tree_depth));
+    const int dpad = dpad_of(d);
+    c->has_tree = false;
+    RCHK(c->pts_a.ensure((size_t)n * dpad * sizeof(float)));
+    HIPCHK(pad_rows(X, n, d, dpad, c->pts_a.as<float>(), c->stream));
+    RCHK(build_from_padded(c, n, d, dpad, c->tree_depth));
+    return sync_stream(c);
+}
+
+int annf_fit(annf_ctx *c, const float *X, int n, int d) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!X) return fail(ANNF_EINVAL, "null Xtrain");
+    RCHK(set_device(c));
+    RCHK(check_tree_params(c, n, d, c->tree_depth));
+    RCHK(c->stage_in.ensure((size_t)n * d * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(c->stage_in.p, X, (size_t)n * d * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    return annf_fit_dev(c, c->stage_in.as<float>(), n, d);
+}
+
+int annf_neighbors(annf_ctx *c, const float *Xtest, int nXtest, int dXtest, float *distances, int ndistances,
+                   int ddistances, int *indices, int nindices, int dindices) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    RCHK(set_device(c));
+    const TreeView &t = c->tree;
+    const int K = ddistances;
+    if (dXtest != t.d) return fail(ANNF_EINVAL, "query dimension %d != tree dimension %d", dXtest, t.d);
+    if (ndistances != nXtest || nindices != nXtest || dindices != K)
+        return fail(ANNF_EINVAL, "output shapes do not match the queries");
+    if (!k_supported(K)) return fail(ANNF_EUNSUPPORTED, "n_neighbors %d outside [1, %d]", K, ANNF_MAX_K);
+    if (nXtest == 0) return ANNF_OK;
+    const size_t nk = (size_t)nXtest * K;
+    RCHK(c->stage_in.ensure((size_t)nXtest * dXtest * sizeof(float)));
+    RCHK(c->qbuf.ensure((size_t)nXtest * t.dpad * sizeof(float)));
+    RCHK(c->p1d.ensure(nk * sizeof(float)));
+    RCHK(c->p1p.ensure(nk * sizeof(int)));
+    RCHK(c->final_i.ensure(nk * sizeof(int)));
+    RCHK(c->final_d.ensure(nk * sizeof(float)));
+    HIPCHK(hipMemcpyAsync(c->stage_in.p, Xtest, (size_t)nXtest * dXtest * sizeof(float), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(pad_rows(c->stage_in.as<float>(), nXtest, dXtest, t.dpad, c->qbuf.as<float>(), c->stream));
+    HIPCHK(exact_search(t, c->qbuf.as<float>(), nXtest, K, (long long)c->max_leaves, nullptr, c->p1d.as<float>(),
+                        c->p1p.as<int>(), nullptr, c->stream));
+    HIPCHK(convert_knn(c->p1p.as<int>(), c->p1d.as<float>(), t.orig, (long long)nk, c->final_i.as<int>(),
+                       c->final_d.as<float>(), c->stream));
+    HIPCHK(hipMemcpyAsync(indices, c->final_i.p, nk * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(distances, c->final_d.p, nk * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    return sync_stream(c);
+}
+
+int annf_process_rows(annf_ctx *c, int *finalindices, int nfinalindices, const uint8_t *patchesA, int npatchesA,
+                      int dpatchesA, const uint8_t *patchesB, int npatchesB, int dpatchesB, const float *patchesred,
+                      int npatchesred, int dpatchesred, int n_rows, int n_cols, int n_neighbors, int propagate,
+                      int select_best_nn) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    RCHK(set_device(c));
+    const TreeView &t = c->tree;
+    const int K = n_neighbors;
+    if (!k_supported(K)) return fail(ANNF_EUNSUPPORTED, "n_neighbors %d outside [1, %d]", K, ANNF_MAX_K);
+    if (dpatchesred != t.d) return fail(ANNF_EINVAL, "reduced dimension %d != tree dimension %d", dpatchesred, t.d);
+    if (n_rows < 1 || n_cols < 1 || (long long)n_rows * n_cols != npatchesred)
+        return fail(ANNF_EINVAL, "n_rows * n_cols (%d * %d) != number of reduced patches %d", n_rows, n_cols,
+                    npatchesred);
+    if (nfinalindices != npatchesred) return fail(ANNF_EINVAL, "finalindices has %d entries, expected %d",
+                                                  nfinalindices, npatchesred);
+    if (propagate && npatchesred > t.n)
+        return fail(ANNF_EINVAL, "propagation needs at most as many query patches (%d) as tree points (%d)",
+                    npatchesred, t.n);
+    if (select_best_nn) {
+        if (!patchesA || !patchesB) return fail(ANNF_EINVAL, "select_best_nn needs both patch arrays");
+        if (npatchesA != npatchesred) return fail(ANNF_EINVAL, "patchesA rows %d != %d", npatchesA, npatchesred);
+        if (npatchesB != t.n) return fail(ANNF_EINVAL, "patchesB rows %d != tree points %d", npatchesB, t.n);
+        if (dpatchesA != dpatchesB || dpatchesA < 1) return fail(ANNF_EINVAL, "patch dimensions differ");
+    }
+    const int n_a = npatchesred;
+    c->reset_marks();
+    c->mark(0);
+    RCHK(c->stage_in.ensure((size_t)n_a * t.d * sizeof(float)));
+    RCHK(c->qbuf.ensure((size_t)n_a * t.dpad * sizeof(float)));
+    RCHK(c->final_i.ensure((size_t)n_a * sizeof(int)));
+    HIPCHK(hipMemcpyAsync(c->stage_in.p, patchesred, (size_t)n_a * t.d * sizeof(float), hipMemcpyHostToDevice,
+                          c->stream));
+    HIPCHK(pad_rows(c->stage_in.as<float>(), n_a, t.d, t.dpad, c->qbuf.as<float>(), c->stream));
+    PatchGeom ga{}, gb{};
+    if (select_best_nn) {
+        const size_t ba = (size_t)npatchesA * dpatchesA, bb = (size_t)npatchesB * dpatchesB;
+        RCHK(c->stage_in2.ensure(ba));
+        RCHK(c->stage_in3.ensure(bb));
+        HIPCHK(hipMemcpyAsync(c->stage_in2.p, patchesA, ba, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->stage_in3.p, patchesB, bb, hipMemcpyHostToDevice, c->stream));
+        ga = PatchGeom{c->stage_in2.as<uint8_t>(), 0, 0, 0, 1, dpatchesA, 0};
+        gb = PatchGeom{c->stage_in3.as<uint8_t>(), 0, 0, 0, 1, dpatchesB, 0};
+    }
+    c->mark(1);
+    c->mark(2);
+    RCHK(run_rows(c, n_a, n_rows, n_cols, K, propagate, select_best_nn, ga, gb, c->final_i.as<int>(), nullptr,
+                  nullptr));
+    HIPCHK(hipMemcpyAsync(finalindices, c->final_i.p, (size_t)n_a * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    RCHK(sync_stream(c));
+    c->collect_marks();
+    return ANNF_OK;
+}
+
+float annf_norm_array(const uint8_t *a, const uint8_t *b, int dim) {
+    float s = 0.0f;
+    for (int i = 0; i < dim; ++i) {
+        float df = (float)a[i] - (float)b[i];
+        s += df * df;
+    }
+    return (float)std::sqrt((double)s);
+}
+
+int annf_free_resources(annf_ctx *c) {
+    if (!c) return ANNF_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->release_all();
+    return ANNF_OK;
+}
+
+int annf_tree_info(annf_ctx *c, int *n, int *d, int *depth) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    if (n) *n = c->tree.n;
+    if (d) *d = c->tree.d;
+    if (depth) *depth = c->tree.depth;
+    return ANNF_OK;
+}
+
+int annf_tree_export(annf_ctx *c, int *axis, float *split, int *leaves, int *xi_orig, float *xi_pts, int *pli) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    RCHK(set_device(c));
+    const TreeView &t = c->tree;
+    const size_t nn = ((size_t)1 << t.depth) - 1;
+    if (axis && nn) HIPCHK(hipMemcpyAsync(axis, t.axis, nn * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (split && nn) HIPCHK(hipMemcpyAsync(split, t.split, nn * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (leaves)
+        HIPCHK(hipMemcpyAsync(leaves, t.leaves, ((size_t)2 << t.depth) * sizeof(int), hipMemcpyDeviceToHost,
+                              c->stream));
+    if (xi_orig) HIPCHK(hipMemcpyAsync(xi_orig, t.orig, (size_t)t.n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (pli) HIPCHK(hipMemcpyAsync(pli, t.pli, (size_t)t.n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (xi_pts)
+        HIPCHK(hipMemcpy2DAsync(xi_pts, (size_t)t.d * sizeof(float), t.pts, (size_t)t.dpad * sizeof(float),
+                                (size_t)t.d * sizeof(float), (size_t)t.n, hipMemcpyDeviceToHost, c->stream));
+    return sync_stream(c);
+}
+
+}  // extern "C"
+
+namespace {
+
+__global__ void k_prep_basis(const float *basis, const double *mean, int d, int D, int dpad, float *bt32,
+                             float *m32, double *bt64, double *m64) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < D * dpad) {
+        int f = i / dpad, k = i - f * dpad;
+        float v = k < d ? basis[(size_t)k * D + f] : 0.0f;
+        bt32[i] = v;
+        bt64[i] = (double)v;
+    }
+    if (i < D) {
+        m32[i] = (float)mean[i];
+        m64[i] = mean[i];
+    }
+}
+
+__global__ void k_unpad(const float *src, long long n, int d, int dpad, float *dst) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * d) return;
+    long long r = i / d;
+    int c = (int)(i - r * d);
+    dst[i] = src[r * dpad + c];
+}
+
+}  // namespace
+
+extern "C" {
+
+int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
+                   int psize, const float *basis, const double *mean, int d, int proj_fp64, int tree_depth,
+                   int n_neighbors, int propagate, int select_best_nn, int *nn_out, float *dist_out, int *knn_out,
+                   float *red_a_out, float *red_b_out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!img_a || !img_b || !basis || !mean || !nn_out) return fail(ANNF_EINVAL, "null required pointer");
+    RCHK(set_device(c));
+    const int p = psize;
+    if (p < 1 || ha < p || wa < p || hb < p || wb < p) return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
+    if (d < 1 || d > ANNF_MAX_D) return fail(ANNF_EUNSUPPORTED, "dimension %d outside [1, %d]", d, ANNF_MAX_D);
+    const int K = n_neighbors;
+    if (!k_supported(K)) return fail(ANNF_EUNSUPPORTED, "n_neighbors %d outside [1, %d]", K, ANNF_MAX_K);
+    const int D = p * p * 3;
+    const int nr_a = ha - p + 1, nc_a = wa - p + 1, nr_b = hb - p + 1, nc_b = wb - p + 1;
+    const long long n_a = (long long)nr_a * nc_a, n_b = (long long)nr_b * nc_b;
+    if (n_a > 0x7FFFFFFFLL || n_b > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "too many patches");
+    if (propagate && n_a > n_b)
+        return fail(ANNF_EINVAL, "propagation needs at most as many A patches as B patches");
+    if (!c->inited) RCHK(annf_init(c, K, tree_depth, 2147483640, 1, ANNF_SPLIT_LONGEST, 0));
+    c->tree_depth = tree_depth;
+    RCHK(check_tree_params(c, (int)n_b, d, tree_depth));
+    const int dpad = dpad_of(d);
+    c->reset_marks();
+    c->mark(0);
+    RCHK(c->basis32.ensure((size_t)D * dpad * sizeof(float)));
+    RCHK(c->mean32.ensure((size_t)D * sizeof(float)));
+    RCHK(c->basis64.ensure((size_t)D * dpad * sizeof(double)));
+    RCHK(c->mean64.ensure((size_t)D * sizeof(double)));
+    hipLaunchKernelGGL(k_prep_basis, dim3((D * dpad + 255) / 256), dim3(256), 0, c->stream, basis, mean, d, D, dpad,
+                       c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
+                       c->mean64.as<double>());
+    HIPCHK(hipGetLastError());
+    RCHK(c->qbuf.ensure((size_t)n_a * dpad * sizeof(float)));
+    HIPCHK(project_patches(img_a, ha, wa, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
+                           c->mean64.as<double>(), d, dpad, proj_fp64, c->qbuf.as<float>(), c->stream));
+    c->mark(1);
+    c->has_tree = false;
+    RCHK(c->pts_a.ensure((size_t)n_b * dpad * sizeof(float)));
+    HIPCHK(project_patches(img_b, hb, wb, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
+                           c->mean64.as<double>(), d, dpad, proj_fp64, c->pts_a.as<float>(), c->stream));
+    if (red_b_out) {
+        hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n_b * d + 255) / 256)), dim3(256), 0, c->stream,
+                           c->pts_a.as<float>(), n_b, d, dpad, red_b_out);
+        HIPCHK(hipGetLastError());
+    }
+    c->mark(2);
+    RCHK(build_from_padded(c, (int)n_b, d, dpad, tree_depth));
+    if (red_a_out) {
+        hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n_a * d + 255) / 256)), dim3(256), 0, c->stream,
+                           c->qbuf.as<float>(), n_a, d, dpad, red_a_out);
+        HIPCHK(hipGetLastError());
+    }
+    PatchGeom ga{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa};
+    PatchGeom gb{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb};
+    RCHK(run_rows(c, (int)n_a, nr_a, nc_a, K, propagate, select_best_nn, ga, gb, nn_out, dist_out, knn_out));
+    RCHK(sync_stream(c));
+    c->collect_marks();
+    return ANNF_OK;
+}
+
+int annf_set_timing(annf_ctx *c, int on) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    c->timing = on != 0;
+    return ANNF_OK;
+}
+
+int annf_stage_times(annf_ctx *c, float *ms, int max_stages) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!c->have_stages) return 0;
+    int n = std::min(max_stages, kStages);
+    for (int i = 0; i < n; ++i) ms[i] = c->stage_ms[i];
+    return n;
+}
+
+}  // extern "C"

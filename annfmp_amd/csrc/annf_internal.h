@@ -1,0 +1,94 @@
+// Internal declarations shared by the kernel and ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace annf {
+
+constexpr int kWave = 64;
+constexpr float kFltMax = 3.402823466e+38f;  // list sentinel, float.h:24
+
+// A k-d tree resident in HBM.  Tree positions are the reference's XI order
+// (kdtree.c:434-451 then the in-place partitions): pts holds the reduced
+// vectors in that order, zero-padded from d to dpad (a multiple of 4) so a
+// point is dpad/4 float4 loads.
+struct TreeView {
+    int n = 0, d = 0, dpad = 0, depth = 0;
+    int max_leaf = 0;                 // largest leaf population
+    const float *pts = nullptr;       // n * dpad
+    const int *orig = nullptr;        // n
+    const int *axis = nullptr;        // 2^depth - 1
+    const float *split = nullptr;     // 2^depth - 1
+    const int *leaves = nullptr;      // 2 * 2^depth, [from, to)
+    const int *pli = nullptr;         // n, by original index
+};
+
+// Device scratch owned by the caller of the build.
+struct BuildScratch {
+    float *pts_a, *pts_b;             // n * dpad each (ping-pong)
+    int *orig_a, *orig_b;             // n each
+    uint32_t *keys;                   // n
+    int *eqoff;                       // n
+    uint32_t *box;                    // nodes_max * dpad * 2
+    int *nax;                         // nodes_max
+    uint32_t *prefix;                 // nodes_max
+    int *rank;                        // nodes_max
+    uint32_t *hist;                   // nodes_max * 256
+    int2 *cnt;                        // total chunks of the widest level
+    int2 *tot;                        // nodes_max
+    int *bnd;                         // all levels' bounds
+};
+
+// How a patch is laid out for best-of-K: `rows` runs of `rowlen` bytes,
+// consecutive runs `rstride` apart.  Patch i of an image starts at
+// ((i / ncols) * width + i % ncols) * 3; a patch matrix row at i * rowlen.
+struct PatchGeom {
+    const uint8_t *base;
+    int image;        // 1: windows of an image, 0: patch matrix
+    int ncols;        // patches per image row (image mode)
+    int width;        // image width in pixels (image mode)
+    int rows;
+    int rowlen;
+    long long rstride;
+};
+
+// ---- launchers (annf_kernels.hip); all enqueue on `st` ----
+hipError_t pad_rows(const float *src, int n, int d, int dpad, float *dst, hipStream_t st);
+hipError_t iota(int *dst, int n, hipStream_t st);
+
+// Builds the tree; `t` receives the device pointers of the result (inside
+// scratch ping-pong buffers and the given node/leaf arrays).
+hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
+                      const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
+                      BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
+                      TreeView &t, hipStream_t st);
+
+hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count,
+                          hipStream_t st);
+hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset,
+                          int *cursor, int *order, hipStream_t st);
+hipError_t leaf_scan(const TreeView &t, const float *q, const int *order, const int *offset,
+                     int K, const int *propleaf, float *out_d, int *out_p, int *out_l,
+                     hipStream_t st);
+hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long long max_leaves,
+                        const int *propleaf, float *out_d, int *out_p, int *out_l,
+                        hipStream_t st);
+hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hipStream_t st);
+hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K,
+                     const int *propleaf, const float *p1d, const int *p1p, const int *p1l,
+                     int *fl_pos, const int *row0_l, hipStream_t st);
+hipError_t best_of_k(const PatchGeom &ga, const PatchGeom &gb, const int *fl_pos,
+                     const int *orig, int nq, int K, int select_best, int *final_out,
+                     float *dist_out, int *knn_out, hipStream_t st);
+hipError_t convert_knn(const int *pos, const float *d2, const int *orig, long long count,
+                       int *idx_out, float *dist_out, hipStream_t st);
+hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float *basis_t,
+                           const float *mean_f, const double *basis_t64, const double *mean64,
+                           int d, int dpad, int fp64, float *out, hipStream_t st);
+
+int dpad_of(int d);
+bool dpad_supported(int dpad);
+bool k_supported(int K);
+
+}  // namespace annf

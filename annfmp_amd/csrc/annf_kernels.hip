@@ -1,0 +1,1069 @@
+// HIP kernels of the propagation-assisted k-d-tree NNF path, for gfx950.
+//
+// The semantics follow the reference's openmp core (annfmp/openmp/kdtree/
+// src/*.c); the structure is GPU-first:
+//   * the tree is built level-synchronously: per-node data box (min/max
+//     reduction), first longest axis, radix select (4 x 8-bit LDS histograms)
+//     of the rank-c/2 value, and an order-exact partition computed from
+//     block scans (the closed form of helper.c:63-113, see k_scatter);
+//   * the natural-leaf scan of every query is one pass over queries bucketed
+//     by leaf, the leaf's points staged in LDS;
+//   * row 0's exact search and the per-row propagation run one query per
+//     wavefront: lanes score candidates, and the K best are extracted with
+//     wave-wide minima of (distance, arrival) keys, which is exactly the
+//     reference's sequential insert order (kdtree.c:502-532);
+//   * the propagation is a persistent sweep: one wavefront owns one image
+//     column and walks its rows; columns never wait on each other.
+//
+// Every FP32 distance is the sequential sum sum_j (p_j - q_j)^2 with one
+// rounding per operation (the library is compiled with -ffp-contract=off),
+// which is the reference's arithmetic (helper.c:164-179).
+#include "annf_internal.h"
+
+#include <type_traits>
+#include <utility>
+
+namespace annf {
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Order-preserving map float -> uint32 with -0 folded onto +0, so that keys
+// compare like floats under '<' and '==' (finite values).
+__device__ __forceinline__ uint32_t f2key(float f) {
+    uint32_t u = __float_as_uint(f);
+    if (u == 0x80000000u) u = 0u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+__device__ __forceinline__ int readlane_i(int v, int lane) {
+    return __builtin_amdgcn_readlane(v, lane);
+}
+__device__ __forceinline__ float readlane_f(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    int lo = __shfl_xor((int)(uint32_t)v, m, kWave);
+    int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, kWave);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+template <int DP>
+__device__ __forceinline__ void load_vec(const float *__restrict__ p, float (&v)[DP]) {
+#pragma unroll
+    for (int c = 0; c < DP / 4; ++c) {
+        float4 t = reinterpret_cast<const float4 *>(p)[c];
+        v[4 * c + 0] = t.x;
+        v[4 * c + 1] = t.y;
+        v[4 * c + 2] = t.z;
+        v[4 * c + 3] = t.w;
+    }
+}
+
+// helper.c:164-179 order: t = p_j - q_j; s += t * t, j ascending.  The
+// zero padding adds exact +0 terms after the last real dimension.
+template <int DP>
+__device__ __forceinline__ float dist_sq(const float *__restrict__ p, const float (&q)[DP]) {
+    float s = 0.0f;
+#pragma unroll
+    for (int c = 0; c < DP / 4; ++c) {
+        float4 v = reinterpret_cast<const float4 *>(p)[c];
+        float t;
+        t = v.x - q[4 * c + 0]; s = s + t * t;
+        t = v.y - q[4 * c + 1]; s = s + t * t;
+        t = v.z - q[4 * c + 2]; s = s + t * t;
+        t = v.w - q[4 * c + 3]; s = s + t * t;
+    }
+    return s;
+}
+
+template <typename Fn>
+static hipError_t with_dp(int dpad, Fn &&fn) {
+    switch (dpad) {
+    case 4: return fn(std::integral_constant<int, 4>{});
+    case 8: return fn(std::integral_constant<int, 8>{});
+    case 12: return fn(std::integral_constant<int, 12>{});
+    case 16: return fn(std::integral_constant<int, 16>{});
+    case 20: return fn(std::integral_constant<int, 20>{});
+    case 24: return fn(std::integral_constant<int, 24>{});
+    case 28: return fn(std::integral_constant<int, 28>{});
+    case 32: return fn(std::integral_constant<int, 32>{});
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename Fn>
+static hipError_t with_dp_km(int dpad, int K, Fn &&fn) {
+    return with_dp(dpad, [&](auto dp) {
+        if (K <= 8) return fn(dp, std::integral_constant<int, 8>{});
+        return fn(dp, std::integral_constant<int, 32>{});
+    });
+}
+
+int dpad_of(int d) { return (d + 3) & ~3; }
+bool dpad_supported(int dpad) { return dpad >= 4 && dpad <= 32 && dpad % 4 == 0; }
+bool k_supported(int K) { return K >= 1 && K <= 32; }
+
+static inline unsigned blocks_for(long long n, int tpb) {
+    return (unsigned)((n + tpb - 1) / tpb);
+}
+
+// ---------------------------------------------------------------------------
+// small utility kernels
+// ---------------------------------------------------------------------------
+
+__global__ void k_pad(const float *__restrict__ src, int n, int d, int dpad, float *__restrict__ dst) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    long long tot = (long long)n * dpad;
+    if (i >= tot) return;
+    long long r = i / dpad;
+    int c = (int)(i - r * dpad);
+    dst[i] = c < d ? src[r * d + c] : 0.0f;
+}
+
+__global__ void k_iota(int *dst, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = i;
+}
+
+hipError_t pad_rows(const float *src, int n, int d, int dpad, float *dst, hipStream_t st) {
+    long long tot = (long long)n * dpad;
+    if (tot == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pad, dim3(blocks_for(tot, 256)), dim3(256), 0, st, src, n, d, dpad, dst);
+    return hipGetLastError();
+}
+
+hipError_t iota(int *dst, int n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_iota, dim3(blocks_for(n, 256)), dim3(256), 0, st, dst, n);
+    return hipGetLastError();
+}
+
+// Block-wide exclusive scan of an int2 over 256 threads (4 waves).
+__device__ __forceinline__ int2 block_scan_int2(int2 v, int2 *wsum /* __shared__ [4] */) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int2 inc = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int tx = __shfl_up(inc.x, o, kWave);
+        int ty = __shfl_up(inc.y, o, kWave);
+        if (lane >= o) { inc.x += tx; inc.y += ty; }
+    }
+    if (lane == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    int2 off = make_int2(0, 0);
+    for (int w = 0; w < wid; ++w) { off.x += wsum[w].x; off.y += wsum[w].y; }
+    __syncthreads();
+    return make_int2(inc.x - v.x + off.x, inc.y - v.y + off.y);
+}
+
+// ---------------------------------------------------------------------------
+// tree build (level-synchronous)
+// ---------------------------------------------------------------------------
+
+constexpr int kBuildTPB = 256;
+constexpr int kBuildEPT = 8;
+constexpr int kChunk = kBuildTPB * kBuildEPT;
+
+struct LevelArgs {
+    int nodes;        // 2^lvl
+    int cpn;          // blocks (chunks of kChunk elements) per node
+    const int *bnd;   // nodes + 1 segment bounds
+    int node0;        // heap index of the level's first node
+    int lvl;
+};
+
+__device__ __forceinline__ void chunk_range(const LevelArgs &L, int &node, int &lo, int &hi, int &s, int &e) {
+    node = blockIdx.x / L.cpn;
+    int chunk = blockIdx.x - node * L.cpn;
+    lo = L.bnd[node];
+    hi = L.bnd[node + 1];
+    s = lo + chunk * kChunk;
+    e = min(s + kChunk, hi);
+}
+
+__global__ void k_level_init(LevelArgs L, int dpad, uint32_t *box, uint32_t *prefix, int *rank, uint32_t *hist) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L.nodes * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
+    if (i < L.nodes) { prefix[i] = 0u; rank[i] = (L.bnd[i + 1] - L.bnd[i]) / 2; }
+    if (i < L.nodes * 256) hist[i] = 0u;
+}
+
+// Data box of each node (kdtree.c:379-394), as min/max of ordered keys.
+template <int DP>
+__global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts, LevelArgs L, int d, uint32_t *box) {
+    int node, lo, hi, s, e;
+    chunk_range(L, node, lo, hi, s, e);
+    if (s >= e) return;
+    __shared__ uint32_t smn[DP], smx[DP];
+    if (threadIdx.x < DP) { smn[threadIdx.x] = 0xFFFFFFFFu; smx[threadIdx.x] = 0u; }
+    __syncthreads();
+    uint32_t mn[DP], mx[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) { mn[j] = 0xFFFFFFFFu; mx[j] = 0u; }
+    for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
+        float v[DP];
+        load_vec<DP>(pts + (size_t)i * DP, v);
+#pragma unroll
+        for (int j = 0; j < DP; ++j) {
+            uint32_t k = f2key(v[j]);
+            mn[j] = min(mn[j], k);
+            mx[j] = max(mx[j], k);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < DP; ++j) {
+#pragma unroll
+        for (int m = kWave / 2; m >= 1; m >>= 1) {
+            mn[j] = min(mn[j], (uint32_t)__shfl_xor((int)mn[j], m, kWave));
+            mx[j] = max(mx[j], (uint32_t)__shfl_xor((int)mx[j], m, kWave));
+        }
+    }
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+#pragma unroll
+        for (int j = 0; j < DP; ++j) {
+            atomicMin(&smn[j], mn[j]);
+            atomicMax(&smx[j], mx[j]);
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < d) {
+        atomicMin(&box[((size_t)node * DP + threadIdx.x) * 2 + 0], smn[threadIdx.x]);
+        atomicMax(&box[((size_t)node * DP + threadIdx.x) * 2 + 1], smx[threadIdx.x]);
+    }
+}
+
+// Split axis (kdtree.c:369-418): cyclic = depth % d; longest = first axis of
+// strictly largest FP32 width max - min, starting from 0.0.
+__global__ void k_axis(LevelArgs L, int d, int dpad, const uint32_t *box, int splitting_type, int *nax, int *axis_out) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= L.nodes) return;
+    int ax = 0;
+    if (splitting_type == 0) {
+        ax = L.lvl % d;
+    } else {
+        float longest = 0.0f;
+        for (int k = 0; k < d; ++k) {
+            float mn = key2f(box[((size_t)j * dpad + k) * 2 + 0]);
+            float mx = key2f(box[((size_t)j * dpad + k) * 2 + 1]);
+            float w = mx - mn;
+            if (w > longest) { longest = w; ax = k; }
+        }
+    }
+    nax[j] = ax;
+    axis_out[L.node0 + j] = ax;
+}
+
+// One 8-bit digit histogram pass of the radix select of the rank-c/2 key.
+template <int DP>
+__global__ __launch_bounds__(kBuildTPB) void k_hist(const float *__restrict__ pts, LevelArgs L, const int *nax,
+                                                    const uint32_t *prefix, int pass, uint32_t *keys, uint32_t *hist) {
+    int node, lo, hi, s, e;
+    chunk_range(L, node, lo, hi, s, e);
+    if (s >= e) return;
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const int ax = nax[node];
+    const uint32_t pre = prefix[node];
+    const int dsh = 24 - 8 * pass;
+    for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
+        uint32_t k;
+        if (pass == 0) {
+            k = f2key(pts[(size_t)i * DP + ax]);
+            keys[i] = k;
+        } else {
+            k = keys[i];
+        }
+        bool m = (pass == 0) || (((k ^ pre) >> (dsh + 8)) == 0u);
+        if (m) atomicAdd(&h[(k >> dsh) & 255u], 1u);
+    }
+    __syncthreads();
+    uint32_t c = h[threadIdx.x];
+    if (c) atomicAdd(&hist[(size_t)node * 256 + threadIdx.x], c);
+}
+
+// Picks the digit holding the remaining rank; one wavefront per node.
+__global__ void k_pick(LevelArgs L, int pass, uint32_t *hist, uint32_t *prefix, int *rank) {
+    const int node = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (node >= L.nodes) return;
+    uint32_t *h = hist + (size_t)node * 256;
+    uint32_t c0 = h[lane * 4 + 0], c1 = h[lane * 4 + 1], c2 = h[lane * 4 + 2], c3 = h[lane * 4 + 3];
+    h[lane * 4 + 0] = 0u; h[lane * 4 + 1] = 0u; h[lane * 4 + 2] = 0u; h[lane * 4 + 3] = 0u;
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    uint32_t inc = tot;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t t = (uint32_t)__shfl_up((int)inc, o, kWave);
+        if (lane >= o) inc += t;
+    }
+    const uint32_t exc = inc - tot;
+    const uint32_t r = (uint32_t)rank[node];
+    const bool mine = r >= exc && r < inc;
+    int digit = 0;
+    uint32_t before = exc;
+    if (mine) {
+        if (r < exc + c0) { digit = lane * 4 + 0; before = exc; }
+        else if (r < exc + c0 + c1) { digit = lane * 4 + 1; before = exc + c0; }
+        else if (r < exc + c0 + c1 + c2) { digit = lane * 4 + 2; before = exc + c0 + c1; }
+        else { digit = lane * 4 + 3; before = exc + c0 + c1 + c2; }
+    }
+    const unsigned long long b = __ballot(mine);
+    const int owner = __ffsll(b) - 1;
+    digit = readlane_i(digit, owner);
+    before = (uint32_t)readlane_i((int)before, owner);
+    if (lane == 0) {
+        prefix[node] |= (uint32_t)digit << (24 - 8 * pass);
+        rank[node] = (int)(r - before);
+    }
+}
+
+// Per-chunk counts of keys below / equal to the node's split key.
+__global__ __launch_bounds__(kBuildTPB) void k_count(LevelArgs L, const uint32_t *keys, const uint32_t *vkey, int2 *cnt) {
+    __shared__ int2 wsum[kBuildTPB / kWave];
+    int node, lo, hi, s, e;
+    chunk_range(L, node, lo, hi, s, e);
+    const uint32_t vk = vkey[node];
+    int lt = 0, eq = 0;
+    for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
+        uint32_t k = keys[i];
+        lt += k < vk;
+        eq += k == vk;
+    }
+    int2 tot;
+    int2 ex = block_scan_int2(make_int2(lt, eq), wsum);
+    if (threadIdx.x == blockDim.x - 1) {
+        tot = make_int2(ex.x + lt, ex.y + eq);
+        cnt[blockIdx.x] = tot;
+    }
+}
+
+// Exclusive offsets of the chunk counts inside each node; one block per node.
+__global__ __launch_bounds__(kBuildTPB) void k_scan_nodes(LevelArgs L, int2 *cnt, int2 *tot) {
+    __shared__ int2 wsum[kBuildTPB / kWave];
+    const int node = blockIdx.x;
+    int2 *c = cnt + (size_t)node * L.cpn;
+    const int per = (L.cpn + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per;
+    int2 loc = make_int2(0, 0);
+    for (int k = 0; k < per && b + k < L.cpn; ++k) { loc.x += c[b + k].x; loc.y += c[b + k].y; }
+    int2 ex = block_scan_int2(loc, wsum);
+    int2 run = ex;
+    for (int k = 0; k < per && b + k < L.cpn; ++k) {
+        int2 v = c[b + k];
+        c[b + k] = run;
+        run.x += v.x;
+        run.y += v.y;
+    }
+    if (threadIdx.x == blockDim.x - 1) tot[node] = run;
+}
+
+// Order-exact partition of one node segment (helper.c:63-113), in closed
+// form.  With L = #(x < v), E = #(x == v), g = c - L:
+//   x <  v : slot = rank among '<' elements (segment order)
+//   x == v : slot = L + t, t = (E-1) - rank among equal elements
+//   x >  v : G-offset go = (g-1) - rank among '>=' elements; follow
+//            go -> eqoff[go] while go < E; slot = L + go
+// where eqoff[t] is the G-offset of the t-th equal element in G order.
+// WRITE = false records eqoff; WRITE = true moves the records and the split.
+template <int DP, bool WRITE>
+__global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
+                                                       LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
+                                                       const int2 *cnt_off, const int2 *tot, int *eqoff,
+                                                       float *__restrict__ pts_out, int *__restrict__ orig_out,
+                                                       const int *nax, float *split) {
+    __shared__ int2 wsum[kBuildTPB / kWave];
+    int node, lo, hi, s, e;
+    chunk_range(L, node, lo, hi, s, e);
+    if (s >= e) return;
+    const int c = hi - lo;
+    const int Lt = tot[node].x, E = tot[node].y, g = c - Lt;
+    const uint32_t vk = vkey[node];
+    const int i0 = s + threadIdx.x * kBuildEPT;
+    uint32_t k[kBuildEPT];
+    int nlt = 0, neq = 0;
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) {
+        int i = i0 + u;
+        k[u] = (i < e) ? keys[i] : 0u;
+        if (i < e) { nlt += k[u] < vk; neq += k[u] == vk; }
+    }
+    int2 ex = block_scan_int2(make_int2(nlt, neq), wsum);
+    int r_lt = cnt_off[blockIdx.x].x + ex.x;
+    int r_eq = cnt_off[blockIdx.x].y + ex.y;
+    const int ax = WRITE ? nax[node] : 0;
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) {
+        int i = i0 + u;
+        if (i >= e) break;
+        const int off = i - lo;
+        int slot;
+        if (k[u] < vk) {
+            slot = r_lt++;
+        } else {
+            const int go = (g - 1) - (off - r_lt);
+            if (k[u] == vk) {
+                const int t = (E - 1) - r_eq++;
+                if (!WRITE) eqoff[lo + t] = go;
+                slot = Lt + t;
+            } else {
+                int p = go;
+                if (WRITE) {
+                    while (p < E) p = eqoff[lo + p];
+                }
+                slot = Lt + p;
+            }
+        }
+        if (WRITE) {
+            const int dst = lo + slot;
+            const float4 *src4 = reinterpret_cast<const float4 *>(pts_in + (size_t)i * DP);
+            float4 *dst4 = reinterpret_cast<float4 *>(pts_out + (size_t)dst * DP);
+#pragma unroll
+            for (int q = 0; q < DP / 4; ++q) dst4[q] = src4[q];
+            orig_out[dst] = orig_in[i];
+            if (slot == c / 2) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
+        }
+    }
+}
+
+// Leaf bounds and point -> leaf map (kdtree.c:86-100).
+__global__ void k_leaves(const int *bnd, int nleaves, const int *orig, int n, int *leaves, int *pli) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nleaves) { leaves[2 * i] = bnd[i]; leaves[2 * i + 1] = bnd[i + 1]; }
+    if (i < n) {
+        int a = 0, b = nleaves - 1;
+        while (a < b) {
+            int m = (a + b + 1) >> 1;
+            if (bnd[m] <= i) a = m; else b = m - 1;
+        }
+        pli[orig[i]] = a;
+    }
+}
+
+hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
+                      const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
+                      BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
+                      TreeView &t, hipStream_t st) {
+    (void)host_bnd;
+    float *pin = s.pts_a, *pout = s.pts_b;
+    int *oin = s.orig_a, *oout = s.orig_b;
+    hipError_t err = hipSuccess;
+    for (int lvl = 0; lvl < depth; ++lvl) {
+        LevelArgs L;
+        L.nodes = 1 << lvl;
+        L.cpn = host_cpn[lvl];
+        L.bnd = s.bnd + host_bnd_off[lvl];
+        L.node0 = L.nodes - 1;
+        L.lvl = lvl;
+        const unsigned nblk = (unsigned)(L.nodes * L.cpn);
+        hipLaunchKernelGGL(k_level_init, dim3(blocks_for((long long)L.nodes * 256, 256)), dim3(256), 0, st, L, dpad,
+                           s.box, s.prefix, s.rank, s.hist);
+        err = with_dp(dpad, [&](auto dp) {
+            constexpr int DP = decltype(dp)::value;
+            if (splitting_type != 0)
+                hipLaunchKernelGGL(k_box<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, d, s.box);
+            hipLaunchKernelGGL(k_axis, dim3(blocks_for(L.nodes, 256)), dim3(256), 0, st, L, d, dpad, s.box,
+                               splitting_type, s.nax, axis);
+            for (int pass = 0; pass < 4; ++pass) {
+                hipLaunchKernelGGL(k_hist<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, s.nax, s.prefix, pass,
+                                   s.keys, s.hist);
+                hipLaunchKernelGGL(k_pick, dim3(blocks_for((long long)L.nodes * kWave, 256)), dim3(256), 0, st, L,
+                                   pass, s.hist, s.prefix, s.rank);
+            }
+            hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, s.prefix, s.cnt);
+            hipLaunchKernelGGL(k_scan_nodes, dim3(L.nodes), dim3(kBuildTPB), 0, st, L, s.cnt, s.tot);
+            hipLaunchKernelGGL((k_scatter<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
+                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split);
+            hipLaunchKernelGGL((k_scatter<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
+                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split);
+            return hipGetLastError();
+        });
+        if (err != hipSuccess) return err;
+        std::swap(pin, pout);
+        std::swap(oin, oout);
+    }
+    const int nleaves = 1 << depth;
+    hipLaunchKernelGGL(k_leaves, dim3(blocks_for(std::max(n, nleaves), 256)), dim3(256), 0, st,
+                       s.bnd + host_bnd_off[depth], nleaves, oin, n, leaves, pli);
+    err = hipGetLastError();
+    t.n = n;
+    t.d = d;
+    t.dpad = dpad;
+    t.depth = depth;
+    t.pts = pin;
+    t.orig = oin;
+    t.axis = axis;
+    t.split = split;
+    t.leaves = leaves;
+    t.pli = pli;
+    return err;
+}
+
+// ---------------------------------------------------------------------------
+// natural leaves and the bucketed leaf scan (phase 1)
+// ---------------------------------------------------------------------------
+
+// kdtree.c:198-214 with only_leaf_idx = 1: right iff q[axis] - split >= 0.
+template <int DP>
+__global__ __launch_bounds__(256) void k_natural(const float *__restrict__ q, int nq, const int *__restrict__ axis,
+                                                 const float *__restrict__ split, int depth, int *leaf, int *count) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const float *qr = q + (size_t)i * DP;
+    int idx = 0;
+    for (int lvl = 0; lvl < depth; ++lvl) {
+        float diff = qr[axis[idx]] - split[idx];
+        idx = diff >= 0.0f ? 2 * idx + 2 : 2 * idx + 1;
+    }
+    int l = idx - ((1 << depth) - 1);
+    leaf[i] = l;
+    if (count) atomicAdd(&count[l], 1);
+}
+
+hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    return with_dp(t.dpad, [&](auto dp) {
+        constexpr int DP = decltype(dp)::value;
+        hipLaunchKernelGGL(k_natural<DP>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, q, nq, t.axis, t.split,
+                           t.depth, leaf, count);
+        return hipGetLastError();
+    });
+}
+
+// Exclusive scan of n counts into offset[0..n] with one 1024-thread block.
+__global__ __launch_bounds__(1024) void k_scan_single(const int *count, int n, int *offset, int *cursor) {
+    __shared__ int wsum[1024 / kWave];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int per = (n + blockDim.x - 1) / blockDim.x;
+    const int b = threadIdx.x * per;
+    int loc = 0;
+    for (int k = 0; k < per && b + k < n; ++k) loc += count[b + k];
+    int inc = loc;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int t = __shfl_up(inc, o, kWave);
+        if (lane >= o) inc += t;
+    }
+    if (lane == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wid; ++w) off += wsum[w];
+    int run = inc - loc + off;
+    for (int k = 0; k < per && b + k < n; ++k) {
+        offset[b + k] = run;
+        cursor[b + k] = run;
+        run += count[b + k];
+    }
+    if (threadIdx.x == blockDim.x - 1) offset[n] = run;
+}
+
+__global__ void k_bucket(const int *leaf, int nq, int *cursor, int *order) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    int p = atomicAdd(&cursor[leaf[i]], 1);
+    order[p] = i;
+}
+
+hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset, int *cursor, int *order,
+                          hipStream_t st) {
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, offset, cursor);
+    if (nq > 0) hipLaunchKernelGGL(k_bucket, dim3(blocks_for(nq, 256)), dim3(256), 0, st, leaf, nq, cursor, order);
+    return hipGetLastError();
+}
+
+// Sorted k-list of runtime length K <= KM held in registers.  The K live
+// entries occupy the top slots KM-K..KM-1; the front slots hold -inf, which
+// never moves, so the K-th entry is always slot KM-1 and every index is
+// static.  Insertion is the reference's rule (kdtree.c:502-532): reject if
+// list[K-1] <= dist, else the entry lands after every entry <= dist.
+template <int KM>
+__device__ __forceinline__ void list_init(float (&ld)[KM], int (&lp)[KM], int K) {
+#pragma unroll
+    for (int s = 0; s < KM; ++s) {
+        ld[s] = (s < KM - K) ? -INFINITY : kFltMax;
+        lp[s] = 0;
+    }
+}
+
+template <int KM>
+__device__ __forceinline__ void list_insert(float (&ld)[KM], int (&lp)[KM], float dd, int pos) {
+    int r = 0;
+#pragma unroll
+    for (int s = 0; s < KM - 1; ++s) r += (ld[s] <= dd) ? 1 : 0;
+#pragma unroll
+    for (int s = KM - 1; s >= 1; --s) {
+        if (s > r) { ld[s] = ld[s - 1]; lp[s] = lp[s - 1]; }
+    }
+#pragma unroll
+    for (int s = 0; s < KM; ++s) {
+        if (s == r) { ld[s] = dd; lp[s] = pos; }
+    }
+}
+
+// One block per leaf: the leaf's points go to LDS, each thread scans them
+// for one query of the leaf's bucket (kdtree.c:286-303, natural leaf only).
+template <int DP, int KM>
+__global__ __launch_bounds__(256) void k_leafscan(TreeView t, const float *__restrict__ q, const int *__restrict__ order,
+                                                  const int *__restrict__ offset, int K, const int *__restrict__ propleaf,
+                                                  float *__restrict__ od, int *__restrict__ op, int *__restrict__ ol) {
+    extern __shared__ float4 sp4[];
+    const int leaf = blockIdx.x;
+    const int qb = offset[leaf], qe = offset[leaf + 1];
+    if (qb == qe) return;
+    const int from = t.leaves[2 * leaf], np = t.leaves[2 * leaf + 1] - from;
+    const float4 *src = reinterpret_cast<const float4 *>(t.pts + (size_t)from * DP);
+    for (int c = threadIdx.x; c < np * (DP / 4); c += blockDim.x) sp4[c] = src[c];
+    __syncthreads();
+    const float *sp = reinterpret_cast<const float *>(sp4);
+    for (int qi = qb + threadIdx.x; qi < qe; qi += blockDim.x) {
+        const int qidx = order[qi];
+        float qv[DP];
+        load_vec<DP>(q + (size_t)qidx * DP, qv);
+        float ld[KM];
+        int lp[KM];
+        list_init<KM>(ld, lp, K);
+        for (int i = 0; i < np; ++i) {
+            float dd = dist_sq<DP>(sp + i * DP, qv);
+            if (ld[KM - 1] <= dd) continue;
+            list_insert<KM>(ld, lp, dd, from + i);
+        }
+        const size_t ob = (size_t)qidx * K - (size_t)(KM - K);
+#pragma unroll
+        for (int s = 0; s < KM; ++s) {
+            if (s >= KM - K) {
+                od[ob + s] = ld[s];
+                op[ob + s] = lp[s];
+                if (ol) ol[ob + s] = propleaf ? propleaf[lp[s]] : 0;
+            }
+        }
+    }
+}
+
+hipError_t leaf_scan(const TreeView &t, const float *q, const int *order, const int *offset, int K,
+                     const int *propleaf, float *out_d, int *out_p, int *out_l, hipStream_t st) {
+    const int nleaves = 1 << t.depth;
+    const size_t lds = (size_t)t.max_leaf * t.dpad * sizeof(float);
+    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
+        constexpr int DP = decltype(dp)::value;
+        constexpr int KM = decltype(km)::value;
+        hipLaunchKernelGGL((k_leafscan<DP, KM>), dim3(nleaves), dim3(256), lds, st, t, q, order, offset, K, propleaf,
+                           out_d, out_p, out_l);
+        return hipGetLastError();
+    });
+}
+
+// ---------------------------------------------------------------------------
+// wavefront k-selection
+// ---------------------------------------------------------------------------
+
+// Per-lane sorted list of (key, pos, propagated leaf), key = distance bits
+// << 32 | arrival, arrivals numbered from 1.  Keys are unique, so ordering by
+// key is the reference's (distance, arrival) order.  As in list_insert, the
+// live entries sit in the top K slots and the front slots hold key 0, below
+// every real key, so that no slot index depends on K.
+template <int KM>
+struct LaneList {
+    uint64_t key[KM];
+    int pos[KM];
+    int pl[KM];
+
+    __device__ __forceinline__ void reset(int K) {
+#pragma unroll
+        for (int s = 0; s < KM; ++s) {
+            key[s] = (s < KM - K) ? 0ull : ~0ull;
+            pos[s] = 0;
+            pl[s] = 0;
+        }
+    }
+    __device__ __forceinline__ void insert(uint64_t k, int p, int l) {
+        if (k >= key[KM - 1]) return;
+        int r = 0;
+#pragma unroll
+        for (int s = 0; s < KM - 1; ++s) r += (key[s] < k) ? 1 : 0;
+#pragma unroll
+        for (int s = KM - 1; s >= 1; --s) {
+            if (s > r) { key[s] = key[s - 1]; pos[s] = pos[s - 1]; pl[s] = pl[s - 1]; }
+        }
+#pragma unroll
+        for (int s = 0; s < KM; ++s) {
+            if (s == r) { key[s] = k; pos[s] = p; pl[s] = l; }
+        }
+    }
+    // smallest live key (~0 when empty)
+    __device__ __forceinline__ uint64_t head() const {
+        uint64_t h = ~0ull;
+#pragma unroll
+        for (int s = 0; s < KM; ++s) h = (key[s] != 0ull && key[s] < h) ? key[s] : h;
+        return h;
+    }
+    // payload of the entry holding key g
+    __device__ __forceinline__ void payload(uint64_t g, int &p, int &l) const {
+        p = 0;
+        l = 0;
+#pragma unroll
+        for (int s = 0; s < KM; ++s) {
+            if (key[s] == g) { p = pos[s]; l = pl[s]; }
+        }
+    }
+    // removes the smallest live entry (slot KM-K)
+    __device__ __forceinline__ void pop(int K) {
+#pragma unroll
+        for (int s = 0; s < KM - 1; ++s) {
+            if (s >= KM - K) { key[s] = key[s + 1]; pos[s] = pos[s + 1]; pl[s] = pl[s + 1]; }
+        }
+        key[KM - 1] = ~0ull;
+    }
+};
+
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t seq) {
+    return ((uint64_t)__float_as_uint(d) << 32) | seq;
+}
+
+// K rounds of wave-wide minimum; lane r receives the r-th smallest entry.
+template <int KM>
+__device__ __forceinline__ void wave_select(LaneList<KM> &L, int K, int lane, float &od, int &opos, int &opl) {
+    for (int r = 0; r < K; ++r) {
+        const uint64_t mine = L.head();
+        uint64_t g = mine;
+#pragma unroll
+        for (int m = kWave / 2; m >= 1; m >>= 1) {
+            uint64_t o = shfl_xor_u64(g, m);
+            g = o < g ? o : g;
+        }
+        const unsigned long long b = __ballot(mine == g);
+        const int owner = __ffsll(b) - 1;
+        int mp = 0, ml = 0;
+        L.payload(g, mp, ml);
+        const int wp = readlane_i(mp, owner);
+        const int wl = readlane_i(ml, owner);
+        if (lane == owner) L.pop(K);
+        if (lane == r) {
+            od = __uint_as_float((uint32_t)(g >> 32));
+            opos = wp;
+            opl = wl;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exact k-NN search (row 0 and neighbors_extern), one wavefront per query
+// ---------------------------------------------------------------------------
+
+// kdtree.c:129-268: depth-first, near child first, far child iff
+// (q[axis]-split)^2 <= current K-th distance; at most max_leaves leaf visits.
+// phase bits per level: 0 = fresh, 1 = near child done, 2 = far child done.
+template <int DP, int KM>
+__global__ __launch_bounds__(256) void k_exact(TreeView t, const float *__restrict__ q, int nq, int K,
+                                               long long max_leaves, const int *__restrict__ propleaf,
+                                               float *__restrict__ od, int *__restrict__ op, int *__restrict__ ol) {
+    const int w = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    if (w >= nq) return;
+    float qv[DP];
+    load_vec<DP>(q + (size_t)w * DP, qv);
+    const float qlane = lane < DP ? q[(size_t)w * DP + lane] : 0.0f;
+    float my_d = kFltMax;
+    int my_p = 0;
+    int my_l = propleaf ? propleaf[0] : 0;
+    float kth = kFltMax;
+    const int leaf0 = (1 << t.depth) - 1;
+    int idx = 0, lvl = 0;
+    uint64_t phase = 0;
+    long long visited = 0;
+    while (true) {
+        if (lvl == t.depth) {
+            if (++visited > max_leaves) break;
+            const int leaf = idx - leaf0;
+            const int from = t.leaves[2 * leaf], cnt = t.leaves[2 * leaf + 1] - from;
+            LaneList<KM> L;
+            L.reset(K);
+            if (lane < K) L.insert(make_key(my_d, (uint32_t)(1 + lane)), my_p, my_l);
+            for (int off = lane; off < cnt; off += kWave) {
+                const int pos = from + off;
+                const float dd = dist_sq<DP>(t.pts + (size_t)pos * DP, qv);
+                const int pl = propleaf ? propleaf[pos] : 0;
+                L.insert(make_key(dd, (uint32_t)(1 + K + off)), pos, pl);
+            }
+            float nd = 0.0f;
+            int np = 0, nl = 0;
+            wave_select<KM>(L, K, lane, nd, np, nl);
+            if (lane < K) { my_d = nd; my_p = np; my_l = nl; }
+            kth = readlane_f(my_d, K - 1);
+            if (lvl == 0) break;
+            idx = (idx - 1) >> 1;
+            --lvl;
+            continue;
+        }
+        const int ph = (int)((phase >> (2 * lvl)) & 3u);
+        const float diff = readlane_f(qlane, t.axis[idx]) - t.split[idx];
+        if (ph == 0) {
+            phase |= (uint64_t)1 << (2 * lvl);
+            idx = diff >= 0.0f ? 2 * idx + 2 : 2 * idx + 1;
+            ++lvl;
+        } else if (ph == 1 && diff * diff <= kth) {
+            phase = (phase & ~((uint64_t)3 << (2 * lvl))) | ((uint64_t)2 << (2 * lvl));
+            idx = diff >= 0.0f ? 2 * idx + 1 : 2 * idx + 2;
+            ++lvl;
+        } else {
+            phase &= ~((uint64_t)3 << (2 * lvl));
+            if (lvl == 0) break;
+            idx = (idx - 1) >> 1;
+            --lvl;
+        }
+    }
+    if (lane < K) {
+        od[(size_t)w * K + lane] = my_d;
+        op[(size_t)w * K + lane] = my_p;
+        if (ol) ol[(size_t)w * K + lane] = my_l;
+    }
+}
+
+hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long long max_leaves,
+                        const int *propleaf, float *out_d, int *out_p, int *out_l, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
+        constexpr int DP = decltype(dp)::value;
+        constexpr int KM = decltype(km)::value;
+        hipLaunchKernelGGL((k_exact<DP, KM>), dim3(blocks_for((long long)nq * kWave, 256)), dim3(256), 0, st, t, q,
+                           nq, K, max_leaves, propleaf, out_d, out_p, out_l);
+        return hipGetLastError();
+    });
+}
+
+// ---------------------------------------------------------------------------
+// propagation (rows >= 1)
+// ---------------------------------------------------------------------------
+
+// propleaf[pos] = leaf of the patch one row below the B patch at tree
+// position pos (base.c:265-276): o = orig[pos]; o += n_cols if o + n_cols <
+// n_a; leaf = pli[o].
+__global__ void k_propleaf(const int *orig, const int *pli, int n_b, int n_a, int n_cols, int *out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_b) return;
+    int o = orig[i];
+    if (o + n_cols < n_a) o += n_cols;
+    out[i] = pli[o];
+}
+
+hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hipStream_t st) {
+    hipLaunchKernelGGL(k_propleaf, dim3(blocks_for(t.n, 256)), dim3(256), 0, st, t.orig, t.pli, t.n, n_a, n_cols,
+                       propleaf);
+    return hipGetLastError();
+}
+
+// One wavefront per image column, walking rows 1..n_rows-1.  The list of
+// row y starts from the natural-leaf list (arrivals 0..K-1) and then takes
+// the candidates of the K propagated leaves in order (base.c:255-304).
+template <int DP, int KM>
+__global__ __launch_bounds__(256) void k_sweep(TreeView t, const float *__restrict__ q, int n_rows, int n_cols, int K,
+                                               const int *__restrict__ propleaf, const float *__restrict__ p1d,
+                                               const int *__restrict__ p1p, const int *__restrict__ p1l,
+                                               int *__restrict__ fl_pos, const int *__restrict__ row0_l, int stride) {
+    const int x = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave));
+    const int lane = threadIdx.x & (kWave - 1);
+    if (x >= n_cols) return;
+    int my_l = (lane < K) ? row0_l[(size_t)x * K + lane] : 0;
+    for (int y = 1; y < n_rows; ++y) {
+        const size_t qi = (size_t)y * n_cols + x;
+        float qv[DP];
+        load_vec<DP>(q + qi * DP, qv);
+        LaneList<KM> L;
+        L.reset(K);
+        if (lane < K)
+            L.insert(make_key(p1d[qi * K + lane], (uint32_t)(1 + lane)), p1p[qi * K + lane], p1l[qi * K + lane]);
+        for (int k = 0; k < K; ++k) {
+            const int leaf = readlane_i(my_l, k);
+            const int from = t.leaves[2 * leaf], cnt = t.leaves[2 * leaf + 1] - from;
+            const uint32_t seq0 = (uint32_t)(1 + K + k * stride);
+            for (int off = lane; off < cnt; off += kWave) {
+                const int pos = from + off;
+                const float dd = dist_sq<DP>(t.pts + (size_t)pos * DP, qv);
+                L.insert(make_key(dd, seq0 + (uint32_t)off), pos, propleaf[pos]);
+            }
+        }
+        float nd = 0.0f;
+        int np = 0, nl = 0;
+        wave_select<KM>(L, K, lane, nd, np, nl);
+        if (lane < K) {
+            fl_pos[qi * K + lane] = np;
+            my_l = nl;
+        }
+    }
+}
+
+hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K, const int *propleaf,
+                     const float *p1d, const int *p1p, const int *p1l, int *fl_pos, const int *row0_l,
+                     hipStream_t st) {
+    if (n_rows < 2) return hipSuccess;
+    return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
+        constexpr int DP = decltype(dp)::value;
+        constexpr int KM = decltype(km)::value;
+        hipLaunchKernelGGL((k_sweep<DP, KM>), dim3(blocks_for((long long)n_cols * kWave, 256)), dim3(256), 0, st, t,
+                           q, n_rows, n_cols, K, propleaf, p1d, p1p, p1l, fl_pos, row0_l, t.max_leaf);
+        return hipGetLastError();
+    });
+}
+
+// ---------------------------------------------------------------------------
+// index conversion + best-of-K (base.c:309-353, norm_array base.c:362-374)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ const uint8_t *patch_ptr(const PatchGeom &g, long long i) {
+    if (g.image) {
+        long long y = i / g.ncols, x = i - y * g.ncols;
+        return g.base + (y * g.width + x) * 3;
+    }
+    return g.base + i * (long long)g.rowlen;
+}
+
+// sqrt of the sequential FP32 sum of ((float)a - (float)b)^2.
+__device__ __forceinline__ float patch_l2(const PatchGeom &ga, long long i, const PatchGeom &gb, long long j) {
+    const uint8_t *a = patch_ptr(ga, i);
+    const uint8_t *b = patch_ptr(gb, j);
+    float s = 0.0f;
+    for (int r = 0; r < ga.rows; ++r) {
+        const uint8_t *ra = a + r * ga.rstride;
+        const uint8_t *rb = b + r * gb.rstride;
+        for (int u = 0; u < ga.rowlen; ++u) {
+            float df = (float)ra[u] - (float)rb[u];
+            s = s + df * df;
+        }
+    }
+    return __fsqrt_rn(s);
+}
+
+__global__ __launch_bounds__(256) void k_best(PatchGeom ga, PatchGeom gb, const int *__restrict__ fl_pos,
+                                              const int *__restrict__ orig, int nq, int K, int sel,
+                                              int *__restrict__ fin, float *__restrict__ fdist, int *__restrict__ knn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int *lp = fl_pos + (size_t)i * K;
+    if (knn) {
+        for (int k = 0; k < K; ++k) knn[(size_t)i * K + k] = orig[lp[k]];
+    }
+    if (!sel) {
+        const int j = orig[lp[0]];
+        fin[i] = j;
+        if (fdist) fdist[i] = patch_l2(ga, i, gb, j);
+        return;
+    }
+    int best = -1;
+    float bd = kFltMax;
+    for (int k = 0; k < K; ++k) {
+        const int j = orig[lp[k]];
+        const float dd = patch_l2(ga, i, gb, j);
+        if (dd < bd) { bd = dd; best = j; }
+    }
+    fin[i] = best;
+    if (fdist) fdist[i] = bd;
+}
+
+hipError_t best_of_k(const PatchGeom &ga, const PatchGeom &gb, const int *fl_pos, const int *orig, int nq, int K,
+                     int select_best, int *final_out, float *dist_out, int *knn_out, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_best, dim3(blocks_for(nq, 256)), dim3(256), 0, st, ga, gb, fl_pos, orig, nq, K, select_best,
+                       final_out, dist_out, knn_out);
+    return hipGetLastError();
+}
+
+// neighbors_extern's conversion (base.c:117-130): original index, sqrt.
+__global__ void k_convert(const int *pos, const float *d2, const int *orig, long long count, int *idx, float *dist) {
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    idx[i] = orig[pos[i]];
+    dist[i] = __fsqrt_rn(d2[i]);
+}
+
+hipError_t convert_knn(const int *pos, const float *d2, const int *orig, long long count, int *idx_out,
+                       float *dist_out, hipStream_t st) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_convert, dim3(blocks_for(count, 256)), dim3(256), 0, st, pos, d2, orig, count, idx_out,
+                       dist_out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// patch extraction fused with the PCA projection
+// ---------------------------------------------------------------------------
+
+// out[i][k] = sum_f (x_f - mean_f) * basis[k][f] for the p x p x 3 window of
+// patch i (f = (r*p + c)*3 + ch, openmp/base.py:152-166), zero-padded to DP.
+// Acc = float: FP32 accumulation; Acc = double: FP64, rounded once to FP32.
+template <int DP, typename Acc>
+__global__ __launch_bounds__(256) void k_project(const uint8_t *__restrict__ img, int h, int w, int p,
+                                                 const Acc *__restrict__ basis_t, const Acc *__restrict__ mean, int d,
+                                                 float *__restrict__ out) {
+    extern __shared__ double smem_d[];
+    Acc *sb = reinterpret_cast<Acc *>(smem_d);
+    const int D = p * p * 3;
+    Acc *sm = sb + (size_t)D * DP;
+    for (int c = threadIdx.x; c < D * DP; c += blockDim.x) sb[c] = basis_t[c];
+    for (int c = threadIdx.x; c < D; c += blockDim.x) sm[c] = mean[c];
+    __syncthreads();
+    const int nr = h - p + 1, nc = w - p + 1;
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)nr * nc) return;
+    const int y = (int)(i / nc), x = (int)(i - (long long)y * nc);
+    Acc acc[DP];
+#pragma unroll
+    for (int j = 0; j < DP; ++j) acc[j] = 0;
+    const int rl = 3 * p;
+    for (int r = 0; r < p; ++r) {
+        const uint8_t *row = img + ((size_t)(y + r) * w + x) * 3;
+        for (int u = 0; u < rl; ++u) {
+            const int f = r * rl + u;
+            const Acc v = (Acc)row[u] - sm[f];
+            const Acc *bf = sb + (size_t)f * DP;
+#pragma unroll
+            for (int j = 0; j < DP; ++j) acc[j] = fma(v, bf[j], acc[j]);
+        }
+    }
+    float4 *o4 = reinterpret_cast<float4 *>(out + (size_t)i * DP);
+#pragma unroll
+    for (int c = 0; c < DP / 4; ++c) {
+        float4 v;
+        v.x = (4 * c + 0 < d) ? (float)acc[4 * c + 0] : 0.0f;
+        v.y = (4 * c + 1 < d) ? (float)acc[4 * c + 1] : 0.0f;
+        v.z = (4 * c + 2 < d) ? (float)acc[4 * c + 2] : 0.0f;
+        v.w = (4 * c + 3 < d) ? (float)acc[4 * c + 3] : 0.0f;
+        o4[c] = v;
+    }
+}
+
+hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float *basis_t, const float *mean_f,
+                           const double *basis_t64, const double *mean64, int d, int dpad, int fp64, float *out,
+                           hipStream_t st) {
+    const int D = p * p * 3;
+    const long long n = (long long)(h - p + 1) * (w - p + 1);
+    if (n <= 0) return hipSuccess;
+    return with_dp(dpad, [&](auto dp) {
+        constexpr int DP = decltype(dp)::value;
+        if (fp64) {
+            const size_t lds = (size_t)D * (DP + 1) * sizeof(double);
+            if (lds > 160 * 1024) return hipErrorInvalidValue;
+            hipError_t e = hipFuncSetAttribute((const void *)k_project<DP, double>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_project<DP, double>), dim3(blocks_for(n, 256)), dim3(256), lds, st, img, h, w, p,
+                               basis_t64, mean64, d, out);
+        } else {
+            const size_t lds = (size_t)D * (DP + 1) * sizeof(float);
+            if (lds > 160 * 1024) return hipErrorInvalidValue;
+            hipError_t e = hipFuncSetAttribute((const void *)k_project<DP, float>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_project<DP, float>), dim3(blocks_for(n, 256)), dim3(256), lds, st, img, h, w, p,
+                               basis_t, mean_f, d, out);
+        }
+        return hipGetLastError();
+    });
+}
+
+}  // namespace annf

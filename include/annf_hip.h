@@ -1,0 +1,139 @@
+/*
+ * annf_hip.h -- C ABI of the MI355X (gfx950) propagation-assisted k-d-tree
+ * nearest-neighbour-field library (annfmp_amd/libannf_hip.so).
+ *
+ * Each entry point replaces one function of the reference's openmp C core
+ * (annfmp/openmp/kdtree/src/include/base.h, bound to Python through SWIG in
+ * annfmp/openmp/kdtree/swig/cpu_float.i).  The argument order and meaning
+ * follow the reference; the conventions differ where the reference's cannot
+ * serve a GPU library:
+ *   - state lives in an opaque handle (annf_ctx) instead of the caller-owned
+ *     KD_TREE_PARAMETERS / KD_TREE_RECORD structs;
+ *   - every call returns an int status (ANNF_OK or a negative ANNF_E* code,
+ *     text in annf_last_error()) instead of printf + exit();
+ *   - "host" entry points take host buffers and copy them; "_dev" entry
+ *     points take device pointers (HBM-resident data, no copies);
+ *   - every call runs on the handle's own HIP stream and returns after that
+ *     stream has finished, unless its name ends in _async;
+ *   - floating-point types are fixed to float (the reference's USE_DOUBLE=0
+ *     build, the only one its Python front ever selects).
+ *
+ * Threading: one handle must not be used from two threads at once; distinct
+ * handles are independent.
+ */
+#ifndef ANNF_HIP_H
+#define ANNF_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ANNF_OK 0
+#define ANNF_EINVAL (-1)       /* bad argument (the reference exits here)  */
+#define ANNF_ENOTREE (-2)      /* no tree fitted on this handle            */
+#define ANNF_EHIP (-3)         /* HIP runtime error                        */
+#define ANNF_ENOMEM (-4)       /* device allocation failed                 */
+#define ANNF_EUNSUPPORTED (-5) /* outside this build's limits (K, d)       */
+
+#define ANNF_SPLIT_CYCLIC 0    /* global.h:14 SPLITTING_TYPE_CYCLIC         */
+#define ANNF_SPLIT_LONGEST 1   /* global.h:15 SPLITTING_TYPE_LONGEST_BOX    */
+
+/* Build limits: n_neighbors <= ANNF_MAX_K, reduced dimension <= ANNF_MAX_D. */
+#define ANNF_MAX_K 32
+#define ANNF_MAX_D 32
+
+typedef struct annf_ctx annf_ctx;
+
+/* ---- handle management (no reference counterpart: the reference keeps
+ *      global state and never frees it, base.c:379-381) ---------------- */
+int annf_create(annf_ctx **out, int device);
+int annf_destroy(annf_ctx *ctx);
+const char *annf_last_error(void);
+/* The handle's HIP stream, as a hipStream_t cast to void*. */
+void *annf_stream(annf_ctx *ctx);
+
+/* ---- init_extern (base.h:36-42) --------------------------------------
+ * Validates and stores the parameters (check_parameters, helper.c:30-52).
+ * num_threads is accepted for signature parity and ignored. */
+int annf_init(annf_ctx *ctx, int n_neighbors, int tree_depth, int max_leaves,
+              int num_threads, int splitting_type, int verbosity_level);
+
+/* ---- fit_extern (base.h:53-57) ---------------------------------------
+ * Builds the tree over Xtrain (nXtrain x dXtrain, row-major float32) at the
+ * depth given to annf_init.  The tree stays on the device. */
+int annf_fit(annf_ctx *ctx, const float *Xtrain, int nXtrain, int dXtrain);
+int annf_fit_dev(annf_ctx *ctx, const float *Xtrain_dev, int nXtrain, int dXtrain);
+
+/* ---- neighbors_extern (base.h:76-86) ---------------------------------
+ * Exact k nearest neighbours (backtracking search, at most max_leaves leaf
+ * visits) of nXtest queries; K = ddistances = dindices.  Writes sqrt
+ * distances and original training indices. */
+int annf_neighbors(annf_ctx *ctx, const float *Xtest, int nXtest, int dXtest,
+                   float *distances, int ndistances, int ddistances,
+                   int *indices, int nindices, int dindices);
+
+/* ---- process_rows_extern (base.h:89-107) -----------------------------
+ * Row 0: exact search; rows >= 1: natural leaf + (propagate) the leaves of
+ * the patches one row below the K matches of the patch above; then index
+ * conversion and (select_best_nn) best-of-K by full-patch L2 on patchesA /
+ * patchesB.  patchesA/patchesB are only read when select_best_nn != 0. */
+int annf_process_rows(annf_ctx *ctx, int *finalindices, int nfinalindices,
+                      const uint8_t *patchesA, int npatchesA, int dpatchesA,
+                      const uint8_t *patchesB, int npatchesB, int dpatchesB,
+                      const float *patchesred, int npatchesred, int dpatchesred,
+                      int n_rows, int n_cols, int n_neighbors, int propagate,
+                      int select_best_nn);
+
+/* ---- norm_array (base.h:109) -----------------------------------------
+ * Host helper with the reference's arithmetic: sqrt of the sequential FP32
+ * sum of squared differences of two uint8 vectors. */
+float annf_norm_array(const uint8_t *patchA, const uint8_t *patchB, int dim);
+
+/* ---- free_resources_extern (base.h:115) ------------------------------
+ * Releases the tree and all device workspaces of the handle. */
+int annf_free_resources(annf_ctx *ctx);
+
+/* ---- tree export (no reference counterpart; the reference's arrays are
+ *      reachable through its KD_TREE_RECORD, global.h:38-50) -------------
+ * Any output pointer may be NULL.  Sizes: axis/split 2^depth-1, leaves
+ * 2*2^depth, xi_orig n, xi_pts n*d, pli n. */
+int annf_tree_info(annf_ctx *ctx, int *n, int *d, int *depth);
+int annf_tree_export(annf_ctx *ctx, int *axis, float *split, int *leaves,
+                     int *xi_orig, float *xi_pts, int *pli);
+
+/* ---- image-level fused field (the openmp backend's fit,
+ *      annfmp/openmp/base.py:53-150, from projection onwards) -----------
+ * All pointers are device pointers.  img_a is ha x wa x 3, img_b is
+ * hb x wb x 3 (uint8, row-major).  basis is d x (p*p*3) float32 (PCA
+ * components, sklearn's components_ layout), mean is p*p*3 float64.
+ * Patches are projected as sum_f (x_f - mean_f) * basis[k][f], accumulated
+ * in FP32 (proj_fp64 = 0) or FP64 (proj_fp64 = 1, then rounded to FP32).
+ * The tree over B's projected patches is built at depth tree_depth.
+ * Outputs: nn_out (n_a int32, original B indices), optionally dist_out
+ * (n_a float32: full-patch L2 of the chosen match), knn_out (n_a x K int32
+ * original indices before best-of-K), red_a_out / red_b_out (n x d
+ * float32 projected patches).  Any optional output may be NULL. */
+int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
+                   const uint8_t *img_b, int hb, int wb, int psize,
+                   const float *basis, const double *mean, int d,
+                   int proj_fp64, int tree_depth, int n_neighbors,
+                   int propagate, int select_best_nn, int *nn_out,
+                   float *dist_out, int *knn_out, float *red_a_out,
+                   float *red_b_out);
+
+/* Per-stage device times (ms) of the last annf_field_dev /
+ * annf_process_rows call, measured with HIP events on the handle's stream.
+ * Stage order: 0 project A, 1 project B, 2 tree build, 3 natural leaves +
+ * leaf scan, 4 row 0 exact search, 5 row propagation sweep, 6 conversion +
+ * best-of-K.  Returns the number of stages written (<= max_stages). */
+int annf_stage_times(annf_ctx *ctx, float *ms, int max_stages);
+/* Enables (1) or disables (0) the per-stage events. */
+int annf_set_timing(annf_ctx *ctx, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ANNF_HIP_H */

@@ -1,0 +1,395 @@
+"""CPU oracle for the propagation-assisted k-d-tree NNF path.
+
+TEST INFRASTRUCTURE ONLY: this module is the checker.  Only ``tests/``,
+``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg may import
+it; the product package ``annfmp_amd`` never does.
+
+Two CPU implementations are exposed through ctypes:
+
+* ``liboracle.so`` -- this repository's C restatement (``annf_oracle.c``) of the
+  reference's openmp semantics (annfmp/openmp/kdtree/src/*.c);
+* ``_ref/libannf_ref.so`` -- the reference's own openmp C core, compiled from
+  its sources by ``oracle/Makefile`` where ``/root/reference`` exists (used to
+  pin the restatement and to write the golden fixtures; absent on machines
+  that only receive this repository).
+
+Plus numpy restatements of the Python-side stages:
+
+* ``tree_depth``       -- annfmp/openmp/kdtree/base.py:260-300
+* ``extract_patches``  -- annfmp/openmp/base.py:152-166 (sklearn extract_patches_2d order)
+* ``project_f64``      -- annfmp/openmp/base.py:193-200 (sklearn PCA.transform in FP64, cast to f32)
+* ``exact_field``      -- brute-force k-NN over full uint8 patches with exact
+                          integer squared distances (equal to FP64: every
+                          value is an integer below 2**25)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import warnings
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libannf_ref.so")
+
+_fp = C.POINTER(C.c_float)
+_ip = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+# --------------------------------------------------------------------------
+# numpy restatements
+# --------------------------------------------------------------------------
+
+def tree_depth(n, leaf_size, tree_depth=None):
+    """annfmp/openmp/kdtree/base.py:260-300 (math.log float semantics kept)."""
+    d = int(math.floor(math.log(n, 2)))
+    if tree_depth is not None:
+        if tree_depth > d:
+            warnings.warn("tree_depth %i too large (using smaller depth): %i.\n" % (tree_depth, d))
+            return d
+        return tree_depth
+    if not leaf_size > 1:
+        raise Exception("Parameter 'leaf_size' must be larger than 1")
+    subtr = int(math.floor(math.log(leaf_size, 2)))
+    if d - subtr < 0:
+        warnings.warn("tree_depth smaller than 0; setting tree_depth to 0")
+        return 0
+    return d - subtr
+
+
+def extract_patches(img, p):
+    """All p x p windows, row-major over the top-left corner, feature
+    f = (i*p + j)*3 + ch  (openmp/base.py:152-166)."""
+    img = np.ascontiguousarray(img)
+    H, W, ch = img.shape
+    nr, nc = H - p + 1, W - p + 1
+    v = np.lib.stride_tricks.as_strided(
+        img, shape=(nr, nc, p, p, ch),
+        strides=(img.strides[0], img.strides[1], img.strides[0], img.strides[1], img.strides[2]))
+    return np.ascontiguousarray(v).reshape(nr * nc, p * p * ch)
+
+
+def project_f64(patches, components, mean):
+    """sklearn PCA.transform (X @ C.T - mean @ C.T in FP64) then astype(f32)
+    (openmp/base.py:195)."""
+    comp = np.asarray(components, dtype=np.float64)
+    out = np.empty((patches.shape[0], comp.shape[0]), dtype=np.float32)
+    off = np.asarray(mean, dtype=np.float64) @ comp.T
+    step = 1 << 16
+    for s in range(0, patches.shape[0], step):
+        x = patches[s:s + step].astype(np.float64)
+        out[s:s + step] = (x @ comp.T - off).astype(np.float32)
+    return out
+
+
+def exact_field(pa, pb, k, chunk=2048):
+    """Exact k nearest B patches of every A patch by squared L2 on uint8
+    values.  Integer arithmetic (int64), ties broken by lower index.
+    Returns (idx [n_a, k] int64, sqdist [n_a, k] int64)."""
+    a = pa.astype(np.int64)
+    b = pb.astype(np.int64)
+    bn = (b * b).sum(1)
+    bt = b.T.copy()
+    idx = np.empty((a.shape[0], k), np.int64)
+    dst = np.empty((a.shape[0], k), np.int64)
+    for s in range(0, a.shape[0], chunk):
+        x = a[s:s + chunk]
+        d2 = (x * x).sum(1)[:, None] + bn[None, :] - 2 * (x @ bt)
+        part = np.argpartition(d2, kth=min(k, d2.shape[1] - 1) - 1 if k < d2.shape[1] else d2.shape[1] - 1, axis=1)[:, :k]
+        # exact tie order: sort candidates by (distance, index) among all
+        # entries not greater than the k-th distance
+        kth = np.take_along_axis(d2, part, 1).max(1)
+        for r in range(x.shape[0]):
+            cand = np.nonzero(d2[r] <= kth[r])[0]
+            order = np.lexsort((cand, d2[r, cand]))[:k]
+            idx[s + r] = cand[order]
+            dst[s + r] = d2[r, cand[order]]
+    return idx, dst
+
+
+# --------------------------------------------------------------------------
+# the C restatement (liboracle.so)
+# --------------------------------------------------------------------------
+
+class _OrcTree(C.Structure):
+    _fields_ = [("n", C.c_int), ("d", C.c_int), ("depth", C.c_int),
+                ("pts", _fp), ("orig", _ip), ("axis", _ip), ("split", _fp),
+                ("leaves", _ip), ("pli", _ip)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            raise RuntimeError("oracle not built: run `make -C oracle oracle`")
+        L = C.CDLL(ORACLE_SO)
+        L.orc_build_tree.argtypes = [_fp, C.c_int, C.c_int, C.c_int, C.POINTER(_OrcTree)]
+        L.orc_build_tree.restype = C.c_int
+        L.orc_natural_leaf.argtypes = [C.POINTER(_OrcTree), _fp]
+        L.orc_natural_leaf.restype = C.c_int
+        L.orc_exact_knn.argtypes = [C.POINTER(_OrcTree), _fp, C.c_int, C.c_long, _fp, _ip]
+        L.orc_scan_leaves.argtypes = [C.POINTER(_OrcTree), _fp, _ip, C.c_int, C.c_int, _fp, _ip]
+        L.orc_neighbors.argtypes = [C.POINTER(_OrcTree), _fp, C.c_int, C.c_int, C.c_long, _fp, _ip, C.c_int]
+        L.orc_neighbors.restype = C.c_int
+        L.orc_norm_array.argtypes = [_u8p, _u8p, C.c_int]
+        L.orc_norm_array.restype = C.c_float
+        L.orc_process_rows.argtypes = [C.POINTER(_OrcTree), _u8p, _u8p, C.c_int, _fp, C.c_int,
+                                       C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _ip, _ip, _fp, C.c_int]
+        L.orc_process_rows.restype = C.c_int
+        L.orc_rows_range.argtypes = [C.POINTER(_OrcTree), _fp, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_int, C.c_int, _ip, _fp, C.c_int]
+        L.orc_rows_range.restype = C.c_int
+        L.orc_best_of_k_images.argtypes = [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
+                                           _ip, _fp, C.c_int]
+        L.orc_best_of_k_images.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+class OracleTree:
+    """A k-d tree held in numpy arrays (tree positions = XI order)."""
+
+    def __init__(self, X, depth):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        n, d = X.shape
+        self.n, self.d, self.depth = n, d, depth
+        self.pts = np.empty((n, d), np.float32)
+        self.orig = np.empty(n, np.int32)
+        self.axis = np.zeros(max((1 << depth) - 1, 1), np.int32)
+        self.split = np.zeros(max((1 << depth) - 1, 1), np.float32)
+        self.leaves = np.empty(2 << depth, np.int32)
+        self.pli = np.empty(n, np.int32)
+        self._s = _OrcTree(n, d, depth, _ptr(self.pts, _fp), _ptr(self.orig, _ip), _ptr(self.axis, _ip),
+                           _ptr(self.split, _fp), _ptr(self.leaves, _ip), _ptr(self.pli, _ip))
+        rc = lib().orc_build_tree(_ptr(X, _fp), n, d, depth, C.byref(self._s))
+        if rc != 0:
+            raise ValueError("orc_build_tree failed")
+        self.n_nodes = (1 << depth) - 1
+        self.axis = self.axis[:self.n_nodes]
+        self.split = self.split[:self.n_nodes]
+
+    def natural_leaf(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        return lib().orc_natural_leaf(C.byref(self._s), _ptr(q, _fp))
+
+    def natural_leaves(self, Q):
+        return np.array([self.natural_leaf(q) for q in np.asarray(Q, np.float32)], np.int32)
+
+    def exact_knn(self, q, K, max_leaves=2147483640):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        ld = np.empty(K, np.float32)
+        lp = np.empty(K, np.int32)
+        lib().orc_exact_knn(C.byref(self._s), _ptr(q, _fp), K, max_leaves, _ptr(ld, _fp), _ptr(lp, _ip))
+        return ld, lp
+
+    def scan_leaves(self, q, leaf_list, K):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        ll = np.ascontiguousarray(leaf_list, dtype=np.int32)
+        ld = np.empty(K, np.float32)
+        lp = np.empty(K, np.int32)
+        lib().orc_scan_leaves(C.byref(self._s), _ptr(q, _fp), _ptr(ll, _ip), len(ll), K,
+                              _ptr(ld, _fp), _ptr(lp, _ip))
+        return ld, lp
+
+    def neighbors(self, Q, K, max_leaves=2147483640, nthreads=0):
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        dist = np.empty((Q.shape[0], K), np.float32)
+        idx = np.empty((Q.shape[0], K), np.int32)
+        rc = lib().orc_neighbors(C.byref(self._s), _ptr(Q, _fp), Q.shape[0], K, max_leaves,
+                                 _ptr(dist, _fp), _ptr(idx, _ip), nthreads)
+        if rc != 0:
+            raise ValueError("orc_neighbors failed")
+        return dist, idx
+
+    def process_rows(self, red, n_rows, n_cols, K, propagate=True, select_best_nn=True,
+                     pa=None, pb=None, nthreads=0):
+        """Returns (final [N] int32, lists_pos [N,K] tree positions, lists_dst [N,K])."""
+        red = np.ascontiguousarray(red, dtype=np.float32)
+        N = red.shape[0]
+        if select_best_nn:
+            pa = np.ascontiguousarray(pa, dtype=np.uint8)
+            pb = np.ascontiguousarray(pb, dtype=np.uint8)
+            D = pa.shape[1]
+        else:
+            pa = pb = np.zeros((1, 1), np.uint8)
+            D = 1
+        final = np.empty(N, np.int32)
+        pos = np.empty((N, K), np.int32)
+        dst = np.empty((N, K), np.float32)
+        rc = lib().orc_process_rows(C.byref(self._s), _ptr(pa, _u8p), _ptr(pb, _u8p), D, _ptr(red, _fp), N,
+                                    n_rows, n_cols, K, int(bool(propagate)), int(bool(select_best_nn)),
+                                    _ptr(final, _ip), _ptr(pos, _ip), _ptr(dst, _fp), nthreads)
+        if rc != 0:
+            raise ValueError("orc_process_rows failed")
+        return final, pos, dst
+
+    def rows_range(self, red, n_cols, K, propagate, row_begin, row_end, pos, dst, nthreads=0):
+        red = np.ascontiguousarray(red, dtype=np.float32)
+        return lib().orc_rows_range(C.byref(self._s), _ptr(red, _fp), red.shape[0], n_cols, K,
+                                    int(bool(propagate)), row_begin, row_end, _ptr(pos, _ip), _ptr(dst, _fp),
+                                    nthreads)
+
+
+def norm_array(a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    return lib().orc_norm_array(_ptr(a, _u8p), _ptr(b, _u8p), a.size)
+
+
+def best_of_k_images(img_a, img_b, p, cand, nthreads=0):
+    """Best-of-K over original B indices cand [N, K], reading windows of the
+    two images directly.  Returns (final int32 [N], dist float32 [N])."""
+    img_a = np.ascontiguousarray(img_a, dtype=np.uint8)
+    img_b = np.ascontiguousarray(img_b, dtype=np.uint8)
+    H, W, _ = img_a.shape
+    nr, nc = H - p + 1, W - p + 1
+    cand = np.ascontiguousarray(cand, dtype=np.int32)
+    final = np.empty(nr * nc, np.int32)
+    fd = np.empty(nr * nc, np.float32)
+    lib().orc_best_of_k_images(_ptr(img_a, _u8p), _ptr(img_b, _u8p), W, p, nr, nc, _ptr(cand, _ip),
+                               cand.shape[1], _ptr(final, _ip), _ptr(fd, _fp), nthreads)
+    return final, fd
+
+
+# --------------------------------------------------------------------------
+# the reference's own C core (oracle/_ref/libannf_ref.so), container only
+# --------------------------------------------------------------------------
+
+class _RefParams(C.Structure):          # global.h:18-27
+    _fields_ = [("n_neighbors", C.c_int), ("tree_depth", C.c_int), ("max_leaves", C.c_int),
+                ("num_threads", C.c_int), ("verbosity_level", C.c_int), ("splitting_type", C.c_int)]
+
+
+class _RefNode(C.Structure):            # global.h:30-35
+    _fields_ = [("axis", C.c_int), ("splitting_value", C.c_float)]
+
+
+class _RefRecord(C.Structure):          # global.h:38-50
+    _fields_ = [("Xtrain", _fp), ("nXtrain", C.c_int), ("dXtrain", C.c_int), ("XI", C.c_void_p),
+                ("nodes", C.POINTER(_RefNode)), ("leaves", _ip), ("points_leaf_indices", _ip),
+                ("tree_depth", C.c_int)]
+
+
+_ref = None
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref_lib():
+    global _ref
+    if _ref is None:
+        if not ref_available():
+            raise RuntimeError("reference core not built (oracle/_ref/libannf_ref.so)")
+        L = C.CDLL(REF_SO)
+        L.init_extern.argtypes = [C.c_int] * 6 + [C.POINTER(_RefParams)]
+        L.fit_extern.argtypes = [_fp, C.c_int, C.c_int, C.POINTER(_RefRecord), C.POINTER(_RefParams)]
+        L.neighbors_extern.argtypes = [_fp, C.c_int, C.c_int, _fp, C.c_int, C.c_int, _ip, C.c_int, C.c_int,
+                                       C.POINTER(_RefRecord), C.POINTER(_RefParams)]
+        L.process_rows_extern.argtypes = [_ip, C.c_int, _u8p, C.c_int, C.c_int, _u8p, C.c_int, C.c_int,
+                                          _fp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(_RefRecord), C.POINTER(_RefParams)]
+        L.brute_force_group.argtypes = [_fp, C.c_int, _ip, C.c_int, C.POINTER(_RefRecord), C.POINTER(_RefParams),
+                                        _ip, _fp, C.c_int]
+        L.brute_force_group.restype = C.c_int
+        L.norm_array.argtypes = [_u8p, _u8p, C.c_int]
+        L.norm_array.restype = C.c_float
+        L.kd_tree_query_tree_sequential.argtypes = [_fp, _fp, _ip, C.c_int, C.c_int, C.c_int,
+                                                    C.POINTER(_RefRecord)]
+        L.kd_tree_query_tree_sequential.restype = C.c_int
+        _ref = L
+    return _ref
+
+
+class RefTree:
+    """The reference's own tree (fit_extern), with arrays copied out."""
+
+    def __init__(self, X, depth, K=1, max_leaves=2147483640):
+        L = ref_lib()
+        self.X = np.ascontiguousarray(X, dtype=np.float32)   # borrowed by the record
+        n, d = self.X.shape
+        self.n, self.d, self.depth = n, d, depth
+        self.params = _RefParams()
+        L.init_extern(K, depth, max_leaves, 1, 1, 0, C.byref(self.params))
+        self.rec = _RefRecord()
+        L.fit_extern(_ptr(self.X, _fp), n, d, C.byref(self.rec), C.byref(self.params))
+        nn = (1 << depth) - 1
+        raw = np.ctypeslib.as_array(C.cast(self.rec.XI, C.POINTER(C.c_uint8)), shape=(n * (4 * d + 4),))
+        rec = raw.reshape(n, 4 * d + 4)
+        self.pts = rec[:, :4 * d].copy().view(np.float32).reshape(n, d)
+        self.orig = rec[:, 4 * d:].copy().view(np.int32).reshape(n)
+        if nn > 0:
+            nodes = np.ctypeslib.as_array(C.cast(self.rec.nodes, C.POINTER(C.c_int32)), shape=(2 * nn,))
+            nodes = nodes.reshape(nn, 2).copy()
+            self.axis = nodes[:, 0].copy()
+            self.split = nodes[:, 1].copy().view(np.float32)
+        else:
+            self.axis = np.zeros(0, np.int32)
+            self.split = np.zeros(0, np.float32)
+        self.leaves = np.ctypeslib.as_array(self.rec.leaves, shape=(2 << depth,)).copy()
+        self.pli = np.ctypeslib.as_array(self.rec.points_leaf_indices, shape=(n,)).copy()
+
+    def set_k(self, K):
+        self.params.n_neighbors = K
+
+    def neighbors(self, Q, K):
+        self.set_k(K)
+        Q = np.ascontiguousarray(Q, dtype=np.float32)
+        dist = np.zeros((Q.shape[0], K), np.float32)
+        idx = np.zeros((Q.shape[0], K), np.int32)
+        ref_lib().neighbors_extern(_ptr(Q, _fp), Q.shape[0], Q.shape[1], _ptr(dist, _fp), Q.shape[0], K,
+                                   _ptr(idx, _ip), Q.shape[0], K, C.byref(self.rec), C.byref(self.params))
+        return dist, idx
+
+    def brute_force_group(self, q, leaf_list, K, propagate=1):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        ll = np.ascontiguousarray(leaf_list, dtype=np.int32)
+        idx = np.zeros(K, np.int32)
+        dist = np.zeros(K, np.float32)
+        ref_lib().brute_force_group(_ptr(q, _fp), q.size, _ptr(ll, _ip), K, C.byref(self.rec),
+                                    C.byref(self.params), _ptr(idx, _ip), _ptr(dist, _fp), int(propagate))
+        return dist, idx
+
+    def natural_leaf(self, q, K=1):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        dist = np.zeros(K, np.float32)
+        idx = np.zeros(K, np.int32)
+        return ref_lib().kd_tree_query_tree_sequential(_ptr(q, _fp), _ptr(dist, _fp), _ptr(idx, _ip), K, 1, 1,
+                                                       C.byref(self.rec))
+
+    def exact_knn(self, q, K):
+        q = np.ascontiguousarray(q, dtype=np.float32)
+        dist = np.zeros(K, np.float32)
+        idx = np.zeros(K, np.int32)
+        ref_lib().kd_tree_query_tree_sequential(_ptr(q, _fp), _ptr(dist, _fp), _ptr(idx, _ip), K, 2147483640, 0,
+                                                C.byref(self.rec))
+        return dist, idx
+
+    def process_rows(self, pa, pb, red, n_rows, n_cols, K, propagate=True, select_best_nn=True):
+        pa = np.ascontiguousarray(pa, dtype=np.uint8)
+        pb = np.ascontiguousarray(pb, dtype=np.uint8)
+        red = np.ascontiguousarray(red, dtype=np.float32)
+        final = np.zeros(red.shape[0], np.int32)
+        ref_lib().process_rows_extern(_ptr(final, _ip), final.size, _ptr(pa, _u8p), pa.shape[0], pa.shape[1],
+                                      _ptr(pb, _u8p), pb.shape[0], pb.shape[1], _ptr(red, _fp), red.shape[0],
+                                      red.shape[1], n_rows, n_cols, K, int(bool(propagate)),
+                                      int(bool(select_best_nn)), C.byref(self.rec), C.byref(self.params))
+        return final
+
+
+def ref_norm_array(a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    return ref_lib().norm_array(_ptr(a, _u8p), _ptr(b, _u8p), a.size)

@@ -1,0 +1,171 @@
+"""GPU parity: libannf_hip.so against the golden vectors of the reference's
+openmp core and against the CPU oracle on the same float32 inputs.
+
+The bar is bit-exact: tree arrays (axis, split bits, leaf ranges, tree
+order, leaf of every point), k-lists and index fields.
+"""
+import numpy as np
+import pytest
+
+from conftest import golden, TREE_FIXTURES, ROW_FIXTURES
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _fit(X, depth, K=1, splitting="longest"):
+    from annfmp_amd import KDTree
+    t = KDTree(n_neighbors=K, leaf_size=2, tree_depth=depth, float_type="float", splitting_type=splitting)
+    return t.fit(X)
+
+
+def _assert_tree_equal(arr, ref):
+    assert np.array_equal(arr["axis"], ref.axis)
+    assert np.array_equal(bits(arr["split"]), bits(ref.split))
+    assert np.array_equal(arr["leaves"], ref.leaves)
+    assert np.array_equal(arr["xi_orig"], ref.orig)
+    assert np.array_equal(bits(arr["xi_pts"]), bits(ref.pts))
+    assert np.array_equal(arr["pli"], ref.pli)
+
+
+@pytest.mark.parametrize("name", TREE_FIXTURES)
+def test_tree_and_neighbors_match_reference(name, gpu_available):
+    g = golden(name)
+    K = int(g["K"])
+    t = _fit(g["X"], int(g["depth"]), K)
+    arr = t.tree_arrays()
+    assert np.array_equal(arr["axis"], g["axis"])
+    assert np.array_equal(bits(arr["split"]), bits(g["split"]))
+    assert np.array_equal(arr["leaves"], g["leaves"])
+    assert np.array_equal(arr["xi_orig"], g["orig"])
+    assert np.array_equal(bits(arr["xi_pts"]), bits(g["pts"]))
+    assert np.array_equal(arr["pli"], g["pli"])
+    d, idx = t.kneighbors(g["Q"])
+    assert np.array_equal(idx, g["neighbors_idx"])
+    assert np.array_equal(bits(d), bits(g["neighbors_dist"]))
+
+
+@pytest.mark.parametrize("name", ROW_FIXTURES)
+def test_process_rows_matches_reference(name, gpu_available):
+    g = golden(name)
+    p = int(g["psize"])
+    pa = O.extract_patches(g["image_a"], p)
+    pb = O.extract_patches(g["image_b"], p)
+    Ks = [int(g["K"])] + ([5] if "final_k5" in g else [])
+    for K, want in zip(Ks, [g["final"], g.get("final_k5")]):
+        t = _fit(g["red_b"], int(g["depth"]), K)
+        nn = np.zeros(g["red_a"].shape[0], np.int32)
+        t.process_rows(nn, pa, pb, g["red_a"], int(g["n_rows"]), int(g["n_cols"]), K,
+                       bool(g["propagate"]), bool(g["select_best_nn"]))
+        assert np.array_equal(nn, want), (name, K, int((nn != want).sum()))
+
+
+@pytest.mark.parametrize("n,d,kind,leaf", [
+    (100003, 16, "gauss", 64),
+    (65536, 10, "ties", 32),
+    (40000, 3, "dups", 16),
+    (5000, 32, "gauss", 8),
+])
+def test_tree_matches_oracle(n, d, kind, leaf, gpu_available):
+    rng = np.random.default_rng(n + d)
+    if kind == "gauss":
+        X = rng.standard_normal((n, d)).astype(np.float32)
+    elif kind == "ties":
+        X = rng.integers(-2, 3, (n, d)).astype(np.float32)
+    else:
+        X = rng.standard_normal((n, d)).astype(np.float32)
+        X[: n // 4] = X[n // 4: 2 * (n // 4)]
+        X[:, 1] = 0.25
+    depth = O.tree_depth(n, leaf)
+    ref = O.OracleTree(X, depth)
+    _assert_tree_equal(_fit(X, depth).tree_arrays(), ref)
+
+
+def test_cyclic_split_matches_oracle_rule(gpu_available):
+    """splitting_type='cyclic' uses axis = depth % d (kdtree.c:369-371)."""
+    rng = np.random.default_rng(5)
+    X = rng.standard_normal((4096, 5)).astype(np.float32)
+    t = _fit(X, 6, splitting="cyclic")
+    arr = t.tree_arrays()
+    for lvl in range(6):
+        assert np.all(arr["axis"][(1 << lvl) - 1:(1 << (lvl + 1)) - 1] == lvl % 5)
+
+
+@pytest.mark.parametrize("K", [1, 3, 8, 12])
+def test_exact_search_matches_oracle(K, gpu_available):
+    rng = np.random.default_rng(K)
+    X = rng.standard_normal((20000, 8)).astype(np.float32)
+    Q = rng.standard_normal((500, 8)).astype(np.float32)
+    depth = O.tree_depth(20000, 32)
+    ref = O.OracleTree(X, depth)
+    t = _fit(X, depth, K)
+    d, idx = t.kneighbors(Q)
+    rd, ridx = ref.neighbors(Q, K)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(bits(d), bits(rd))
+
+
+def test_max_leaves_limits_the_search(gpu_available):
+    from annfmp_amd import KDTree
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((8192, 4)).astype(np.float32)
+    Q = rng.standard_normal((64, 4)).astype(np.float32)
+    depth = O.tree_depth(8192, 16)
+    ref = O.OracleTree(X, depth)
+    t = KDTree(n_neighbors=3, leaf_size=16, max_leaves=2, float_type="float", splitting_type="longest").fit(X)
+    d, idx = t.kneighbors(Q)
+    rd, ridx = ref.neighbors(Q, 3, max_leaves=2)
+    assert np.array_equal(idx, ridx)
+
+
+def _field_vs_oracle(a, b, p, d, leaf, K, propagate=True, select_best_nn=True, projection="fp64"):
+    from annfmp_amd import field_on_device
+    pa = O.extract_patches(a, p)
+    pb = O.extract_patches(b, p)
+    X = np.concatenate([pa[::7], pb[::7]]).astype(np.float64)
+    mu = X.mean(0)
+    w, V = np.linalg.eigh(np.cov(X.T))
+    comp = V[:, ::-1][:, :d].T.copy()
+    nr, nc = a.shape[0] - p + 1, a.shape[1] - p + 1
+    depth = O.tree_depth(pb.shape[0], leaf)
+    res = field_on_device(a, b, p, comp, mu, depth, K, propagate, select_best_nn, projection=projection,
+                          return_reduced=True)
+    # projection against the FP64 restatement of sklearn's transform
+    ra = O.project_f64(pa, comp, mu)
+    scale = np.abs(ra).max()
+    tol = 1e-6 if projection == "fp64" else 2e-5
+    assert np.abs(res["red_a"] - ra).max() <= tol * scale
+    # rows: the oracle on the GPU's own float32 reduced arrays
+    ref = O.OracleTree(res["red_b"], depth)
+    final, pos, _ = ref.process_rows(res["red_a"], nr, nc, K, propagate, select_best_nn, pa, pb)
+    assert np.array_equal(res["knn"], ref.orig[pos])
+    assert np.array_equal(res["nn"], final)
+    # distance field: norm_array of the chosen match
+    chosen = np.array([O.norm_array(pa[i], pb[final[i]]) for i in range(0, len(final), 97)], np.float32)
+    assert np.array_equal(bits(res["dist"][::97]), bits(chosen))
+    return res
+
+
+@pytest.mark.parametrize("K,propagate,select", [(1, True, True), (5, True, True), (3, False, True),
+                                                (4, True, False)])
+def test_field_256_matches_oracle(K, propagate, select, gpu_available):
+    from annfmp_amd import synth
+    a, b = synth.make_pair("256x256-shift", seed=0)
+    _field_vs_oracle(a, b, 7, 10, 32, K, propagate, select)
+
+
+def test_field_fp32_projection(gpu_available):
+    from annfmp_amd import synth
+    a, b = synth.make_pair(synth.small_spec(96, 128), seed=4)
+    _field_vs_oracle(a, b, 5, 12, 16, 3, projection="fp32")
+
+
+def test_field_1080p_matches_oracle(gpu_available):
+    """The 1920x1080 workload (p=11, d=16, leaf 64, K=5), full size."""
+    from annfmp_amd import synth
+    a, b = synth.make_pair("1080p-warp", seed=0)
+    _field_vs_oracle(a, b, 11, 16, 64, 5, projection="fp32")

@@ -1,0 +1,195 @@
+"""The CPU oracle against the golden vectors written by the reference's own
+openmp C core (tests/golden/make_golden.py), and against closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import golden, TREE_FIXTURES, ROW_FIXTURES
+from oracle import oracle as O
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+@pytest.mark.parametrize("name", TREE_FIXTURES)
+def test_tree_matches_reference(name):
+    g = golden(name)
+    t = O.OracleTree(g["X"], int(g["depth"]))
+    assert np.array_equal(bits(t.pts), bits(g["pts"]))
+    assert np.array_equal(t.orig, g["orig"])
+    assert np.array_equal(t.axis, g["axis"])
+    assert np.array_equal(bits(t.split), bits(g["split"]))
+    assert np.array_equal(t.leaves, g["leaves"])
+    assert np.array_equal(t.pli, g["pli"])
+
+
+@pytest.mark.parametrize("name", TREE_FIXTURES)
+def test_queries_match_reference(name):
+    g = golden(name)
+    t = O.OracleTree(g["X"], int(g["depth"]))
+    K = int(g["K"])
+    for i, q in enumerate(g["Q"]):
+        assert t.natural_leaf(q) == g["natural"][i]
+        d, p = t.exact_knn(q, K)
+        assert np.array_equal(p, g["exact_pos"][i]) and np.array_equal(bits(d), bits(g["exact_dist"][i]))
+        d, p = t.scan_leaves(q, g["group_leaves"][i], K)
+        assert np.array_equal(p, g["group_pos"][i]) and np.array_equal(bits(d), bits(g["group_dist"][i]))
+    d, idx = t.neighbors(g["Q"], K, nthreads=1)
+    assert np.array_equal(idx, g["neighbors_idx"])
+    assert np.array_equal(bits(d), bits(g["neighbors_dist"]))
+
+
+@pytest.mark.parametrize("name", ROW_FIXTURES)
+def test_process_rows_matches_reference(name):
+    g = golden(name)
+    p = int(g["psize"])
+    pa = O.extract_patches(g["image_a"], p)
+    pb = O.extract_patches(g["image_b"], p)
+    t = O.OracleTree(g["red_b"], int(g["depth"]))
+    final, pos, dst = t.process_rows(g["red_a"], int(g["n_rows"]), int(g["n_cols"]), int(g["K"]),
+                                     bool(g["propagate"]), bool(g["select_best_nn"]), pa, pb, nthreads=2)
+    assert np.array_equal(final, g["final"])
+    if "final_k5" in g:
+        f5, _, _ = t.process_rows(g["red_a"], int(g["n_rows"]), int(g["n_cols"]), 5, bool(g["propagate"]),
+                                  bool(g["select_best_nn"]), pa, pb, nthreads=2)
+        assert np.array_equal(f5, g["final_k5"])
+
+
+def test_threads_do_not_change_results():
+    g = golden("rows_k3.npz")
+    p = int(g["psize"])
+    pa = O.extract_patches(g["image_a"], p)
+    pb = O.extract_patches(g["image_b"], p)
+    t = O.OracleTree(g["red_b"], int(g["depth"]))
+    a = t.process_rows(g["red_a"], int(g["n_rows"]), int(g["n_cols"]), 3, True, True, pa, pb, nthreads=1)
+    b = t.process_rows(g["red_a"], int(g["n_rows"]), int(g["n_cols"]), 3, True, True, pa, pb, nthreads=4)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_norm_array_matches_reference():
+    g = golden("norm_array.npz")
+    out = np.array([O.norm_array(a, b) for a, b in zip(g["A"], g["B"])], np.float32)
+    assert np.array_equal(bits(out), bits(g["out"]))
+    # the all-0 vs all-255 pair: 363 * 255^2 = 23,604,075 needs more than 24
+    # bits, so the sequential FP32 sum (23,603,972) is not the exact integer;
+    # the reference's value is sqrt of the FP32 sum, not of the integer
+    s = np.float32(0)
+    for _ in range(363):
+        s = np.float32(s + np.float32(255 * 255))
+    assert float(s) == 23603972.0
+    assert g["out"][0] == np.float32(math.sqrt(float(s)))
+    assert g["out"][0] != np.float32(math.sqrt(363 * 255 * 255))
+
+
+def test_patch_order_closed_form():
+    """extract_patches: feature f = (i*p + j)*3 + ch of window (y, x)."""
+    rng = np.random.default_rng(1)
+    img = rng.integers(0, 256, (9, 11, 3)).astype(np.uint8)
+    p = 4
+    P = O.extract_patches(img, p)
+    nc = 11 - p + 1
+    for n in (0, 5, 17, P.shape[0] - 1):
+        y, x = divmod(n, nc)
+        for f in (0, 7, 20, 47):
+            i, rem = divmod(f, p * 3)
+            j, ch = divmod(rem, 3)
+            assert P[n, f] == img[y + i, x + j, ch]
+
+
+def test_tree_depth_closed_form():
+    for n in (2, 3, 100, 1023, 1024, 1025, 62500, 2043700, 8246464):
+        for leaf in (2, 5, 32, 64, 128):
+            want = max(0, int(math.floor(math.log(n, 2))) - int(math.floor(math.log(leaf, 2))))
+            with np.errstate(all="ignore"):
+                import warnings
+                with warnings.catch_warnings():
+                    warnings.simplefilter("ignore")
+                    assert O.tree_depth(n, leaf) == want
+    assert O.tree_depth(2043700, 64) == 14
+    assert O.tree_depth(62500, 32) == 10
+    assert O.tree_depth(8246464, 128) == 15
+
+
+def _seq_dist(P, q):
+    """Sequential FP32 sum of (p - q)^2 in numpy float32 arithmetic."""
+    s = np.zeros(P.shape[0], np.float32)
+    for j in range(P.shape[1]):
+        t = (P[:, j] - q[j]).astype(np.float32)
+        s = (s + (t * t).astype(np.float32)).astype(np.float32)
+    return s
+
+
+def test_exact_search_equals_brute_force():
+    """Row-0 search is exact: same list as a stable sort of all points."""
+    rng = np.random.default_rng(7)
+    X = rng.standard_normal((3000, 6)).astype(np.float32)
+    t = O.OracleTree(X, O.tree_depth(3000, 16))
+    for q in rng.standard_normal((30, 6)).astype(np.float32):
+        d, p = t.exact_knn(q, 5)
+        dd = _seq_dist(t.pts, q)
+        order = np.lexsort((np.arange(len(dd)), dd))[:5]
+        assert np.array_equal(bits(d), bits(dd[order]))
+        assert np.array_equal(p, order)
+
+
+def test_partition_invariants():
+    """Every internal node: left holds all x < split and right all x > split;
+    leaf sizes differ by at most one; pli inverts the leaf ranges."""
+    rng = np.random.default_rng(3)
+    X = rng.integers(0, 5, (2000, 3)).astype(np.float32)
+    depth = O.tree_depth(2000, 8)
+    t = O.OracleTree(X, depth)
+    sizes = np.diff(t.leaves.reshape(-1, 2), axis=1).ravel()
+    assert sizes.max() - sizes.min() <= 1 or depth == 0
+    for l in range(1 << depth):
+        a, b = t.leaves[2 * l], t.leaves[2 * l + 1]
+        assert np.all(t.pli[t.orig[a:b]] == l)
+    # walk: every point's natural descent with its own coordinates reaches a
+    # leaf whose path splits bound it
+    bnd = [(0, 2000)]
+    for lvl in range(depth):
+        nb = []
+        for j, (lo, hi) in enumerate(bnd):
+            node = (1 << lvl) - 1 + j
+            c = hi - lo
+            ax, v = t.axis[node], t.split[node]
+            left = t.pts[lo:lo + c // 2, ax]
+            right = t.pts[lo + c // 2:hi, ax]
+            assert np.all(left <= v) and np.all(right >= v)
+            # split = value of rank c//2 of the node's axis column (util.c Wirth select)
+            assert np.sort(t.pts[lo:hi, ax])[c // 2] == v
+            nb += [(lo, lo + c // 2), (lo + c // 2, hi)]
+        bnd = nb
+
+
+def test_planted_mistake_is_caught():
+    """A tie rule flipped in a Python copy of the insert (<= -> <) changes a
+    golden list: the fixtures pin the rule."""
+    g = golden("tree_ties.npz")
+    t = O.OracleTree(g["X"], int(g["depth"]))
+    K = int(g["K"])
+
+    def scan(q, leaves, strict):
+        ld = [np.float32(3.402823466e38)] * K
+        lp = [0] * K
+        for l in leaves:
+            for i in range(t.leaves[2 * l], t.leaves[2 * l + 1]):
+                d = _seq_dist(t.pts[i:i + 1], q)[0]
+                rej = (ld[-1] < d) if strict else (ld[-1] <= d)
+                if rej:
+                    continue
+                ld[-1], lp[-1] = d, i
+                j = K - 1
+                while j > 0 and ld[j] < ld[j - 1]:
+                    ld[j], ld[j - 1] = ld[j - 1], ld[j]
+                    lp[j], lp[j - 1] = lp[j - 1], lp[j]
+                    j -= 1
+        return np.array(lp)
+
+    ok = [np.array_equal(scan(q, g["group_leaves"][i], False), g["group_pos"][i]) for i, q in enumerate(g["Q"][:16])]
+    bad = [np.array_equal(scan(q, g["group_leaves"][i], True), g["group_pos"][i]) for i, q in enumerate(g["Q"][:16])]
+    assert all(ok)
+    assert not all(bad)
