@@ -82,7 +82,7 @@ struct annf_ctx {
     // tree storage
     DevBuf pts_a, pts_b, orig_a, orig_b, axis, split, leaves, pli;
     // build scratch
-    DevBuf keys, eqoff, box, nax, prefix, rank, hist, cnt, tot, bnd;
+    DevBuf keys, eqoff, box, nax, prefix, rank, hist, cnt, tot, bnd, errflag;
     // query workspaces
     DevBuf qbuf, natleaf, bcount, boffset, bcursor, order, p1d, p1p, p1l, propleaf;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
@@ -117,7 +117,7 @@ struct annf_ctx {
 
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
-                          &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &qbuf, &natleaf, &bcount, &boffset,
+                          &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &bcount, &boffset,
                           &bcursor, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64};
         for (DevBuf *b : list) b->release();
@@ -129,6 +129,18 @@ namespace {
 
 int sync_stream(annf_ctx *c) {
     HIPCHK(hipStreamSynchronize(c->stream));
+    return ANNF_OK;
+}
+
+// Reads the device error flag of the last build (after the stream is idle).
+int check_build_flag(annf_ctx *c) {
+    if (!c->errflag.p) return ANNF_OK;
+    int flag = 0;
+    HIPCHK(hipMemcpy(&flag, c->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+        c->has_tree = false;
+        return fail(ANNF_EHIP, "tree build detected an inconsistent partition (flag %d)", flag);
+    }
     return ANNF_OK;
 }
 
@@ -190,6 +202,8 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     RCHK(c->cnt.ensure(chunks_max * sizeof(int2)));
     RCHK(c->tot.ensure(nodes_max * sizeof(int2)));
     RCHK(c->bnd.ensure(bnd.size() * sizeof(int)));
+    RCHK(c->errflag.ensure(sizeof(int)));
+    HIPCHK(hipMemsetAsync(c->errflag.p, 0, sizeof(int), c->stream));
     HIPCHK(hipMemcpyAsync(c->bnd.p, bnd.data(), bnd.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     HIPCHK(iota(c->orig_a.as<int>(), n, c->stream));
     BuildScratch s;
@@ -207,6 +221,7 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     s.cnt = c->cnt.as<int2>();
     s.tot = c->tot.as<int2>();
     s.bnd = c->bnd.as<int>();
+    s.err = c->errflag.as<int>();
     TreeView t;
     HIPCHK(build_tree(n, d, dpad, depth, c->splitting_type, bnd.data(), off.data(), cpn.data(), s,
                       c->axis.as<int>(), c->split.as<float>(), c->leaves.as<int>(), c->pli.as<int>(), t,
@@ -355,7 +370,8 @@ int annf_fit_dev(annf_ctx *c, const float *X, int n, int d) {
     RCHK(c->pts_a.ensure((size_t)n * dpad * sizeof(float)));
     HIPCHK(pad_rows(X, n, d, dpad, c->pts_a.as<float>(), c->stream));
     RCHK(build_from_padded(c, n, d, dpad, c->tree_depth));
-    return sync_stream(c);
+    RCHK(sync_stream(c));
+    return check_build_flag(c);
 }
 
 int annf_fit(annf_ctx *c, const float *X, int n, int d) {
@@ -585,6 +601,7 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
     PatchGeom gb{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb};
     RCHK(run_rows(c, (int)n_a, nr_a, nc_a, K, propagate, select_best_nn, ga, gb, nn_out, dist_out, knn_out));
     RCHK(sync_stream(c));
+    RCHK(check_build_flag(c));
     c->collect_marks();
     return ANNF_OK;
 }

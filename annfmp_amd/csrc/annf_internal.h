@@ -38,6 +38,7 @@ struct BuildScratch {
     int2 *cnt;                        // total chunks of the widest level
     int2 *tot;                        // nodes_max
     int *bnd;                         // all levels' bounds
+    int *err;                         // 1 int, non-zero if a kernel saw an impossible state
 };
 
 // How a patch is laid out for best-of-K: `rows` runs of `rowlen` bytes,

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__
                                                        LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
                                                        const int2 *cnt_off, const int2 *tot, int *eqoff,
                                                        float *__restrict__ pts_out, int *__restrict__ orig_out,
-                                                       const int *nax, float *split) {
+                                                       const int *nax, float *split, int *err) {
     __shared__ int2 wsum[kBuildTPB / kWave];
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
@@ -410,17 +410,25 @@ __global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__
             const int go = (g - 1) - (off - r_lt);
             if (k[u] == vk) {
                 const int t = (E - 1) - r_eq++;
-                if (!WRITE) eqoff[lo + t] = go;
+                if (!WRITE) {
+                    if (t < 0 || t >= c) { atomicOr(err, 2); continue; }
+                    eqoff[lo + t] = go;
+                }
                 slot = Lt + t;
             } else {
                 int p = go;
                 if (WRITE) {
-                    while (p < E) p = eqoff[lo + p];
+                    // the chain strictly increases, so it ends within E steps
+                    for (int it = 0; p < E && it <= E; ++it) p = eqoff[lo + p];
                 }
                 slot = Lt + p;
             }
         }
         if (WRITE) {
+            if (slot < 0 || slot >= c) {   // cannot happen for a correct partition
+                atomicOr(err, 1);
+                continue;
+            }
             const int dst = lo + slot;
             const float4 *src4 = reinterpret_cast<const float4 *>(pts_in + (size_t)i * DP);
             float4 *dst4 = reinterpret_cast<float4 *>(pts_out + (size_t)dst * DP);
@@ -479,9 +487,9 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
             hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, s.prefix, s.cnt);
             hipLaunchKernelGGL(k_scan_nodes, dim3(L.nodes), dim3(kBuildTPB), 0, st, L, s.cnt, s.tot);
             hipLaunchKernelGGL((k_scatter<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
-                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split);
+                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
             hipLaunchKernelGGL((k_scatter<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
-                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split);
+                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
