@@ -82,6 +82,25 @@ __device__ __forceinline__ float dist_sq(const float *__restrict__ p, const floa
     return s;
 }
 
+// Correctly rounded FP32 square root (the reference takes sqrt in double and
+// stores a float, base.c:127 and base.c:371, which is the correctly rounded
+// float root).  A first value comes from the FP64 root; it is then checked
+// against the midpoints to its two neighbouring floats.  A midpoint has 25
+// significant bits, so its square is exact in FP64, and no float equals the
+// square of a midpoint, so there are no ties.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    if (!(x > 0.0f) || !(x < INFINITY)) return (float)sqrt((double)x);
+    float r = (float)sqrt((double)x);
+    const double xd = (double)x;
+    const float up = __uint_as_float(__float_as_uint(r) + 1u);
+    const double mu = 0.5 * ((double)r + (double)up);
+    if (mu * mu < xd) return up;
+    const float dn = __uint_as_float(__float_as_uint(r) - 1u);
+    const double md = 0.5 * ((double)r + (double)dn);
+    if (md * md > xd) return dn;
+    return r;
+}
+
 template <typename Fn>
 static hipError_t with_dp(int dpad, Fn &&fn) {
     switch (dpad) {
@@ -944,7 +963,7 @@ __device__ __forceinline__ float patch_l2(const PatchGeom &ga, long long i, cons
             s = s + df * df;
         }
     }
-    return __fsqrt_rn(s);
+    return sqrt_rn(s);
 }
 
 __global__ __launch_bounds__(256) void k_best(PatchGeom ga, PatchGeom gb, const int *__restrict__ fl_pos,
@@ -986,7 +1005,7 @@ __global__ void k_convert(const int *pos, const float *d2, const int *orig, long
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     idx[i] = orig[pos[i]];
-    dist[i] = __fsqrt_rn(d2[i]);
+    dist[i] = sqrt_rn(d2[i]);
 }
 
 hipError_t convert_knn(const int *pos, const float *d2, const int *orig, long long count, int *idx_out,
