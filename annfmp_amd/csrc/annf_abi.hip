@@ -84,7 +84,7 @@ struct annf_ctx {
     // build scratch
     DevBuf keys, eqoff, box, nax, prefix, rank, hist, cnt, tot, bnd, errflag;
     // query workspaces
-    DevBuf qbuf, natleaf, bcount, boffset, bcursor, order, p1d, p1p, p1l, propleaf;
+    DevBuf qbuf, natleaf, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
     DevBuf basis32, mean32, basis64, mean64;
     // timing
@@ -118,7 +118,7 @@ struct annf_ctx {
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
                           &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &bcount, &boffset,
-                          &bcursor, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
+                          &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64};
         for (DevBuf *b : list) b->release();
         has_tree = false;
@@ -263,22 +263,25 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         RCHK(c->bcount.ensure((size_t)nleaves * sizeof(int)));
         RCHK(c->boffset.ensure(((size_t)nleaves + 1) * sizeof(int)));
         RCHK(c->bcursor.ensure((size_t)nleaves * sizeof(int)));
+        RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
         RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
         HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
         HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), c->stream));
         HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
-                              c->bcursor.as<int>(), c->order.as<int>(), c->stream));
+                              c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), c->stream));
         if (propagate) {
             RCHK(c->p1d.ensure(nk * sizeof(float)));
             RCHK(c->p1p.ensure(nk * sizeof(int)));
             RCHK(c->p1l.ensure(nk * sizeof(int)));
-            HIPCHK(leaf_scan(t, c->qbuf.as<float>(), c->order.as<int>(), c->boffset.as<int>(), K, propleaf,
-                             c->p1d.as<float>(), c->p1p.as<int>(), c->p1l.as<int>(), c->stream));
+            HIPCHK(leaf_scan(t, c->qbuf.as<float>(), n_a, c->order.as<int>(), c->boffset.as<int>(),
+                             c->bchunk.as<int>(), K, propleaf, c->p1d.as<float>(), c->p1p.as<int>(),
+                             c->p1l.as<int>(), c->stream));
         } else {
             // without propagation the natural-leaf list is the row's list
             RCHK(c->p1d.ensure(nk * sizeof(float)));
-            HIPCHK(leaf_scan(t, c->qbuf.as<float>(), c->order.as<int>(), c->boffset.as<int>(), K, nullptr,
-                             c->p1d.as<float>(), c->fl_pos.as<int>(), nullptr, c->stream));
+            HIPCHK(leaf_scan(t, c->qbuf.as<float>(), n_a, c->order.as<int>(), c->boffset.as<int>(),
+                             c->bchunk.as<int>(), K, nullptr, c->p1d.as<float>(), c->fl_pos.as<int>(), nullptr,
+                             c->stream));
         }
     }
     c->mark(4);
@@ -287,7 +290,8 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
     c->mark(5);
     if (propagate && n_rows > 1) {
         HIPCHK(row_sweep(t, c->qbuf.as<float>(), n_rows, n_cols, K, propleaf, c->p1d.as<float>(),
-                         c->p1p.as<int>(), c->p1l.as<int>(), c->fl_pos.as<int>(), c->fl_l.as<int>(), c->stream));
+                         c->p1p.as<int>(), c->p1l.as<int>(), c->natleaf.as<int>(), c->fl_pos.as<int>(),
+                         c->fl_l.as<int>(), c->stream));
     }
     c->mark(6);
     HIPCHK(best_of_k(ga, gb, c->fl_pos.as<int>(), t.orig, n_a, K, select_best, final_dev, dist_dev, knn_dev,
@@ -456,8 +460,8 @@ int annf_process_rows(annf_ctx *c, int *finalindices, int nfinalindices, const u
         RCHK(c->stage_in3.ensure(bb));
         HIPCHK(hipMemcpyAsync(c->stage_in2.p, patchesA, ba, hipMemcpyHostToDevice, c->stream));
         HIPCHK(hipMemcpyAsync(c->stage_in3.p, patchesB, bb, hipMemcpyHostToDevice, c->stream));
-        ga = PatchGeom{c->stage_in2.as<uint8_t>(), 0, 0, 0, 1, dpatchesA, 0};
-        gb = PatchGeom{c->stage_in3.as<uint8_t>(), 0, 0, 0, 1, dpatchesB, 0};
+        ga = PatchGeom{c->stage_in2.as<uint8_t>(), 0, 0, 0, 1, dpatchesA, 0, (long long)ba};
+        gb = PatchGeom{c->stage_in3.as<uint8_t>(), 0, 0, 0, 1, dpatchesB, 0, (long long)bb};
     }
     c->mark(1);
     c->mark(2);
@@ -597,8 +601,8 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
                            c->qbuf.as<float>(), n_a, d, dpad, red_a_out);
         HIPCHK(hipGetLastError());
     }
-    PatchGeom ga{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa};
-    PatchGeom gb{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb};
+    PatchGeom ga{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa, 3LL * wa * ha};
+    PatchGeom gb{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb, 3LL * wb * hb};
     RCHK(run_rows(c, (int)n_a, nr_a, nc_a, K, propagate, select_best_nn, ga, gb, nn_out, dist_out, knn_out));
     RCHK(sync_stream(c));
     RCHK(check_build_flag(c));

@@ -52,6 +52,7 @@ struct PatchGeom {
     int rows;
     int rowlen;
     long long rstride;
+    long long nbytes;  // size of the buffer at base (reads never pass it)
 };
 
 // ---- launchers (annf_kernels.hip); all enqueue on `st` ----
@@ -67,18 +68,20 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
 
 hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count,
                           hipStream_t st);
+// Buckets queries by natural leaf; `count` must hold the per-leaf counts and
+// is overwritten with chunks per leaf; chunk_off receives nleaves+1 offsets.
 hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset,
-                          int *cursor, int *order, hipStream_t st);
-hipError_t leaf_scan(const TreeView &t, const float *q, const int *order, const int *offset,
-                     int K, const int *propleaf, float *out_d, int *out_p, int *out_l,
-                     hipStream_t st);
+                          int *cursor, int *order, int *chunk_off, hipStream_t st);
+hipError_t leaf_scan(const TreeView &t, const float *q, int nq, const int *order, const int *offset,
+                     const int *chunk_off, int K, const int *propleaf, float *out_d, int *out_p,
+                     int *out_l, hipStream_t st);
 hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long long max_leaves,
                         const int *propleaf, float *out_d, int *out_p, int *out_l,
                         hipStream_t st);
 hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hipStream_t st);
 hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K,
                      const int *propleaf, const float *p1d, const int *p1p, const int *p1l,
-                     int *fl_pos, const int *row0_l, hipStream_t st);
+                     const int *natleaf, int *fl_pos, const int *row0_l, hipStream_t st);
 hipError_t best_of_k(const PatchGeom &ga, const PatchGeom &gb, const int *fl_pos,
                      const int *orig, int nq, int K, int select_best, int *final_out,
                      float *dist_out, int *knn_out, hipStream_t st);

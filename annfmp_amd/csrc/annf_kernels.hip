@@ -101,6 +101,56 @@ __device__ __forceinline__ float sqrt_rn(float x) {
     return r;
 }
 
+// [from, to) of leaf l computed from n and the depth alone: every node's
+// segment splits at floor(c/2) (kdtree.c:105 and 114-115), so no table read
+// (and no memory latency) is needed.
+__device__ __forceinline__ void leaf_range(int l, int n, int depth, int &from, int &to) {
+    int lo = 0, hi = n;
+    for (int lev = depth - 1; lev >= 0; --lev) {
+        const int mid = lo + ((hi - lo) >> 1);
+        if ((l >> lev) & 1) lo = mid; else hi = mid;
+    }
+    from = lo;
+    to = hi;
+}
+
+// Orders LDS accesses of the lanes of one wavefront.
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// atomicAdd(&ctr[key], 1) for every active lane; lanes of the wavefront that
+// share a key are combined into one atomic (two rounds, then per-lane), so a
+// run of identical keys (flat image regions) does not serialise on one
+// address.  Returns the value this lane's increment saw.
+__device__ __forceinline__ int wave_add1(int *ctr, int key) {
+    int res = 0;
+    bool done = false;
+    for (int round = 0; round < 2; ++round) {
+        if (!done) {
+            const int lead = __builtin_amdgcn_readfirstlane(key);
+            const bool mine = key == lead;
+            const unsigned long long m = __ballot(mine);
+            if (mine) {
+                const int r = lane_rank(m);
+                int base = 0;
+                if (r == 0) base = atomicAdd(&ctr[lead], __popcll(m));
+                base = __builtin_amdgcn_readlane(base, __ffsll(m) - 1);
+                res = base + r;
+                done = true;
+            }
+        }
+    }
+    if (!done) res = atomicAdd(&ctr[key], 1);
+    return res;
+}
+
 template <typename Fn>
 static hipError_t with_dp(int dpad, Fn &&fn) {
     switch (dpad) {
@@ -550,7 +600,7 @@ __global__ __launch_bounds__(256) void k_natural(const float *__restrict__ q, in
     }
     int l = idx - ((1 << depth) - 1);
     leaf[i] = l;
-    if (count) atomicAdd(&count[l], 1);
+    if (count) (void)wave_add1(count, l);
 }
 
 hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, hipStream_t st) {
@@ -583,9 +633,10 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int *count, int n, i
     for (int w = 0; w < wid; ++w) off += wsum[w];
     int run = inc - loc + off;
     for (int k = 0; k < per && b + k < n; ++k) {
+        const int c = count[b + k];
         offset[b + k] = run;
-        cursor[b + k] = run;
-        run += count[b + k];
+        if (cursor) cursor[b + k] = run;
+        run += c;
     }
     if (threadIdx.x == blockDim.x - 1) offset[n] = run;
 }
@@ -593,14 +644,26 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int *count, int n, i
 __global__ void k_bucket(const int *leaf, int nq, int *cursor, int *order) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
-    int p = atomicAdd(&cursor[leaf[i]], 1);
+    const int p = wave_add1(cursor, leaf[i]);
     order[p] = i;
 }
 
+// Blocks of the leaf scan: a leaf's bucket is cut into chunks of kScanChunk
+// queries, so one crowded leaf (a flat image region) spreads over many blocks.
+constexpr int kScanChunk = 256;
+
+__global__ void k_chunk_count(const int *offset, int nleaves, int *chunks) {
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < nleaves) chunks[l] = (offset[l + 1] - offset[l] + kScanChunk - 1) / kScanChunk;
+}
+
 hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset, int *cursor, int *order,
-                          hipStream_t st) {
+                          int *chunk_off, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, offset, cursor);
     if (nq > 0) hipLaunchKernelGGL(k_bucket, dim3(blocks_for(nq, 256)), dim3(256), 0, st, leaf, nq, cursor, order);
+    // chunks per leaf (into count, no longer needed) and their offsets
+    hipLaunchKernelGGL(k_chunk_count, dim3(blocks_for(nleaves, 256)), dim3(256), 0, st, offset, nleaves, count);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, chunk_off, (int *)nullptr);
     return hipGetLastError();
 }
 
@@ -637,12 +700,22 @@ __device__ __forceinline__ void list_insert(float (&ld)[KM], int (&lp)[KM], floa
 // for one query of the leaf's bucket (kdtree.c:286-303, natural leaf only).
 template <int DP, int KM>
 __global__ __launch_bounds__(256) void k_leafscan(TreeView t, const float *__restrict__ q, const int *__restrict__ order,
-                                                  const int *__restrict__ offset, int K, const int *__restrict__ propleaf,
+                                                  const int *__restrict__ offset, const int *__restrict__ chunk_off,
+                                                  int K, const int *__restrict__ propleaf,
                                                   float *__restrict__ od, int *__restrict__ op, int *__restrict__ ol) {
     extern __shared__ float4 sp4[];
-    const int leaf = blockIdx.x;
-    const int qb = offset[leaf], qe = offset[leaf + 1];
-    if (qb == qe) return;
+    const int nleaves = 1 << t.depth;
+    const int b = blockIdx.x;
+    if (b >= chunk_off[nleaves]) return;
+    int lo = 0, hi = nleaves - 1;     // leaf: last l with chunk_off[l] <= b
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (chunk_off[m] <= b) lo = m; else hi = m - 1;
+    }
+    const int leaf = lo;
+    const int qb = offset[leaf] + (b - chunk_off[leaf]) * kScanChunk;
+    const int qe = min(qb + kScanChunk, offset[leaf + 1]);
+    if (qb >= qe) return;
     const int from = t.leaves[2 * leaf], np = t.leaves[2 * leaf + 1] - from;
     const float4 *src = reinterpret_cast<const float4 *>(t.pts + (size_t)from * DP);
     for (int c = threadIdx.x; c < np * (DP / 4); c += blockDim.x) sp4[c] = src[c];
@@ -672,16 +745,19 @@ __global__ __launch_bounds__(256) void k_leafscan(TreeView t, const float *__res
     }
 }
 
-hipError_t leaf_scan(const TreeView &t, const float *q, const int *order, const int *offset, int K,
-                     const int *propleaf, float *out_d, int *out_p, int *out_l, hipStream_t st) {
+hipError_t leaf_scan(const TreeView &t, const float *q, int nq, const int *order, const int *offset,
+                     const int *chunk_off, int K, const int *propleaf, float *out_d, int *out_p, int *out_l,
+                     hipStream_t st) {
     const int nleaves = 1 << t.depth;
     const size_t lds = (size_t)t.max_leaf * t.dpad * sizeof(float);
     if (lds > 64 * 1024) return hipErrorInvalidValue;
+    // chunks <= one per leaf + one per kScanChunk queries
+    const unsigned nblk = (unsigned)(nleaves + (nq + kScanChunk - 1) / kScanChunk);
     return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
         constexpr int DP = decltype(dp)::value;
         constexpr int KM = decltype(km)::value;
-        hipLaunchKernelGGL((k_leafscan<DP, KM>), dim3(nleaves), dim3(256), lds, st, t, q, order, offset, K, propleaf,
-                           out_d, out_p, out_l);
+        hipLaunchKernelGGL((k_leafscan<DP, KM>), dim3(nblk), dim3(256), lds, st, t, q, order, offset, chunk_off, K,
+                           propleaf, out_d, out_p, out_l);
         return hipGetLastError();
     });
 }
@@ -852,9 +928,391 @@ __global__ __launch_bounds__(256) void k_exact(TreeView t, const float *__restri
     }
 }
 
+// A workgroup's candidate set in LDS: unique 64-bit (distance, arrival)
+// keys with their tree positions and leaf links.  Threads append the
+// candidates that beat the current K-th key (wave ballot + one LDS atomic
+// per wavefront); when the set fills up it is cut back to its K smallest
+// keys by rank counting (each thread ranks its items against all items with
+// broadcast reads), which also orders the final list.  Keys are unique, so
+// ranks are a permutation.
+constexpr int kBlock = 256;
+constexpr int kBatch = 4;                            // candidates per thread between checks
+constexpr int kSetCap = 32 + kBatch * kBlock;        // live K <= 32 + one batch of appends
+constexpr int set_words(int cap) { return 4 * cap + 4 * 32 + 4; }  // key(2) pos pl | out key(2) pos pl | count
+constexpr int kSetWords = set_words(kSetCap);
+
+struct BlockSet {
+    uint64_t *key;
+    int *pos;
+    int *pl;
+    uint64_t *okey;
+    int *opos;
+    int *opl;
+    int *count;
+};
+
+__device__ __forceinline__ BlockSet bset_at(uint32_t *base, int cap = kSetCap) {
+    BlockSet S;
+    S.key = reinterpret_cast<uint64_t *>(base);
+    S.pos = reinterpret_cast<int *>(base + 2 * cap);
+    S.pl = S.pos + cap;
+    S.okey = reinterpret_cast<uint64_t *>(base + 4 * cap);
+    S.opos = reinterpret_cast<int *>(base + 4 * cap + 64);
+    S.opl = S.opos + 32;
+    S.count = S.opl + 32;
+    return S;
+}
+
+__device__ __forceinline__ void bset_append(const BlockSet &S, bool take, uint64_t k, int p, int l) {
+    const unsigned long long m = __ballot(take);
+    const int cnt = __popcll(m);
+    const int lane = threadIdx.x & (kWave - 1);
+    int base = 0;
+    if (lane == 0 && cnt) base = atomicAdd(S.count, cnt);
+    base = __builtin_amdgcn_readlane(base, 0);
+    if (take) {
+        const int slot = base + lane_rank(m);
+        S.key[slot] = k;
+        S.pos[slot] = p;
+        S.pl[slot] = l;
+    }
+}
+
+// Keeps the K smallest keys, ascending, in [0, min(n, K)).  Block-wide.
+__device__ __forceinline__ void bset_reduce(const BlockSet &S, int K) {
+    __syncthreads();
+    const int n = *S.count;
+    if (n > K) {
+        for (int j = threadIdx.x; j < n; j += blockDim.x) {
+            const uint64_t kj = S.key[j];
+            int r = 0;
+            for (int i = 0; i < n; ++i) r += (S.key[i] < kj) ? 1 : 0;
+            if (r < K) {
+                S.okey[r] = kj;
+                S.opos[r] = S.pos[j];
+                S.opl[r] = S.pl[j];
+            }
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < K) {
+            S.key[threadIdx.x] = S.okey[threadIdx.x];
+            S.pos[threadIdx.x] = S.opos[threadIdx.x];
+            S.pl[threadIdx.x] = S.opl[threadIdx.x];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) *S.count = K;
+    } else if (n > 1) {
+        // fewer than K+1 items: order them
+        uint64_t kj = 0;
+        int pj = 0, lj = 0, r = 0;
+        const bool mine = (int)threadIdx.x < n;
+        if (mine) {
+            kj = S.key[threadIdx.x];
+            pj = S.pos[threadIdx.x];
+            lj = S.pl[threadIdx.x];
+            for (int i = 0; i < n; ++i) r += (S.key[i] < kj) ? 1 : 0;
+        }
+        __syncthreads();
+        if (mine) {
+            S.key[r] = kj;
+            S.pos[r] = pj;
+            S.pl[r] = lj;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, lane);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), lane);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Append for a one-wavefront block: the item count n is wave-uniform and
+// kept in a register (no LDS atomic).
+__device__ __forceinline__ void wset_append(const BlockSet &S, int &n, bool take, uint64_t k, int p, int l) {
+    const unsigned long long m = __ballot(take);
+    if (take) {
+        const int slot = n + lane_rank(m);
+        S.key[slot] = k;
+        S.pos[slot] = p;
+        S.pl[slot] = l;
+    }
+    n += __popcll(m);
+}
+
+// bset_reduce for a one-wavefront block: with at most 64 items each lane
+// holds one item and ranks it with readlane; lane j < K receives the j-th
+// smallest (returned in opos/opl) and writes it back as item j.
+__device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, int lane, int &opos, int &opl) {
+    wave_sync();
+    if (n > kWave) {
+        if (lane == 0) *S.count = n;
+        bset_reduce(S, K);
+        n = K;
+        opos = lane < K ? S.pos[lane] : 0;
+        opl = lane < K ? S.pl[lane] : 0;
+        return;
+    }
+    const bool have = lane < n;
+    const uint64_t mk = have ? S.key[lane] : ~0ull;
+    const int mp = have ? S.pos[lane] : 0, ml = have ? S.pl[lane] : 0;
+    int r = 0;
+    for (int i = 0; i < n; ++i) r += (readlane_u64(mk, i) < mk) ? 1 : 0;
+    const int m = min(n, K);
+    uint64_t ok = 0;
+    for (int j = 0; j < m; ++j) {
+        const unsigned long long b = __ballot(have && r == j);
+        const int o = __ffsll(b) - 1;
+        const uint64_t kj = readlane_u64(mk, o);
+        const int pj = readlane_i(mp, o), lj = readlane_i(ml, o);
+        if (lane == j) { ok = kj; opos = pj; opl = lj; }
+    }
+    wave_sync();
+    if (lane < m) {
+        S.key[lane] = ok;
+        S.pos[lane] = opos;
+        S.pl[lane] = opl;
+    }
+    n = m;
+    wave_sync();
+}
+
+// Exact k-NN search as a bounded, order-free scan (one workgroup per query).
+//
+// The reference's depth-first search (kdtree.c:129-268) ends with the list
+// of the K smallest (distance, arrival) keys among the points it visited.
+// With D* the K-th smallest distance over all points, every point p with
+// d(p) <= D* is visited: on p's path, a far child is skipped only if
+// (q_a - s)^2 > dK(current) >= D* >= d(p) >= (q_a - p_a)^2 >= (q_a - s)^2
+// (FP32 subtraction, squaring and sequential sums of non-negative terms are
+// monotone), a contradiction.  So the reference's list is the K smallest keys
+// among {p : d(p) <= D*}.  Any radius U >= D* therefore finds a superset: the
+// K-th distance of the natural leaf is such a U.  Arrival order is the
+// depth-first leaf order -- near child first -- which is the integer order
+// of the "went far" bits read from the root, followed by the position in the
+// leaf; that pair replaces the arrival number in the key.
+//
+// Per query: near/pass bits of every internal node go to LDS (coalesced
+// passes over the node arrays); every leaf is tested against its path; the
+// passing leaves are queued in LDS and scanned by the whole workgroup, one
+// leaf per wavefront per step.
+constexpr int kExactQueue = 1024;
+
+template <int DP>
+__global__ __launch_bounds__(kBlock) void k_exact_bound(TreeView t, const float *__restrict__ q, int nq, int K,
+                                                        const int *__restrict__ propleaf, float *__restrict__ od,
+                                                        int *__restrict__ op, int *__restrict__ ol, int offbits,
+                                                        int words) {
+    extern __shared__ uint32_t smem_u[];
+    const int tid = threadIdx.x, wid = tid / kWave, lane = tid & (kWave - 1);
+    const BlockSet S = bset_at(smem_u);
+    uint32_t *nearb = smem_u + kSetWords;
+    uint32_t *passb = nearb + words;
+    int *qfrom = reinterpret_cast<int *>(passb + words);
+    int *qcnt = qfrom + kExactQueue;
+    int *qrank = qcnt + kExactQueue;
+    int *qn_p = qrank + kExactQueue;
+    const int w = blockIdx.x;
+    if (w >= nq) return;
+    float qv[DP];
+    load_vec<DP>(q + (size_t)w * DP, qv);
+    const float qlane = lane < DP ? q[(size_t)w * DP + lane] : 0.0f;
+    const int nn = (1 << t.depth) - 1;
+    const int nleaves = 1 << t.depth;
+
+    // near bits: right child first iff q[axis] - split >= 0
+    for (int base = 0; base < nn; base += kBlock) {
+        const int n = base + tid;
+        const int a = n < nn ? t.axis[n] : 0;
+        const float sv = n < nn ? t.split[n] : 0.0f;
+        const float diff = __shfl(qlane, a, kWave) - sv;
+        const unsigned long long b = __ballot(n < nn && diff >= 0.0f);
+        const int wb = base + wid * kWave;
+        if (lane == 0 && wb < nn) {
+            nearb[wb >> 5] = (uint32_t)b;
+            nearb[(wb >> 5) + 1] = (uint32_t)(b >> 32);
+        }
+    }
+    if (tid == 0) *S.count = 0;
+    __syncthreads();
+
+    // radius U: K-th distance in the natural leaf (FLT_MAX if it holds < K)
+    int node = 0;
+    for (int lev = 0; lev < t.depth; ++lev) node = 2 * node + 1 + (int)((nearb[node >> 5] >> (node & 31)) & 1u);
+    int from, to;
+    leaf_range(node - nn, t.n, t.depth, from, to);
+    bset_append(S, tid < K, make_key(kFltMax, (uint32_t)tid), 0, 0);
+    __syncthreads();
+    for (int ob = 0; ob < to - from; ob += kBlock) {
+        const int off = ob + tid;
+        const bool val = off < to - from;
+        const float dd = val ? dist_sq<DP>(t.pts + (size_t)(from + off) * DP, qv) : 0.0f;
+        bset_append(S, val, make_key(dd, (uint32_t)(K + off)), 0, 0);
+        bset_reduce(S, K);
+    }
+    const float U = __uint_as_float((uint32_t)(S.key[K - 1] >> 32));
+    __syncthreads();
+
+    // pass bits: far child allowed iff (q[axis] - split)^2 <= U
+    for (int base = 0; base < nn; base += kBlock) {
+        const int n = base + tid;
+        const int a = n < nn ? t.axis[n] : 0;
+        const float sv = n < nn ? t.split[n] : 0.0f;
+        const float diff = __shfl(qlane, a, kWave) - sv;
+        const unsigned long long b = __ballot(n < nn && diff * diff <= U);
+        const int wb = base + wid * kWave;
+        if (lane == 0 && wb < nn) {
+            passb[wb >> 5] = (uint32_t)b;
+            passb[(wb >> 5) + 1] = (uint32_t)(b >> 32);
+        }
+    }
+    // the reference's initial list: K sentinels, arrivals 0..K-1
+    const int pl0 = propleaf ? propleaf[0] : 0;
+    if (tid < K) {
+        S.key[tid] = make_key(kFltMax, (uint32_t)tid);
+        S.pos[tid] = 0;
+        S.pl[tid] = pl0;
+    }
+    if (tid == 0) { *S.count = K; *qn_p = 0; }
+    __syncthreads();
+    const uint64_t ubound = (U < kFltMax) ? (((uint64_t)__float_as_uint(U) << 32) | 0xFFFFFFFFull) : ~0ull;
+    uint64_t bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
+    const uint32_t seq0 = (uint32_t)K;
+
+    // scan of the queued leaves: wavefront wid takes leaves wid, wid+4, ...;
+    // each lane keeps up to kBatch points in flight
+    auto scan_queue = [&](int count) {
+        for (int j0 = 0; j0 < count; j0 += 4 * (kBatch / 2)) {
+            float pt[kBatch][DP];
+            int pos[kBatch];
+            bool val[kBatch];
+            uint32_t sq[kBatch];
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                const int j = j0 + wid + 4 * (u >> 1);
+                const int off = lane + (u & 1) * kWave;
+                const bool jv = j < count;
+                const int f = jv ? qfrom[j] : 0, c = jv ? qcnt[j] : 0;
+                const uint32_t rk = jv ? ((uint32_t)qrank[j] << offbits) : 0u;
+                val[u] = off < c;
+                pos[u] = f + (val[u] ? off : 0);
+                sq[u] = seq0 + (rk | (uint32_t)off);
+                if (val[u]) load_vec<DP>(t.pts + (size_t)pos[u] * DP, pt[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < kBatch; ++u) {
+                float s2 = 0.0f;
+                if (val[u]) {
+#pragma unroll
+                    for (int jj = 0; jj < DP; ++jj) {
+                        const float dd = pt[u][jj] - qv[jj];
+                        s2 = s2 + dd * dd;
+                    }
+                }
+                const uint64_t key = make_key(s2, sq[u]);
+                const bool take = val[u] && key < bkey;
+                bset_append(S, take, key, pos[u], (take && propleaf) ? propleaf[pos[u]] : 0);
+            }
+            __syncthreads();
+            if (*S.count > kSetCap - kBatch * kBlock) {
+                bset_reduce(S, K);
+                bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
+            }
+            // leaves longer than 2 * 64 points continue here
+            for (int ob = 2 * kWave; ob < t.max_leaf; ob += 2 * kWave) {
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
+                    const int j = j0 + wid + 4 * (u >> 1);
+                    const int off = ob + lane + (u & 1) * kWave;
+                    const bool jv = j < count;
+                    const int f = jv ? qfrom[j] : 0, c = jv ? qcnt[j] : 0;
+                    const uint32_t rk = jv ? ((uint32_t)qrank[j] << offbits) : 0u;
+                    const bool v = off < c;
+                    float s2 = 0.0f;
+                    const int pp = f + (v ? off : 0);
+                    if (v) s2 = dist_sq<DP>(t.pts + (size_t)pp * DP, qv);
+                    const uint64_t key = make_key(s2, seq0 + (rk | (uint32_t)off));
+                    const bool take = v && key < bkey;
+                    bset_append(S, take, key, pp, (take && propleaf) ? propleaf[pp] : 0);
+                }
+                __syncthreads();
+                if (*S.count > kSetCap - kBatch * kBlock) {
+                    bset_reduce(S, K);
+                    bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
+                }
+            }
+        }
+    };
+    for (int lb = 0; lb < nleaves; lb += kBlock) {
+        const int l = lb + tid;
+        bool ok = l < nleaves;
+        int rank = 0, nd = 0, lo = 0, hi = t.n;
+        for (int lev = t.depth - 1; lev >= 0; --lev) {
+            const int dir = (l >> lev) & 1;
+            const int nr = (int)((nearb[nd >> 5] >> (nd & 31)) & 1u);
+            const int pb = (int)((passb[nd >> 5] >> (nd & 31)) & 1u);
+            const int far = dir ^ nr;
+            ok = ok && (!far || pb);
+            rank = (rank << 1) | far;
+            nd = 2 * nd + 1 + dir;
+            const int mid = lo + ((hi - lo) >> 1);
+            if (dir) lo = mid; else hi = mid;
+        }
+        const unsigned long long m = __ballot(ok);
+        const int cnt = __popcll(m);
+        int base = 0;
+        if (lane == 0 && cnt) base = atomicAdd(qn_p, cnt);
+        base = __builtin_amdgcn_readlane(base, 0);
+        if (ok) {
+            const int slot = base + lane_rank(m);
+            qfrom[slot] = lo;
+            qcnt[slot] = hi - lo;
+            qrank[slot] = rank;
+        }
+        __syncthreads();
+        const int qn = *qn_p;
+        if (qn > kExactQueue - kBlock) {
+            scan_queue(qn);
+            __syncthreads();
+            if (tid == 0) *qn_p = 0;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    scan_queue(*qn_p);
+    bset_reduce(S, K);
+    if (tid < K) {
+        od[(size_t)w * K + tid] = __uint_as_float((uint32_t)(S.key[tid] >> 32));
+        op[(size_t)w * K + tid] = S.pos[tid];
+        if (ol) ol[(size_t)w * K + tid] = S.pl[tid];
+    }
+}
+
 hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long long max_leaves,
                         const int *propleaf, float *out_d, int *out_p, int *out_l, hipStream_t st) {
     if (nq == 0) return hipSuccess;
+    // the bounded scan needs an unlimited visit budget (a finite max_leaves
+    // stops the reference's traversal part-way) and keys that fit 32 bits
+    int offbits = 0;
+    while ((1 << offbits) <= t.max_leaf) ++offbits;
+    const bool bounded = max_leaves >= (1LL << t.depth) && t.depth <= 15 && t.depth + offbits <= 30 &&
+                         K <= 32;
+    if (bounded) {
+        const int nn = (1 << t.depth) - 1;
+        const int words = 2 * ((nn + kWave - 1) / kWave) + 2;
+        const size_t lds = (size_t)(kSetWords + 2 * words + 3 * kExactQueue + 1) * sizeof(uint32_t);
+        if (lds > 160 * 1024) return hipErrorInvalidValue;
+        return with_dp(t.dpad, [&](auto dp) {
+            constexpr int DP = decltype(dp)::value;
+            hipError_t e = hipFuncSetAttribute((const void *)k_exact_bound<DP>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_exact_bound<DP>), dim3(nq), dim3(kBlock), lds, st, t, q, nq, K, propleaf, out_d,
+                               out_p, out_l, offbits, words);
+            return hipGetLastError();
+        });
+    }
     return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
         constexpr int DP = decltype(dp)::value;
         constexpr int KM = decltype(km)::value;
@@ -868,72 +1326,229 @@ hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long l
 // propagation (rows >= 1)
 // ---------------------------------------------------------------------------
 
-// propleaf[pos] = leaf of the patch one row below the B patch at tree
-// position pos (base.c:265-276): o = orig[pos]; o += n_cols if o + n_cols <
-// n_a; leaf = pli[o].
-__global__ void k_propleaf(const int *orig, const int *pli, int n_b, int n_a, int n_cols, int *out) {
+// propleaf[pos] describes the leaf of the patch one row below the B patch
+// at tree position pos (base.c:265-276: o = orig[pos]; o += n_cols if
+// o + n_cols < n_a; leaf = pli[o]).  Every leaf holds m = n >> depth or m + 1
+// points, so the leaf is stored as its range: (from << 1) | (size - m).  The
+// range identifies the leaf, and the row sweep needs no lookup to scan it.
+__global__ void k_propleaf(const int *orig, const int *pli, const int *leaves, int n_b, int n_a, int n_cols, int m,
+                           int *out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_b) return;
     int o = orig[i];
     if (o + n_cols < n_a) o += n_cols;
-    out[i] = pli[o];
+    const int l = pli[o];
+    const int from = leaves[2 * l];
+    out[i] = (from << 1) | (leaves[2 * l + 1] - from - m);
 }
 
 hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hipStream_t st) {
-    hipLaunchKernelGGL(k_propleaf, dim3(blocks_for(t.n, 256)), dim3(256), 0, st, t.orig, t.pli, t.n, n_a, n_cols,
-                       propleaf);
+    hipLaunchKernelGGL(k_propleaf, dim3(blocks_for(t.n, 256)), dim3(256), 0, st, t.orig, t.pli, t.leaves, t.n, n_a,
+                       n_cols, t.n >> t.depth, propleaf);
     return hipGetLastError();
 }
 
 // One wavefront per image column, walking rows 1..n_rows-1.  The list of
-// row y starts from the natural-leaf list (arrivals 0..K-1) and then takes
-// the candidates of the K propagated leaves in order (base.c:255-304).
-template <int DP, int KM>
-__global__ __launch_bounds__(256) void k_sweep(TreeView t, const float *__restrict__ q, int n_rows, int n_cols, int K,
-                                               const int *__restrict__ propleaf, const float *__restrict__ p1d,
-                                               const int *__restrict__ p1p, const int *__restrict__ p1l,
-                                               int *__restrict__ fl_pos, const int *__restrict__ row0_l, int stride) {
-    const int x = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * (long long)blockDim.x + threadIdx.x) / kWave));
-    const int lane = threadIdx.x & (kWave - 1);
+// row y starts from the natural-leaf list (arrivals 1..K) and then takes the
+// candidates of the K propagated leaves in order (base.c:255-304), with
+// arrival 1 + K + k * stride + offset for leaf k.  Three exact shortcuts keep
+// the work per row to about two leaves:
+//   * only candidates strictly closer than the current K-th distance can
+//     enter (an equal distance arrives later and loses), so they are filtered
+//     in registers and the survivors merged in the wavefront's LDS set;
+//   * a leaf listed several times is scored once and every surviving point
+//     is appended once per occurrence, each with its own arrival (the
+//     reference's duplicates);
+//   * a propagated leaf equal to the query's natural leaf is not rescanned:
+//     its points closer than the natural list's K-th distance are exactly
+//     the natural list's entries, which are re-appended with that leaf's
+//     arrivals.
+// The only dependent memory round trip per row is the candidates' load:
+// leaf ranges are computed (leaf_range) and the next row's query, natural
+// leaf and natural list are requested while the current row is scored.
+constexpr int kSweepCap = 32 + 4 * kWave;
+
+template <int DP>
+__global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__restrict__ q, int n_rows, int n_cols,
+                                                 int K, const int *__restrict__ propleaf,
+                                                 const float *__restrict__ p1d, const int *__restrict__ p1p,
+                                                 const int *__restrict__ p1l, const int *__restrict__ natleaf,
+                                                 int *__restrict__ fl_pos, const int *__restrict__ row0_l, int stride) {
+    constexpr int UNR = 4;
+    extern __shared__ uint32_t smem_w[];
+    const int lane = threadIdx.x;
+    const BlockSet S = bset_at(smem_w, kSweepCap);
+    const int x = blockIdx.x;
     if (x >= n_cols) return;
+    const int mlo = t.n >> t.depth;
+    // natural leaf as a packed range (from << 1) | (size - mlo)
+    auto pack_leaf = [&](int l) {
+        int fr, to;
+        leaf_range(l, t.n, t.depth, fr, to);
+        return (fr << 1) | (to - fr - mlo);
+    };
     int my_l = (lane < K) ? row0_l[(size_t)x * K + lane] : 0;
+    float qv[DP];
+    float cd = 0.0f;
+    int cp = 0, cl = 0, nat = 0;
+    if (n_rows > 1) {
+        const size_t qi = (size_t)n_cols + x;
+        load_vec<DP>(q + qi * DP, qv);
+        nat = pack_leaf(natleaf[qi]);
+        if (lane < K) { cd = p1d[qi * K + lane]; cp = p1p[qi * K + lane]; cl = p1l[qi * K + lane]; }
+    }
     for (int y = 1; y < n_rows; ++y) {
         const size_t qi = (size_t)y * n_cols + x;
-        float qv[DP];
-        load_vec<DP>(q + qi * DP, qv);
-        LaneList<KM> L;
-        L.reset(K);
-        if (lane < K)
-            L.insert(make_key(p1d[qi * K + lane], (uint32_t)(1 + lane)), p1p[qi * K + lane], p1l[qi * K + lane]);
-        for (int k = 0; k < K; ++k) {
-            const int leaf = readlane_i(my_l, k);
-            const int from = t.leaves[2 * leaf], cnt = t.leaves[2 * leaf + 1] - from;
-            const uint32_t seq0 = (uint32_t)(1 + K + k * stride);
-            for (int off = lane; off < cnt; off += kWave) {
-                const int pos = from + off;
-                const float dd = dist_sq<DP>(t.pts + (size_t)pos * DP, qv);
-                L.insert(make_key(dd, seq0 + (uint32_t)off), pos, propleaf[pos]);
+        // leaf k (lane k): occurrences, first occurrence, natural or not
+        uint32_t occ = 0;
+        bool first = true;
+        for (int j = 0; j < K; ++j) {
+            const int lj = readlane_i(my_l, j);
+            if (lane < K && lj == my_l) {
+                occ |= 1u << j;
+                if (j < lane) first = false;
             }
         }
-        float nd = 0.0f;
-        int np = 0, nl = 0;
-        wave_select<KM>(L, K, lane, nd, np, nl);
-        if (lane < K) {
-            fl_pos[qi * K + lane] = np;
-            my_l = nl;
+        const bool is_nat = lane < K && my_l == nat;
+        const bool uniq = lane < K && first && !is_nat;
+        const unsigned long long natmask = __ballot(is_nat);
+        const int f = uniq ? (my_l >> 1) : 0;
+        const int c = uniq ? (mlo + (my_l & 1)) : 0;
+        // chunks of 64 points per unique leaf, prefix over lanes
+        const int ch = uniq ? (c + kWave - 1) / kWave : 0;
+        int inc = ch;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int v = __shfl_up(inc, o, kWave);
+            if (lane >= o) inc += v;
         }
+        const int chpre = inc - ch;
+        const int nch = readlane_i(inc, 31);
+        const int nat_from = nat >> 1;
+        // the natural list as the first K items
+        if (lane < K) {
+            S.key[lane] = make_key(cd, (uint32_t)(1 + lane));
+            S.pos[lane] = cp;
+            S.pl[lane] = cl;
+        }
+        int n = K;
+        float bound = readlane_f(cd, K - 1);
+        // next row's inputs
+        float nqv[DP];
+        float nd0 = 0.0f;
+        int np0 = 0, nl0 = 0, nnat = 0;
+        const bool more = y + 1 < n_rows;
+        bool fetched = false;
+        bool touched = false;   // any append this row (the list may have changed)
+        for (int c0 = 0; c0 < nch; c0 += UNR) {
+            float pt[UNR][DP];
+            int pos[UNR], off[UNR], own[UNR], plc[UNR];
+            bool val[UNR];
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                const int cc = c0 + u;
+                // owner lane of chunk cc (uniform)
+                int o = 0;
+                for (int k = 0; k < K; ++k) {
+                    const int pk = readlane_i(chpre, k), ck = readlane_i(ch, k);
+                    if (cc >= pk && cc < pk + ck) o = k;
+                }
+                own[u] = o;
+                const int fo = readlane_i(f, o), co = readlane_i(c, o), po = readlane_i(chpre, o);
+                off[u] = (cc - po) * kWave + lane;
+                val[u] = cc < nch && off[u] < co;
+                pos[u] = val[u] ? fo + off[u] : 0;
+                plc[u] = 0;
+                if (val[u]) {
+                    load_vec<DP>(t.pts + (size_t)pos[u] * DP, pt[u]);
+                    plc[u] = propleaf[pos[u]];
+                }
+            }
+            if (!fetched && more) {
+                const size_t nqi = qi + n_cols;
+                load_vec<DP>(q + nqi * DP, nqv);
+                nnat = natleaf[nqi];
+                if (lane < K) { nd0 = p1d[nqi * K + lane]; np0 = p1p[nqi * K + lane]; nl0 = p1l[nqi * K + lane]; }
+                fetched = true;
+            }
+#pragma unroll
+            for (int u = 0; u < UNR; ++u) {
+                float s2 = 0.0f;
+                if (val[u]) {
+#pragma unroll
+                    for (int jj = 0; jj < DP; ++jj) {
+                        const float dd = pt[u][jj] - qv[jj];
+                        s2 = s2 + dd * dd;
+                    }
+                }
+                const bool take = val[u] && s2 < bound;
+                if (__ballot(take) == 0ull) continue;
+                touched = true;
+                const int pl = plc[u];
+                uint32_t m = (uint32_t)readlane_i((int)occ, own[u]);
+                while (m) {
+                    const int k = __ffs(m) - 1;
+                    m &= m - 1;
+                    wset_append(S, n, take, make_key(s2, (uint32_t)(1 + K + k * stride + off[u])), pos[u], pl);
+                    if (n > kSweepCap - kWave) {
+                        int dp_ = 0, dl_ = 0;
+                        wset_reduce(S, n, K, lane, dp_, dl_);
+                        bound = __uint_as_float((uint32_t)(S.key[K - 1] >> 32));
+                    }
+                }
+            }
+        }
+        if (!fetched && more) {
+            const size_t nqi = qi + n_cols;
+            load_vec<DP>(q + nqi * DP, nqv);
+            nnat = natleaf[nqi];
+            if (lane < K) { nd0 = p1d[nqi * K + lane]; np0 = p1p[nqi * K + lane]; nl0 = p1l[nqi * K + lane]; }
+        }
+        // propagated leaves equal to the natural leaf: re-append the natural
+        // list's entries below the bound with those leaves' arrivals
+        {
+            uint32_t m = (uint32_t)natmask;
+            while (m) {
+                const int k = __ffs(m) - 1;
+                m &= m - 1;
+                const bool take = lane < K && cd < bound;
+                if (__ballot(take) == 0ull) continue;
+                touched = true;
+                wset_append(S, n, take, make_key(cd, (uint32_t)(1 + K + k * stride + (cp - nat_from))), cp, cl);
+                if (n > kSweepCap - kWave) {
+                    int dp_ = 0, dl_ = 0;
+                    wset_reduce(S, n, K, lane, dp_, dl_);
+                    bound = __uint_as_float((uint32_t)(S.key[K - 1] >> 32));
+                }
+            }
+        }
+        int rpos = cp, rl = cl;     // unchanged list when nothing was appended
+        if (touched) wset_reduce(S, n, K, lane, rpos, rl);
+        if (lane < K) {
+            fl_pos[qi * K + lane] = rpos;
+            my_l = rl;
+        }
+        if (more) {
+#pragma unroll
+            for (int j = 0; j < DP; ++j) qv[j] = nqv[j];
+            cd = nd0;
+            cp = np0;
+            cl = nl0;
+            nat = pack_leaf(nnat);
+        }
+        wave_sync();
     }
 }
 
 hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K, const int *propleaf,
-                     const float *p1d, const int *p1p, const int *p1l, int *fl_pos, const int *row0_l,
-                     hipStream_t st) {
+                     const float *p1d, const int *p1p, const int *p1l, const int *natleaf, int *fl_pos,
+                     const int *row0_l, hipStream_t st) {
     if (n_rows < 2) return hipSuccess;
-    return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
+    const size_t lds = (size_t)set_words(kSweepCap) * sizeof(uint32_t);
+    return with_dp(t.dpad, [&](auto dp) {
         constexpr int DP = decltype(dp)::value;
-        constexpr int KM = decltype(km)::value;
-        hipLaunchKernelGGL((k_sweep<DP, KM>), dim3(blocks_for((long long)n_cols * kWave, 256)), dim3(256), 0, st, t,
-                           q, n_rows, n_cols, K, propleaf, p1d, p1p, p1l, fl_pos, row0_l, t.max_leaf);
+        hipLaunchKernelGGL((k_sweep<DP>), dim3(n_cols), dim3(kWave), lds, st, t, q, n_rows, n_cols, K, propleaf, p1d,
+                           p1p, p1l, natleaf, fl_pos, row0_l, t.max_leaf);
         return hipGetLastError();
     });
 }
@@ -942,18 +1557,18 @@ hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, 
 // index conversion + best-of-K (base.c:309-353, norm_array base.c:362-374)
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ const uint8_t *patch_ptr(const PatchGeom &g, long long i) {
+__device__ __forceinline__ long long patch_off(const PatchGeom &g, long long i) {
     if (g.image) {
         long long y = i / g.ncols, x = i - y * g.ncols;
-        return g.base + (y * g.width + x) * 3;
+        return (y * g.width + x) * 3;
     }
-    return g.base + i * (long long)g.rowlen;
+    return i * (long long)g.rowlen;
 }
 
-// sqrt of the sequential FP32 sum of ((float)a - (float)b)^2.
-__device__ __forceinline__ float patch_l2(const PatchGeom &ga, long long i, const PatchGeom &gb, long long j) {
-    const uint8_t *a = patch_ptr(ga, i);
-    const uint8_t *b = patch_ptr(gb, j);
+// sqrt of the sequential FP32 sum of ((float)a - (float)b)^2 (norm_array).
+__device__ __forceinline__ float patch_l2_fp32(const PatchGeom &ga, long long i, const PatchGeom &gb, long long j) {
+    const uint8_t *a = ga.base + patch_off(ga, i);
+    const uint8_t *b = gb.base + patch_off(gb, j);
     float s = 0.0f;
     for (int r = 0; r < ga.rows; ++r) {
         const uint8_t *ra = a + r * ga.rstride;
@@ -966,6 +1581,48 @@ __device__ __forceinline__ float patch_l2(const PatchGeom &ga, long long i, cons
     return sqrt_rn(s);
 }
 
+// 4 bytes at byte offset off of a buffer of nbytes (any alignment); bytes
+// past the end read as 0 and nothing past the end is loaded.
+__device__ __forceinline__ uint32_t load4(const uint8_t *base, long long off, long long nbytes) {
+    const long long al = off & ~3LL;
+    const int sh = (int)(off & 3);
+    if (al + 8 <= nbytes) {
+        const uint32_t lo = *reinterpret_cast<const uint32_t *>(base + al);
+        const uint32_t hi = *reinterpret_cast<const uint32_t *>(base + al + 4);
+        return __builtin_amdgcn_alignbyte(hi, lo, sh);
+    }
+    uint32_t v = 0;
+    for (int k = 0; k < 4; ++k)
+        if (off + k < nbytes) v |= (uint32_t)base[off + k] << (8 * k);
+    return v;
+}
+
+// Exact integer sums over one patch pair: sum a^2, sum b^2, sum a*b with
+// 4-byte dot products (v_dot4_u32_u8); the last word of a row is masked.
+__device__ __forceinline__ void patch_dots(const PatchGeom &ga, long long oa, const PatchGeom &gb, long long ob,
+                                           uint32_t &saa, uint32_t &sbb, uint32_t &sab) {
+    const int nw = (ga.rowlen + 3) >> 2;
+    const int tail = ga.rowlen & 3;
+    const uint32_t tmask = tail ? ((1u << (8 * tail)) - 1u) : 0xFFFFFFFFu;
+    for (int r = 0; r < ga.rows; ++r) {
+        const long long ra = oa + r * ga.rstride, rb = ob + r * gb.rstride;
+        for (int wd = 0; wd < nw; ++wd) {
+            uint32_t wa = load4(ga.base, ra + 4 * wd, ga.nbytes);
+            uint32_t wb = load4(gb.base, rb + 4 * wd, gb.nbytes);
+            if (wd == nw - 1) { wa &= tmask; wb &= tmask; }
+            saa = __builtin_amdgcn_udot4(wa, wa, saa, false);
+            sbb = __builtin_amdgcn_udot4(wb, wb, sbb, false);
+            sab = __builtin_amdgcn_udot4(wa, wb, sab, false);
+        }
+    }
+}
+
+// Index conversion + best-of-K (base.c:309-353).  Each candidate's squared
+// distance is the exact integer S = sum a^2 + sum b^2 - 2 sum a*b.  While S
+// < 2^24 every partial sum of the reference's sequential FP32 loop is an
+// integer below 2^24, so that loop returns S exactly; larger sums take the
+// FP32 loop itself.  The comparison is made on the correctly rounded roots,
+// as in the reference (two sums can share a root).
 __global__ __launch_bounds__(256) void k_best(PatchGeom ga, PatchGeom gb, const int *__restrict__ fl_pos,
                                               const int *__restrict__ orig, int nq, int K, int sel,
                                               int *__restrict__ fin, float *__restrict__ fdist, int *__restrict__ knn) {
@@ -975,17 +1632,18 @@ __global__ __launch_bounds__(256) void k_best(PatchGeom ga, PatchGeom gb, const 
     if (knn) {
         for (int k = 0; k < K; ++k) knn[(size_t)i * K + k] = orig[lp[k]];
     }
-    if (!sel) {
-        const int j = orig[lp[0]];
-        fin[i] = j;
-        if (fdist) fdist[i] = patch_l2(ga, i, gb, j);
-        return;
-    }
+    const long long oa = patch_off(ga, i);
+    const int kk = sel ? K : 1;
     int best = -1;
     float bd = kFltMax;
-    for (int k = 0; k < K; ++k) {
+    for (int k = 0; k < kk; ++k) {
         const int j = orig[lp[k]];
-        const float dd = patch_l2(ga, i, gb, j);
+        if (!sel && !fdist) { best = j; break; }
+        uint32_t saa = 0, sbb = 0, sab = 0;
+        patch_dots(ga, oa, gb, patch_off(gb, j), saa, sbb, sab);
+        const uint32_t S = saa + sbb - 2u * sab;
+        const float dd = (S < (1u << 24)) ? sqrt_rn((float)S) : patch_l2_fp32(ga, i, gb, j);
+        if (!sel) { best = j; bd = dd; break; }
         if (dd < bd) { bd = dd; best = j; }
     }
     fin[i] = best;

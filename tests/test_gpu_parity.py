@@ -169,3 +169,44 @@ def test_field_1080p_matches_oracle(gpu_available):
     from annfmp_amd import synth
     a, b = synth.make_pair("1080p-warp", seed=0)
     _field_vs_oracle(a, b, 11, 16, 64, 5, projection="fp32")
+
+
+@pytest.mark.parametrize("K", [2, 8, 20])
+def test_exact_search_ties_match_oracle(K, gpu_available):
+    """Many equal distances across leaves: integer points with duplicates and
+    integer queries; the tie order must be the reference's visit order."""
+    rng = np.random.default_rng(40 + K)
+    X = rng.integers(-3, 4, (30000, 3)).astype(np.float32)
+    Q = rng.integers(-3, 4, (400, 3)).astype(np.float32)
+    depth = O.tree_depth(30000, 16)
+    ref = O.OracleTree(X, depth)
+    t = _fit(X, depth, K)
+    d, idx = t.kneighbors(Q)
+    rd, ridx = ref.neighbors(Q, K)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(bits(d), bits(rd))
+
+
+def test_process_rows_flat_regions_match_oracle(gpu_available):
+    """A pair with large flat areas (many queries share a natural leaf and
+    many distances tie) through the reference-shaped process_rows entry."""
+    from annfmp_amd import synth
+    a, b = synth.make_pair(synth.PairSpec(160, 200, 12, 1.0, grid=(4, 4), max_offset=8), seed=9)
+    a[20:120, 30:150] = 77          # one large noiseless block in A
+    p, d, leaf, K = 5, 6, 16, 5
+    pa = O.extract_patches(a, p)
+    pb = O.extract_patches(b, p)
+    X = np.concatenate([pa[::5], pb[::5]]).astype(np.float64)
+    mu = X.mean(0)
+    _, V = np.linalg.eigh(np.cov(X.T))
+    comp = V[:, ::-1][:, :d].T.copy()
+    ra = O.project_f64(pa, comp, mu)
+    rb = O.project_f64(pb, comp, mu)
+    nr, nc = a.shape[0] - p + 1, a.shape[1] - p + 1
+    depth = O.tree_depth(rb.shape[0], leaf)
+    ref = O.OracleTree(rb, depth)
+    want, _, _ = ref.process_rows(ra, nr, nc, K, True, True, pa, pb)
+    t = _fit(rb, depth, K)
+    nn = np.zeros(ra.shape[0], np.int32)
+    t.process_rows(nn, pa, pb, ra, nr, nc, K, True, True)
+    assert np.array_equal(nn, want)

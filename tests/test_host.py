@@ -114,3 +114,23 @@ def test_warp_pair_block_offsets():
     b, offs = synth._warp(a, spec, rng)
     dy, dx = offs[1, 2]
     assert np.array_equal(b[30:60, 60:90], a[(np.arange(30, 60) + dy) % 90][:, (np.arange(60, 90) + dx) % 120])
+
+
+def test_leaf_range_formula_matches_oracle_tree():
+    """The kernels compute a leaf's [from, to) from (n, depth) alone: each
+    segment splits at floor(c/2).  Check that formula against built trees."""
+    def leaf_range(l, n, depth):
+        lo, hi = 0, n
+        for lev in range(depth - 1, -1, -1):
+            mid = lo + (hi - lo) // 2
+            if (l >> lev) & 1:
+                lo = mid
+            else:
+                hi = mid
+        return lo, hi
+    rng = np.random.default_rng(0)
+    for n, leaf in [(777, 5), (100003, 64), (62500, 32), (4099, 2)]:
+        depth = O.tree_depth(n, leaf)
+        t = O.OracleTree(rng.standard_normal((n, 2)).astype(np.float32), depth)
+        for l in range(1 << depth):
+            assert leaf_range(l, n, depth) == (t.leaves[2 * l], t.leaves[2 * l + 1])
