@@ -84,7 +84,7 @@ struct annf_ctx {
     // build scratch
     DevBuf keys, eqoff, box, nax, prefix, rank, hist, cnt, tot, bnd, errflag;
     // query workspaces
-    DevBuf qbuf, natleaf, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf;
+    DevBuf qbuf, natleaf, natdesc, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
     DevBuf basis32, mean32, basis64, mean64;
     // timing
@@ -117,7 +117,7 @@ struct annf_ctx {
 
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
-                          &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &bcount, &boffset,
+                          &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64};
         for (DevBuf *b : list) b->release();
@@ -266,7 +266,13 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
         RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
         HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
-        HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), c->stream));
+        int *natdesc = nullptr;
+        if (propagate) {
+            RCHK(c->natdesc.ensure((size_t)n_a * sizeof(int)));
+            natdesc = c->natdesc.as<int>();
+        }
+        HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), natdesc,
+                              c->stream));
         HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
                               c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), c->stream));
         if (propagate) {
@@ -290,7 +296,7 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
     c->mark(5);
     if (propagate && n_rows > 1) {
         HIPCHK(row_sweep(t, c->qbuf.as<float>(), n_rows, n_cols, K, propleaf, c->p1d.as<float>(),
-                         c->p1p.as<int>(), c->p1l.as<int>(), c->natleaf.as<int>(), c->fl_pos.as<int>(),
+                         c->p1p.as<int>(), c->p1l.as<int>(), c->natdesc.as<int>(), c->fl_pos.as<int>(),
                          c->fl_l.as<int>(), c->stream));
     }
     c->mark(6);

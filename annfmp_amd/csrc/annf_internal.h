@@ -66,7 +66,9 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
                       TreeView &t, hipStream_t st);
 
-hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count,
+// Natural leaf of every query (leaf), optional per-leaf counts, and optional
+// packed leaf ranges (desc: (from << 1) | (size - (n >> depth))).
+hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, int *desc,
                           hipStream_t st);
 // Buckets queries by natural leaf; `count` must hold the per-leaf counts and
 // is overwritten with chunks per leaf; chunk_off receives nleaves+1 offsets.

@@ -589,7 +589,8 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
 // kdtree.c:198-214 with only_leaf_idx = 1: right iff q[axis] - split >= 0.
 template <int DP>
 __global__ __launch_bounds__(256) void k_natural(const float *__restrict__ q, int nq, const int *__restrict__ axis,
-                                                 const float *__restrict__ split, int depth, int *leaf, int *count) {
+                                                 const float *__restrict__ split, const int *__restrict__ leaves,
+                                                 int depth, int m, int *leaf, int *count, int *desc) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
     const float *qr = q + (size_t)i * DP;
@@ -600,15 +601,21 @@ __global__ __launch_bounds__(256) void k_natural(const float *__restrict__ q, in
     }
     int l = idx - ((1 << depth) - 1);
     leaf[i] = l;
+    if (desc) {
+        // the leaf as its packed range (from << 1) | (size - m), as k_propleaf
+        const int from = leaves[2 * l];
+        desc[i] = (from << 1) | (leaves[2 * l + 1] - from - m);
+    }
     if (count) (void)wave_add1(count, l);
 }
 
-hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, hipStream_t st) {
+hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, int *desc,
+                          hipStream_t st) {
     if (nq == 0) return hipSuccess;
     return with_dp(t.dpad, [&](auto dp) {
         constexpr int DP = decltype(dp)::value;
         hipLaunchKernelGGL(k_natural<DP>, dim3(blocks_for(nq, 256)), dim3(256), 0, st, q, nq, t.axis, t.split,
-                           t.depth, leaf, count);
+                           t.leaves, t.depth, t.n >> t.depth, leaf, count, desc);
         return hipGetLastError();
     });
 }
@@ -1381,12 +1388,6 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
     const int x = blockIdx.x;
     if (x >= n_cols) return;
     const int mlo = t.n >> t.depth;
-    // natural leaf as a packed range (from << 1) | (size - mlo)
-    auto pack_leaf = [&](int l) {
-        int fr, to;
-        leaf_range(l, t.n, t.depth, fr, to);
-        return (fr << 1) | (to - fr - mlo);
-    };
     int my_l = (lane < K) ? row0_l[(size_t)x * K + lane] : 0;
     float qv[DP];
     float cd = 0.0f;
@@ -1394,7 +1395,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
     if (n_rows > 1) {
         const size_t qi = (size_t)n_cols + x;
         load_vec<DP>(q + qi * DP, qv);
-        nat = pack_leaf(natleaf[qi]);
+        nat = natleaf[qi];
         if (lane < K) { cd = p1d[qi * K + lane]; cp = p1p[qi * K + lane]; cl = p1l[qi * K + lane]; }
     }
     for (int y = 1; y < n_rows; ++y) {
@@ -1425,6 +1426,12 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
         const int chpre = inc - ch;
         const int nch = readlane_i(inc, 31);
         const int nat_from = nat >> 1;
+        // owner (lane k of a unique leaf) of chunk j, held by lane j
+        int own_l = 0;
+        for (int k = 0; k < K; ++k) {
+            const int pk = readlane_i(chpre, k), ck = readlane_i(ch, k);
+            if (lane >= pk && lane < pk + ck) own_l = k;
+        }
         // the natural list as the first K items
         if (lane < K) {
             S.key[lane] = make_key(cd, (uint32_t)(1 + lane));
@@ -1448,10 +1455,12 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
             for (int u = 0; u < UNR; ++u) {
                 const int cc = c0 + u;
                 // owner lane of chunk cc (uniform)
-                int o = 0;
-                for (int k = 0; k < K; ++k) {
-                    const int pk = readlane_i(chpre, k), ck = readlane_i(ch, k);
-                    if (cc >= pk && cc < pk + ck) o = k;
+                int o = readlane_i(own_l, cc & (kWave - 1));
+                if (cc >= kWave) {
+                    for (int k = 0; k < K; ++k) {
+                        const int pk = readlane_i(chpre, k), ck = readlane_i(ch, k);
+                        if (cc >= pk && cc < pk + ck) o = k;
+                    }
                 }
                 own[u] = o;
                 const int fo = readlane_i(f, o), co = readlane_i(c, o), po = readlane_i(chpre, o);
@@ -1534,7 +1543,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
             cd = nd0;
             cp = np0;
             cl = nl0;
-            nat = pack_leaf(nnat);
+            nat = nnat;
         }
         wave_sync();
     }
