@@ -87,6 +87,7 @@ struct annf_ctx {
     DevBuf qbuf, natleaf, natdesc, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
     DevBuf basis32, mean32, basis64, mean64;
+    DevBuf ev_am, ev_an, ev_bm, ev_bn, ev_part;
     // timing
     bool timing = false;
     hipEvent_t ev[kStages + 1] = {};
@@ -119,7 +120,8 @@ struct annf_ctx {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
                           &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
-                          &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64};
+                          &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
+                          &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part};
         for (DevBuf *b : list) b->release();
         has_tree = false;
     }
@@ -614,6 +616,35 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
     RCHK(check_build_flag(c));
     c->collect_marks();
     return ANNF_OK;
+}
+
+int annf_exact_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
+                         int psize, const int *query_idx, int nq, int k, int *out_idx, int *out_d2) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!img_a || !img_b || (nq > 0 && (!query_idx || !out_idx || !out_d2))) return fail(ANNF_EINVAL, "null pointer");
+    RCHK(set_device(c));
+    const int p = psize;
+    if (p < 1 || ha < p || wa < p || hb < p || wb < p) return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
+    if (k < 1 || k > 32) return fail(ANNF_EUNSUPPORTED, "k %d outside [1, 32]", k);
+    const int D = p * p * 3, dpad_b = (D + 3) & ~3;
+    if (dpad_b / 4 > 128) return fail(ANNF_EUNSUPPORTED, "patch of %d bytes: at most 512 supported", D);
+    const long long n_a = (long long)(ha - p + 1) * (wa - p + 1), n_b = (long long)(hb - p + 1) * (wb - p + 1);
+    if (n_b > 0x7FFFFFFFLL || n_a > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "too many patches");
+    if ((long long)k > n_b) return fail(ANNF_EINVAL, "k %d exceeds the %lld B patches", k, n_b);
+    RCHK(c->ev_am.ensure((size_t)n_a * dpad_b));
+    RCHK(c->ev_an.ensure((size_t)n_a * sizeof(uint32_t)));
+    RCHK(c->ev_bm.ensure((size_t)n_b * dpad_b));
+    RCHK(c->ev_bn.ensure((size_t)n_b * sizeof(uint32_t)));
+    const int splits = exact_splits(n_b);
+    RCHK(c->ev_part.ensure((size_t)(nq > 0 ? nq : 1) * splits * k * sizeof(uint64_t)));
+    HIPCHK(patch_matrix(img_a, ha, wa, p, c->ev_am.as<uint8_t>(), dpad_b, c->stream));
+    HIPCHK(patch_norms(c->ev_am.as<uint8_t>(), n_a, dpad_b, c->ev_an.as<uint32_t>(), c->stream));
+    HIPCHK(patch_matrix(img_b, hb, wb, p, c->ev_bm.as<uint8_t>(), dpad_b, c->stream));
+    HIPCHK(patch_norms(c->ev_bm.as<uint8_t>(), n_b, dpad_b, c->ev_bn.as<uint32_t>(), c->stream));
+    HIPCHK(exact_sample(c->ev_am.as<uint8_t>(), c->ev_an.as<uint32_t>(), query_idx, nq, c->ev_bm.as<uint8_t>(),
+                        c->ev_bn.as<uint32_t>(), n_b, dpad_b, k, c->ev_part.as<uint64_t>(), out_idx, out_d2,
+                        c->stream));
+    return sync_stream(c);
 }
 
 int annf_set_timing(annf_ctx *c, int on) {

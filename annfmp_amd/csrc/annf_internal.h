@@ -93,6 +93,14 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
                            const float *mean_f, const double *basis_t64, const double *mean64,
                            int d, int dpad, int fp64, float *out, hipStream_t st);
 
+// exact field (annf_eval.hip)
+hipError_t patch_matrix(const uint8_t *img, int h, int w, int p, uint8_t *out, int dpad_b, hipStream_t st);
+hipError_t patch_norms(const uint8_t *m, long long n, int dpad_b, uint32_t *nrm, hipStream_t st);
+int exact_splits(long long nb);
+hipError_t exact_sample(const uint8_t *am, const uint32_t *an, const int *qidx, int nq, const uint8_t *bm,
+                        const uint32_t *bn, long long nb, int dpad_b, int k, uint64_t *part, int *idx, int *d2,
+                        hipStream_t st);
+
 int dpad_of(int d);
 bool dpad_supported(int dpad);
 bool k_supported(int K);

@@ -147,10 +147,17 @@ def main():
     value = n_a * args.steps / (t1 - t0)
     stage_ms = stage_sum / args.steps
 
-    # one more step that also returns the reduced arrays (parity leg)
-    step(red=True)
+    # one more step that also returns the reduced arrays and K lists
+    knn = torch.empty((n_a, K), dtype=torch.int32, device=cuda)
+    _lib.check("annf_field_dev", L.annf_field_dev(
+        h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, 0, depth, K, 1, 1,
+        dev.dptr(nn), None, dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
     torch.cuda.synchronize()
     gpu_nn = nn.cpu().numpy()
+    from annfmp_amd import metrics
+    tq = time.perf_counter()
+    quality = metrics.field_quality(a, b, p, gpu_nn, knn.cpu().numpy(), k=5, n_sample=8192, handle=h)
+    quality["eval_s"] = round(time.perf_counter() - tq, 2)
 
     dom = int(np.argmax([stage_ms[i] if STAGES[i] != "tree_build" else 0 for i in range(7)]))
     dom_name = STAGES[dom]
@@ -175,7 +182,7 @@ def main():
                                "leaf 64 (depth 14), K=5, propagation + best-of-K",
                    "queries_per_step": n_a, "seed": args.seed},
         "stage_ms": {s: round(float(v), 4) for s, v in zip(STAGES, stage_ms)},
-        "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity,
+        "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity, "quality": quality,
     }
     print(json.dumps(out), flush=True)
 

@@ -123,6 +123,16 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                    float *dist_out, int *knn_out, float *red_a_out,
                    float *red_b_out);
 
+/* ---- exact field (evaluation reference; no reference counterpart) -----
+ * For the nq A patches listed in query_idx (device int32 patch indices),
+ * the k nearest B patches by exact squared L2 over the full p*p*3 uint8
+ * patch, ordered by (distance, index).  Device pointers throughout; out_idx
+ * and out_d2 are nq x k int32 (original B index, squared distance). */
+int annf_exact_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
+                         const uint8_t *img_b, int hb, int wb, int psize,
+                         const int *query_idx, int nq, int k, int *out_idx,
+                         int *out_d2);
+
 /* Per-stage device times (ms) of the last annf_field_dev /
  * annf_process_rows call, measured with HIP events on the handle's stream.
  * Stage order: 0 project A, 1 project B, 2 tree build, 3 natural leaves +
