@@ -7,7 +7,7 @@ Public API (mirrors diku-dk/annfmp's openmp backend):
 The compute path is libannf_hip.so (C ABI: include/annf_hip.h).
 """
 from .field import ANNField
-from .base import ANNFieldPropKDTreeHIP, field_on_device
+from .base import ANNFieldPropKDTreeHIP, field_on_device, field_mm_on_device
 from .kdtree import KDTree
 
-__all__ = ["ANNField", "ANNFieldPropKDTreeHIP", "KDTree", "field_on_device"]
+__all__ = ["ANNField", "ANNFieldPropKDTreeHIP", "KDTree", "field_on_device", "field_mm_on_device"]

@@ -24,7 +24,11 @@ EXPORTS = (
     "annf_create", "annf_destroy", "annf_last_error", "annf_stream", "annf_init", "annf_fit", "annf_fit_dev",
     "annf_neighbors", "annf_process_rows", "annf_norm_array", "annf_free_resources", "annf_tree_info",
     "annf_tree_export", "annf_field_dev", "annf_stage_times", "annf_set_timing", "annf_exact_field_dev",
+    "annf_field_mm_dev",
 )
+
+PREC_BF16 = 0
+PREC_FP8 = 1
 
 
 class ANNFError(RuntimeError):
@@ -70,6 +74,8 @@ def _declare(L):
     L.annf_stage_times.argtypes = [_vp, _fp, i]
     L.annf_set_timing.argtypes = [_vp, i]
     L.annf_exact_field_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, i, i, _vp, _vp]
+    L.annf_field_mm_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, _vp, i, i, i, i, i, i, i, _vp, _vp, _vp,
+                                    _vp, _vp]
     for name in EXPORTS:
         if name not in ("annf_last_error", "annf_stream", "annf_norm_array"):
             getattr(L, name).restype = C.c_int

@@ -180,3 +180,49 @@ def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_nei
         out["red_a"] = ra.cpu().numpy()
         out["red_b"] = rb.cpu().numpy()
     return out
+
+
+def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_neighbors, nprobe=1,
+                       precision="bf16", shortlist=20, projection="fp32", device=0, handle=None,
+                       return_reduced=False, timing=True):
+    """One call of annf_field_mm_dev (full-patch leaf k-NN on MFMA) on host
+    images; returns numpy results like field_on_device.
+
+    precision: "bf16" (exact distances) or "fp8" (shortlist re-ranked
+    exactly); nprobe: leaves per query (a power of two)."""
+    torch = dev.torch()
+    h = handle or _lib.Handle(device)
+    L = _lib.lib()
+    cuda = dev.cuda_device(device)
+    ia = torch.from_numpy(numpy.ascontiguousarray(image_a, dtype=numpy.uint8)).to(cuda)
+    ib = torch.from_numpy(numpy.ascontiguousarray(image_b, dtype=numpy.uint8)).to(cuda)
+    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=numpy.float32)).to(cuda)
+    mu = torch.from_numpy(numpy.ascontiguousarray(mean, dtype=numpy.float64)).to(cuda)
+    d = comp.shape[0]
+    ha, wa = image_a.shape[:2]
+    hb, wb = image_b.shape[:2]
+    n_a = (ha - psize + 1) * (wa - psize + 1)
+    n_b = (hb - psize + 1) * (wb - psize + 1)
+    K = int(n_neighbors)
+    prec = {"bf16": _lib.PREC_BF16, "fp8": _lib.PREC_FP8}[precision]
+    nn = torch.empty(n_a, dtype=torch.int32, device=cuda)
+    dist = torch.empty(n_a, dtype=torch.float32, device=cuda)
+    knn = torch.empty((n_a, K), dtype=torch.int32, device=cuda)
+    ra = torch.empty((n_a, d), dtype=torch.float32, device=cuda) if return_reduced else None
+    rb = torch.empty((n_b, d), dtype=torch.float32, device=cuda) if return_reduced else None
+    torch.cuda.synchronize(cuda)
+    _lib.check("annf_set_timing", L.annf_set_timing(h.h, int(bool(timing))))
+    t0 = time.time()
+    _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
+        h.h, dev.dptr(ia), ha, wa, dev.dptr(ib), hb, wb, int(psize), dev.dptr(comp), dev.dptr(mu), int(d),
+        1 if projection == "fp64" else 0, int(tree_depth), K, int(nprobe), prec, int(shortlist), dev.dptr(nn),
+        dev.dptr(dist), dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
+    wall = time.time() - t0
+    st = numpy.zeros(7, numpy.float32)
+    L.annf_stage_times(h.h, _lib.ptr(st, _lib._fp), 7)
+    out = {"nn": nn.cpu().numpy(), "dist": dist.cpu().numpy(), "knn": knn.cpu().numpy(),
+           "stages_ms": st[:5].tolist(), "wall_s": wall}
+    if return_reduced:
+        out["red_a"] = ra.cpu().numpy()
+        out["red_b"] = rb.cpu().numpy()
+    return out

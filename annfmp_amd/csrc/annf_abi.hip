@@ -276,7 +276,8 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), natdesc,
                               c->stream));
         HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
-                              c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), c->stream));
+                              c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), leafscan_chunk(),
+                              c->stream));
         if (propagate) {
             RCHK(c->p1d.ensure(nk * sizeof(float)));
             RCHK(c->p1p.ensure(nk * sizeof(int)));
@@ -553,28 +554,29 @@ __global__ void k_unpad(const float *src, long long n, int d, int dpad, float *d
     dst[i] = src[r * dpad + c];
 }
 
-}  // namespace
+// Validates a field request, projects A (into c->qbuf) and B, and builds the
+// tree over B.  Stage marks 0..3.
+struct FieldShape {
+    int p = 0, nr_a = 0, nc_a = 0;
+    long long n_a = 0, n_b = 0;
+    PatchGeom ga{}, gb{};
+};
 
-extern "C" {
-
-int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
-                   int psize, const float *basis, const double *mean, int d, int proj_fp64, int tree_depth,
-                   int n_neighbors, int propagate, int select_best_nn, int *nn_out, float *dist_out, int *knn_out,
-                   float *red_a_out, float *red_b_out) {
+int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
+                      int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
+                      int K, float *red_a_out, float *red_b_out, FieldShape &fs) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
-    if (!img_a || !img_b || !basis || !mean || !nn_out) return fail(ANNF_EINVAL, "null required pointer");
+    if (!img_a || !img_b || !basis || !mean) return fail(ANNF_EINVAL, "null required pointer");
     RCHK(set_device(c));
     const int p = psize;
     if (p < 1 || ha < p || wa < p || hb < p || wb < p) return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
     if (d < 1 || d > ANNF_MAX_D) return fail(ANNF_EUNSUPPORTED, "dimension %d outside [1, %d]", d, ANNF_MAX_D);
-    const int K = n_neighbors;
+    if (projection != 0 && projection != 1) return fail(ANNF_EINVAL, "projection %d is not 0 (FP32) or 1 (FP64)", projection);
     if (!k_supported(K)) return fail(ANNF_EUNSUPPORTED, "n_neighbors %d outside [1, %d]", K, ANNF_MAX_K);
     const int D = p * p * 3;
     const int nr_a = ha - p + 1, nc_a = wa - p + 1, nr_b = hb - p + 1, nc_b = wb - p + 1;
     const long long n_a = (long long)nr_a * nc_a, n_b = (long long)nr_b * nc_b;
     if (n_a > 0x7FFFFFFFLL || n_b > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "too many patches");
-    if (propagate && n_a > n_b)
-        return fail(ANNF_EINVAL, "propagation needs at most as many A patches as B patches");
     if (!c->inited) RCHK(annf_init(c, K, tree_depth, 2147483640, 1, ANNF_SPLIT_LONGEST, 0));
     c->tree_depth = tree_depth;
     RCHK(check_tree_params(c, (int)n_b, d, tree_depth));
@@ -591,12 +593,12 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
     HIPCHK(hipGetLastError());
     RCHK(c->qbuf.ensure((size_t)n_a * dpad * sizeof(float)));
     HIPCHK(project_patches(img_a, ha, wa, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
-                           c->mean64.as<double>(), d, dpad, proj_fp64, c->qbuf.as<float>(), c->stream));
+                           c->mean64.as<double>(), d, dpad, projection, c->qbuf.as<float>(), c->stream));
     c->mark(1);
     c->has_tree = false;
     RCHK(c->pts_a.ensure((size_t)n_b * dpad * sizeof(float)));
     HIPCHK(project_patches(img_b, hb, wb, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
-                           c->mean64.as<double>(), d, dpad, proj_fp64, c->pts_a.as<float>(), c->stream));
+                           c->mean64.as<double>(), d, dpad, projection, c->pts_a.as<float>(), c->stream));
     if (red_b_out) {
         hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n_b * d + 255) / 256)), dim3(256), 0, c->stream,
                            c->pts_a.as<float>(), n_b, d, dpad, red_b_out);
@@ -609,9 +611,73 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
                            c->qbuf.as<float>(), n_a, d, dpad, red_a_out);
         HIPCHK(hipGetLastError());
     }
-    PatchGeom ga{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa, 3LL * wa * ha};
-    PatchGeom gb{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb, 3LL * wb * hb};
-    RCHK(run_rows(c, (int)n_a, nr_a, nc_a, K, propagate, select_best_nn, ga, gb, nn_out, dist_out, knn_out));
+    fs.p = p;
+    fs.nr_a = nr_a;
+    fs.nc_a = nc_a;
+    fs.n_a = n_a;
+    fs.n_b = n_b;
+    fs.ga = PatchGeom{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa, 3LL * wa * ha};
+    fs.gb = PatchGeom{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb, 3LL * wb * hb};
+    return ANNF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
+                   int psize, const float *basis, const double *mean, int d, int proj_fp64, int tree_depth,
+                   int n_neighbors, int propagate, int select_best_nn, int *nn_out, float *dist_out, int *knn_out,
+                   float *red_a_out, float *red_b_out) {
+    if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
+    if (propagate && (long long)(ha - psize + 1) * (wa - psize + 1) > (long long)(hb - psize + 1) * (wb - psize + 1))
+        return fail(ANNF_EINVAL, "propagation needs at most as many A patches as B patches");
+    FieldShape fs;
+    RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, proj_fp64, tree_depth,
+                           n_neighbors, red_a_out, red_b_out, fs));
+    RCHK(run_rows(c, (int)fs.n_a, fs.nr_a, fs.nc_a, n_neighbors, propagate, select_best_nn, fs.ga, fs.gb, nn_out,
+                  dist_out, knn_out));
+    RCHK(sync_stream(c));
+    RCHK(check_build_flag(c));
+    c->collect_marks();
+    return ANNF_OK;
+}
+
+int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
+                      int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
+                      int n_neighbors, int nprobe, int precision, int shortlist, int *nn_out, float *dist_out,
+                      int *knn_out, float *red_a_out, float *red_b_out) {
+    if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
+    if (!leafmm_supported(psize)) return fail(ANNF_EUNSUPPORTED, "patch size %d: padded patch above 512 bytes", psize);
+    if (precision != ANNF_PREC_BF16 && precision != ANNF_PREC_FP8)
+        return fail(ANNF_EINVAL, "precision %d is not ANNF_PREC_BF16 or ANNF_PREC_FP8", precision);
+    const int K = n_neighbors;
+    if (precision == ANNF_PREC_FP8 && (shortlist < K || shortlist > 32))
+        return fail(ANNF_EUNSUPPORTED, "shortlist %d outside [n_neighbors, 32]", shortlist);
+    if (tree_depth < 0 || tree_depth > 30 || nprobe < 1 || nprobe > (1 << tree_depth) || (nprobe & (nprobe - 1)))
+        return fail(ANNF_EINVAL, "nprobe %d is not a power of two in [1, 2^tree_depth]", nprobe);
+    FieldShape fs;
+    RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, projection, tree_depth, K,
+                           red_a_out, red_b_out, fs));
+    const TreeView &t = c->tree;
+    const int nleaves = 1 << t.depth;
+    const int n_a = (int)fs.n_a;
+    c->mark(3);
+    RCHK(c->natleaf.ensure((size_t)n_a * sizeof(int)));
+    RCHK(c->bcount.ensure((size_t)nleaves * sizeof(int)));
+    RCHK(c->boffset.ensure(((size_t)nleaves + 1) * sizeof(int)));
+    RCHK(c->bcursor.ensure((size_t)nleaves * sizeof(int)));
+    RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
+    RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
+    HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
+    HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), nullptr,
+                          c->stream));
+    HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
+                          c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), leafmm_chunk(), c->stream));
+    c->mark(4);
+    HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), n_a, nprobe,
+                   shortlist, K, precision == ANNF_PREC_FP8, knn_out, nn_out, dist_out, c->stream));
+    c->mark(5);
     RCHK(sync_stream(c));
     RCHK(check_build_flag(c));
     c->collect_marks();

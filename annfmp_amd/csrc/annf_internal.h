@@ -71,9 +71,11 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
 hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, int *desc,
                           hipStream_t st);
 // Buckets queries by natural leaf; `count` must hold the per-leaf counts and
-// is overwritten with chunks per leaf; chunk_off receives nleaves+1 offsets.
+// is overwritten with the number of `chunk`-query chunks per leaf; chunk_off
+// receives nleaves+1 offsets.
 hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset,
-                          int *cursor, int *order, int *chunk_off, hipStream_t st);
+                          int *cursor, int *order, int *chunk_off, int chunk, hipStream_t st);
+int leafscan_chunk();
 hipError_t leaf_scan(const TreeView &t, const float *q, int nq, const int *order, const int *offset,
                      const int *chunk_off, int K, const int *propleaf, float *out_d, int *out_p,
                      int *out_l, hipStream_t st);
@@ -92,6 +94,16 @@ hipError_t convert_knn(const int *pos, const float *d2, const int *orig, long lo
 hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float *basis_t,
                            const float *mean_f, const double *basis_t64, const double *mean64,
                            int d, int dpad, int fp64, float *out, hipStream_t st);
+
+// full-patch leaf k-NN on MFMA (annf_leafmm.hip): queries bucketed with
+// chunks of leafmm_chunk(); BF16 (exact distances) or FP8 (shortlist of
+// `shortlist` re-ranked exactly); nprobe leaves leaf ^ 0 .. leaf ^ (nprobe-1)
+int leafmm_chunk();
+int leafmm_dk(int p);
+bool leafmm_supported(int p);
+hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
+                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int *knn, int *nn,
+                   float *dist, hipStream_t st);
 
 // exact field (annf_eval.hip)
 hipError_t patch_matrix(const uint8_t *img, int h, int w, int p, uint8_t *out, int dpad_b, hipStream_t st);

@@ -659,17 +659,20 @@ __global__ void k_bucket(const int *leaf, int nq, int *cursor, int *order) {
 // queries, so one crowded leaf (a flat image region) spreads over many blocks.
 constexpr int kScanChunk = 256;
 
-__global__ void k_chunk_count(const int *offset, int nleaves, int *chunks) {
+__global__ void k_chunk_count(const int *offset, int nleaves, int chunk, int *chunks) {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < nleaves) chunks[l] = (offset[l + 1] - offset[l] + kScanChunk - 1) / kScanChunk;
+    if (l < nleaves) chunks[l] = (offset[l + 1] - offset[l] + chunk - 1) / chunk;
 }
 
+int leafscan_chunk() { return kScanChunk; }
+
 hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset, int *cursor, int *order,
-                          int *chunk_off, hipStream_t st) {
+                          int *chunk_off, int chunk, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, offset, cursor);
     if (nq > 0) hipLaunchKernelGGL(k_bucket, dim3(blocks_for(nq, 256)), dim3(256), 0, st, leaf, nq, cursor, order);
     // chunks per leaf (into count, no longer needed) and their offsets
-    hipLaunchKernelGGL(k_chunk_count, dim3(blocks_for(nleaves, 256)), dim3(256), 0, st, offset, nleaves, count);
+    hipLaunchKernelGGL(k_chunk_count, dim3(blocks_for(nleaves, 256)), dim3(256), 0, st, offset, nleaves, chunk,
+                       count);
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, chunk_off, (int *)nullptr);
     return hipGetLastError();
 }

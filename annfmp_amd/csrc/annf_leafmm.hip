@@ -1,0 +1,503 @@
+// Full-patch leaf k-NN on MFMA (the precision variants and the learned-
+// encoder path).  For the queries bucketed by natural leaf, the distance to
+// every point of the probed leaves is |a|^2 + |b|^2 - 2 a.b over the full
+// p*p*3 uint8 patch, the dot products on MFMA with FP32 accumulation:
+//
+//  * BF16: A patches centred (a - 128, in [-128, 127]) and B patches as they
+//    are (0..255) are exact in BF16; every product and every partial sum of
+//    (a - 128).b stays below 2^24 in magnitude for D <= 512, so FP32
+//    accumulation is exact in any order and the distance is the exact
+//    integer.  The K smallest (distance, B index) keys are the answer; no
+//    re-rank can change them.
+//  * FP8-E4M3: both patches centred and scaled per patch by 448 / max|x-128|,
+//    dot products de-scaled in FP32.  The S smallest approximate keys form a
+//    shortlist that is re-ranked with exact integer distances into the K
+//    list.
+//
+// Keys are (squared distance << 32) | B index, so ties go to the lower B
+// index, the order of the exact field.  The first list entry is the match.
+//
+// Block: 256 threads, 64 queries of one leaf bucket (two 32-row MFMA blocks)
+// against tiles of 128 leaf points (one 32-column block per wavefront).
+// Patches are staged in LDS as bytes, rows padded to whole dwords.
+#include "annf_internal.h"
+
+#include <type_traits>
+
+namespace annf {
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kMQ = 64;                // queries per block
+constexpr int kNP = 128;               // points per tile
+constexpr int kThreads = 256;
+constexpr int kDS = kNP + 4;           // distance tile row stride (dwords)
+constexpr uint64_t kNoKey = ~0ull;
+
+__device__ __forceinline__ long long pofs(const PatchGeom &g, long long i) {
+    const long long y = i / g.ncols, x = i - y * g.ncols;
+    return (y * g.width + x) * 3;
+}
+
+// One patch row of `rw` dwords starting at byte `off`; the last dword is
+// masked by `tmask` (bytes past the row are zero).  fn(j, word).
+template <typename Fn>
+__device__ __forceinline__ void load_row(const PatchGeom &g, long long off, int rw, uint32_t tmask, Fn &&fn) {
+    const long long al = off & ~3LL;
+    const int sh = (int)(off & 3);
+    if (al + 4LL * (rw + 1) <= g.nbytes) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(g.base + al);
+        uint32_t prev = src[0];
+        for (int j = 0; j < rw; ++j) {
+            const uint32_t next = src[j + 1];
+            uint32_t v = __builtin_amdgcn_alignbyte(next, prev, sh);
+            if (j == rw - 1) v &= tmask;
+            fn(j, v);
+            prev = next;
+        }
+    } else {
+        for (int j = 0; j < rw; ++j) {
+            uint32_t v = 0;
+            for (int k = 0; k < 4; ++k) {
+                const long long o = off + 4 * j + k;
+                if (o < g.nbytes) v |= (uint32_t)g.base[o] << (8 * k);
+            }
+            if (j == rw - 1) v &= tmask;
+            fn(j, v);
+        }
+    }
+}
+
+// x -> round-to-nearest E4M3 code of (x - 128) * scale, byte by byte; bytes
+// outside `mask` become code 0.
+__device__ __forceinline__ uint32_t fp8_word(uint32_t w, float scale, uint32_t mask) {
+    const float a0 = ((float)(w & 255u) - 128.0f) * scale;
+    const float a1 = ((float)((w >> 8) & 255u) - 128.0f) * scale;
+    const float a2 = ((float)((w >> 16) & 255u) - 128.0f) * scale;
+    const float a3 = ((float)(w >> 24) - 128.0f) * scale;
+    int packed = __builtin_amdgcn_cvt_pk_fp8_f32(a0, a1, 0, false);
+    packed = __builtin_amdgcn_cvt_pk_fp8_f32(a2, a3, packed, true);
+    return (uint32_t)packed & mask;
+}
+
+__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
+    // the values are small integers: their BF16 is the top half of the FP32
+    return (__builtin_bit_cast(uint32_t, hi) & 0xFFFF0000u) | (__builtin_bit_cast(uint32_t, lo) >> 16);
+}
+
+// eight bytes (two dwords at p) as BF16; CENTER subtracts 128 first
+template <bool CENTER>
+__device__ __forceinline__ bf16x8 frag_bf16(const uint8_t *p) {
+    const uint32_t w0 = *reinterpret_cast<const uint32_t *>(p);
+    const uint32_t w1 = *reinterpret_cast<const uint32_t *>(p + 4);
+    const float c = CENTER ? 128.0f : 0.0f;
+    uint4 r;
+    r.x = bf16_pair((float)(w0 & 255u) - c, (float)((w0 >> 8) & 255u) - c);
+    r.y = bf16_pair((float)((w0 >> 16) & 255u) - c, (float)(w0 >> 24) - c);
+    r.z = bf16_pair((float)(w1 & 255u) - c, (float)((w1 >> 8) & 255u) - c);
+    r.w = bf16_pair((float)((w1 >> 16) & 255u) - c, (float)(w1 >> 24) - c);
+    return __builtin_bit_cast(bf16x8, r);
+}
+
+__device__ __forceinline__ long frag_fp8(const uint8_t *p) {
+    const uint32_t w0 = *reinterpret_cast<const uint32_t *>(p);
+    const uint32_t w1 = *reinterpret_cast<const uint32_t *>(p + 4);
+    return (long)(((uint64_t)w1 << 32) | w0);
+}
+
+__device__ __forceinline__ uint64_t shfl_xor64(uint64_t v, int m) {
+    const int lo = __shfl_xor((int)(uint32_t)v, m, 64);
+    const int hi = __shfl_xor((int)(uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo;
+}
+
+// Sorted key list of LM slots whose live part is the top L slots; the front
+// slots hold 0, which never moves (an insert lands after every key <= it),
+// so every index is static.
+template <int LM>
+__device__ __forceinline__ void klist_init(uint64_t (&l)[LM], int L) {
+#pragma unroll
+    for (int s = 0; s < LM; ++s) l[s] = (s < LM - L) ? 0ull : kNoKey;
+}
+
+template <int LM>
+__device__ __forceinline__ void klist_insert(uint64_t (&l)[LM], uint64_t key) {
+    int r = 0;
+#pragma unroll
+    for (int s = 0; s < LM - 1; ++s) r += (l[s] <= key) ? 1 : 0;
+#pragma unroll
+    for (int s = LM - 1; s >= 1; --s)
+        if (s > r) l[s] = l[s - 1];
+#pragma unroll
+    for (int s = 0; s < LM; ++s)
+        if (s == r) l[s] = key;
+}
+
+// exact squared distance of A patch i and B patch j, both read from HBM
+__device__ __forceinline__ uint32_t exact_d2(const PatchGeom &ga, long long i, const PatchGeom &gb, long long j,
+                                             int rw, uint32_t tmask) {
+    uint32_t saa = 0, sbb = 0, sab = 0;
+    const long long oa = pofs(ga, i), ob = pofs(gb, j);
+    for (int r = 0; r < ga.rows; ++r) {
+        for (int k = 0; k < rw; ++k) {
+            // dword k of row r of both patches (unaligned reads, masked tail)
+            uint32_t wa = 0, wb = 0;
+            load_row(ga, oa + r * ga.rstride + 4 * k, 1, k == rw - 1 ? tmask : ~0u,
+                     [&](int, uint32_t v) { wa = v; });
+            load_row(gb, ob + r * gb.rstride + 4 * k, 1, k == rw - 1 ? tmask : ~0u,
+                     [&](int, uint32_t v) { wb = v; });
+            saa = __builtin_amdgcn_udot4(wa, wa, saa, false);
+            sbb = __builtin_amdgcn_udot4(wb, wb, sbb, false);
+            sab = __builtin_amdgcn_udot4(wa, wb, sab, false);
+        }
+    }
+    return saa + sbb - 2u * sab;
+}
+
+// Stages patch idx into an LDS row (rows of rw dwords); the 4 lanes of a
+// quad take rows q4, q4+4, ...; dwords from rows*rw to dkw are zero.
+// Returns through ss / sum this lane's share of sum(x^2) and sum(x).
+__device__ __forceinline__ void stage_patch(const PatchGeom &g, long long idx, bool valid, uint32_t *row, int rw,
+                                            uint32_t tmask, int dkw, int q4, uint32_t &ss, uint32_t &sum) {
+    ss = 0;
+    sum = 0;
+    if (valid) {
+        const long long o = pofs(g, idx);
+        for (int r = q4; r < g.rows; r += 4) {
+            uint32_t *dst = row + r * rw;
+            load_row(g, o + r * g.rstride, rw, tmask, [&](int k, uint32_t v) {
+                dst[k] = v;
+                ss = __builtin_amdgcn_udot4(v, v, ss, false);
+                sum = __builtin_amdgcn_udot4(v, 0x01010101u, sum, false);
+            });
+        }
+    } else {
+        for (int r = q4; r < g.rows; r += 4)
+            for (int k = 0; k < rw; ++k) row[r * rw + k] = 0;
+    }
+    for (int k = g.rows * rw + q4; k < dkw; k += 4) row[k] = 0;
+}
+
+// In-place FP8 conversion of the rows this lane staged (after the quad knows
+// the patch scale).
+__device__ __forceinline__ void fp8_rows(uint32_t *row, int rows, int rw, uint32_t tmask, float scale, int q4) {
+    for (int r = q4; r < rows; r += 4)
+        for (int k = 0; k < rw; ++k) row[r * rw + k] = fp8_word(row[r * rw + k], scale, k == rw - 1 ? tmask : ~0u);
+}
+
+// max |x - 128| over the real bytes of the rows this lane staged
+__device__ __forceinline__ uint32_t maxabs_rows(const uint32_t *row, int rows, int rw, uint32_t tmask, int q4) {
+    int mx = 0, mn = 255;
+    for (int r = q4; r < rows; r += 4)
+        for (int k = 0; k < rw; ++k) {
+            const uint32_t w = row[r * rw + k];
+            const uint32_t m = k == rw - 1 ? tmask : ~0u;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if ((m >> (8 * b)) & 255u) {
+                    const int x = (int)((w >> (8 * b)) & 255u);
+                    mx = max(mx, x);
+                    mn = min(mn, x);
+                }
+            }
+        }
+    if (mx < mn) return 0;   // this lane staged no bytes
+    return (uint32_t)max(mx - 128, 128 - mn);
+}
+
+__device__ __forceinline__ uint32_t quad_sum(uint32_t v) {
+    v += (uint32_t)__shfl_xor((int)v, 1, 64);
+    v += (uint32_t)__shfl_xor((int)v, 2, 64);
+    return v;
+}
+
+__device__ __forceinline__ uint32_t quad_max(uint32_t v) {
+    v = max(v, (uint32_t)__shfl_xor((int)v, 1, 64));
+    v = max(v, (uint32_t)__shfl_xor((int)v, 2, 64));
+    return v;
+}
+
+__device__ __forceinline__ float sqrt_f(uint32_t d2) {
+    // correctly rounded: the FP64 root rounded once more to FP32 (2*24+2 <= 53)
+    return (float)__builtin_sqrt((double)d2);
+}
+
+template <int DK, int LM, bool FP8>
+__global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, PatchGeom gb,
+                                                         const int *__restrict__ order,
+                                                         const int *__restrict__ offset,
+                                                         const int *__restrict__ chunk_off, int nprobe, int K, int *__restrict__ knn, int *__restrict__ nn,
+                                                         float *__restrict__ dist) {
+    constexpr int RB = DK + 4;                              // LDS patch stride (bytes, odd dword count)
+    constexpr int kBD = (kNP * RB > kMQ * kDS * 4) ? kNP * RB : kMQ * kDS * 4;
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint8_t *Aq = smem;                                      // kMQ x RB
+    uint8_t *BD = Aq + kMQ * RB;                             // points, then distances, then candidates
+    uint32_t *Dm = reinterpret_cast<uint32_t *>(BD);         // kMQ x kDS
+    uint64_t *cand = reinterpret_cast<uint64_t *>(BD);       // kMQ x LM
+    uint32_t *na = reinterpret_cast<uint32_t *>(BD + kBD);   // kMQ
+    uint32_t *nb = na + kMQ;                                 // kNP
+    int *nbsum = reinterpret_cast<int *>(nb + kNP);          // kNP
+    int *qid = nbsum + kNP;                                  // kMQ
+    int *borig = qid + kMQ;                                  // kNP
+    float *sa = reinterpret_cast<float *>(borig + kNP);      // kMQ
+    float *sb = sa + kMQ;                                    // kNP
+
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    const int q = tid >> 2, q4 = tid & 3;                    // query row / quad lane
+    const int nleaves = 1 << t.depth;
+    const int rw = (ga.rowlen + 3) >> 2;
+    const int tail = ga.rowlen & 3;
+    const uint32_t tmask = tail ? ((1u << (8 * tail)) - 1u) : ~0u;
+    const int D = ga.rows * ga.rowlen;
+    constexpr int dkw = DK / 4;
+    const int b = blockIdx.x;
+    if (b >= chunk_off[nleaves]) return;
+    int lo = 0, hi = nleaves - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (chunk_off[m] <= b) lo = m; else hi = m - 1;
+    }
+    const int leaf = lo;
+    const int qb = offset[leaf] + (b - chunk_off[leaf]) * kMQ;
+    const int nqb = min(kMQ, offset[leaf + 1] - qb);
+    if (nqb <= 0) return;
+
+    // queries: one quad per query
+    {
+        const bool v = q < nqb;
+        const int qa = v ? order[qb + q] : 0;
+        uint32_t *row = reinterpret_cast<uint32_t *>(Aq + q * RB);
+        uint32_t ss, sm;
+        stage_patch(ga, qa, v, row, rw, tmask, dkw, q4, ss, sm);
+        ss = quad_sum(ss);
+        sm = quad_sum(sm);
+        if constexpr (FP8) {
+            const uint32_t mx = quad_max(maxabs_rows(row, ga.rows, rw, tmask, q4));
+            const float sc = mx ? 448.0f / (float)mx : 1.0f;
+            fp8_rows(row, ga.rows, rw, tmask, sc, q4);
+            if (q4 == 0) {
+                sa[q] = sc;
+                na[q] = ss - 256u * sm + 16384u * (uint32_t)D;   // sum (a - 128)^2
+            }
+        } else {
+            if (q4 == 0) na[q] = ss;
+        }
+        if (q4 == 0) qid[q] = qa;
+    }
+
+    // this lane's LM smallest keys (every slot live)
+    uint64_t lst[LM];
+    klist_init(lst, LM);
+
+    for (int pr = 0; pr < nprobe; ++pr) {
+        const int pleaf = leaf ^ pr;
+        const int from = t.leaves[2 * pleaf], cnt = t.leaves[2 * pleaf + 1] - from;
+        for (int t0 = 0; t0 < cnt; t0 += kNP) {
+            const int np = min(kNP, cnt - t0);
+            __syncthreads();   // the previous tile's distances are consumed
+            for (int e = tid; e < kNP * 4; e += kThreads) {
+                const int i = e >> 2;
+                const bool v = i < np;
+                const int bj = v ? t.orig[from + t0 + i] : 0;
+                uint32_t *row = reinterpret_cast<uint32_t *>(BD + i * RB);
+                uint32_t ss, sm;
+                stage_patch(gb, bj, v, row, rw, tmask, dkw, q4, ss, sm);
+                ss = quad_sum(ss);
+                sm = quad_sum(sm);
+                if constexpr (FP8) {
+                    const uint32_t mx = quad_max(maxabs_rows(row, gb.rows, rw, tmask, q4));
+                    const float sc = mx ? 448.0f / (float)mx : 1.0f;
+                    fp8_rows(row, gb.rows, rw, tmask, sc, q4);
+                    if (q4 == 0) {
+                        sb[i] = sc;
+                        nb[i] = ss - 256u * sm + 16384u * (uint32_t)D;
+                    }
+                } else {
+                    if (q4 == 0) {
+                        nb[i] = ss;
+                        nbsum[i] = (int)sm;
+                    }
+                }
+                if (q4 == 0) borig[i] = bj;
+            }
+            __syncthreads();
+            // 64 x 32 block of dot products per wavefront
+            floatx16 acc0 = {}, acc1 = {};
+            {
+                const uint8_t *ar0 = Aq + (lane & 31) * RB + 8 * (lane >> 5);
+                const uint8_t *ar1 = ar0 + 32 * RB;
+                const uint8_t *br = BD + (32 * wid + (lane & 31)) * RB + 8 * (lane >> 5);
+#pragma unroll 5
+                for (int k0 = 0; k0 < DK; k0 += 16) {
+                    if constexpr (FP8) {
+                        const long fb = frag_fp8(br + k0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(frag_fp8(ar0 + k0), fb, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(frag_fp8(ar1 + k0), fb, acc1, 0, 0, 0);
+                    } else {
+                        const bf16x8 fb = frag_bf16<false>(br + k0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16<true>(ar0 + k0), fb, acc0, 0, 0,
+                                                                       0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16<true>(ar1 + k0), fb, acc1, 0, 0,
+                                                                       0);
+                    }
+                }
+            }
+            const int pcol = 32 * wid + (lane & 31);
+            const uint32_t nbp = nb[pcol];
+            const int sump = nbsum[pcol];
+            const float sbp = sb[pcol];
+            __syncthreads();   // every wavefront is done with the staged points
+            {
+                const bool pv = pcol < np;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int qq = row + 32 * h;
+                        const float acc = h ? acc1[r] : acc0[r];
+                        uint32_t d;
+                        if constexpr (FP8) {
+                            const float dd = ((float)na[qq] + (float)nbp) - 2.0f * (acc / (sa[qq] * sbp));
+                            d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
+                        } else {
+                            // a.b = (a - 128).b + 128 sum(b); the dot is an exact integer
+                            d = na[qq] + nbp - 2u * ((uint32_t)(int)acc + 128u * (uint32_t)sump);
+                        }
+                        Dm[qq * kDS + pcol] = pv ? d : 0xFFFFFFFFu;
+                    }
+                }
+            }
+            __syncthreads();
+            if (q < nqb) {
+                const uint32_t *dr = Dm + q * kDS;
+                for (int p = q4; p < np; p += 4) {
+                    const uint64_t key = ((uint64_t)dr[p] << 32) | (uint32_t)borig[p];
+                    if (key < lst[LM - 1]) klist_insert(lst, key);
+                }
+            }
+        }
+    }
+    __syncthreads();   // the distance tile is free for the candidate lists
+
+    // merge the quad's four lists: the LM smallest keys, in order, into cand[q]
+    for (int s = 0; s < LM; ++s) {
+        const uint64_t head = lst[0];
+        uint64_t m = head;
+        const uint64_t m1 = shfl_xor64(m, 1);
+        m = m1 < m ? m1 : m;
+        const uint64_t m2 = shfl_xor64(m, 2);
+        m = m2 < m ? m2 : m;
+        if (head == m && m != kNoKey) {
+#pragma unroll
+            for (int k = 0; k < LM - 1; ++k) lst[k] = lst[k + 1];
+            lst[LM - 1] = kNoKey;
+        }
+        if (q4 == 0) cand[q * LM + s] = m;
+    }
+    if constexpr (FP8) {
+        __syncthreads();
+        // exact re-rank of the shortlist
+        if (q < nqb) {
+            for (int s = q4; s < LM; s += 4) {
+                const uint64_t key = cand[q * LM + s];
+                if (key == kNoKey) continue;
+                const uint32_t j = (uint32_t)key;
+                cand[q * LM + s] = ((uint64_t)exact_d2(ga, qid[q], gb, j, rw, tmask) << 32) | j;
+            }
+        }
+        __syncthreads();
+        if (q < nqb && q4 == 0) {
+            // K smallest exact keys, in order, into the front of cand[q]
+            uint64_t best[LM];
+            klist_init(best, K);
+            for (int s = 0; s < LM; ++s) {
+                const uint64_t key = cand[q * LM + s];
+                if (key < best[LM - 1]) klist_insert(best, key);
+            }
+#pragma unroll
+            for (int s = 0; s < LM; ++s)
+                if (s >= LM - K) cand[q * LM + s - (LM - K)] = best[s];
+        }
+    }
+    __syncthreads();
+    if (q < nqb) {
+        const int qa = qid[q];
+        for (int s = q4; s < K; s += 4) {
+            const uint64_t key = cand[q * LM + s];
+            const int j = key == kNoKey ? -1 : (int)(uint32_t)key;
+            if (knn) knn[(size_t)qa * K + s] = j;
+            if (s == 0) {
+                nn[qa] = j;
+                if (dist) dist[qa] = key == kNoKey ? kFltMax : sqrt_f((uint32_t)(key >> 32));
+            }
+        }
+    }
+}
+
+template <typename Fn>
+hipError_t with_dk_lm(int dk_need, int lm_need, Fn &&fn) {
+    auto pick_lm = [&](auto dk) -> hipError_t {
+        if (lm_need <= 5) return fn(dk, std::integral_constant<int, 5>{});
+        if (lm_need <= 8) return fn(dk, std::integral_constant<int, 8>{});
+        if (lm_need <= 16) return fn(dk, std::integral_constant<int, 16>{});
+        if (lm_need <= 20) return fn(dk, std::integral_constant<int, 20>{});
+        return fn(dk, std::integral_constant<int, 32>{});
+    };
+    if (dk_need <= 176) return pick_lm(std::integral_constant<int, 176>{});
+    if (dk_need <= 256) return pick_lm(std::integral_constant<int, 256>{});
+    if (dk_need <= 400) return pick_lm(std::integral_constant<int, 400>{});
+    return pick_lm(std::integral_constant<int, 512>{});
+}
+
+}  // namespace
+
+int leafmm_chunk() { return kMQ; }
+
+// padded patch length (bytes) the kernel stages: rows of whole dwords,
+// rounded up to the MFMA depth of 16
+int leafmm_dk(int p) {
+    const int rw = (3 * p + 3) / 4;
+    return (p * rw * 4 + 15) / 16 * 16;
+}
+
+bool leafmm_supported(int p) { return p >= 1 && leafmm_dk(p) <= 512; }
+
+hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
+                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int *knn, int *nn,
+                   float *dist, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    const int p = ga.rows;
+    if (!leafmm_supported(p) || gb.rows != p || ga.rowlen != 3 * p || gb.rowlen != 3 * p) return hipErrorInvalidValue;
+    const int nleaves = 1 << t.depth;
+    if (nprobe < 1 || nprobe > nleaves || K < 1 || K > 32) return hipErrorInvalidValue;
+    // list length: K for BF16 (exact keys), the shortlist for FP8; rounded
+    // up to an instantiated length (a longer list only adds candidates)
+    const int L = fp8 ? shortlist : K;
+    if (L < K || L > 32) return hipErrorInvalidValue;
+    const unsigned nblk = (unsigned)(nleaves + (nq + kMQ - 1) / kMQ);
+    return with_dk_lm(leafmm_dk(p), L, [&](auto dk, auto lm) -> hipError_t {
+        constexpr int DK = decltype(dk)::value;
+        constexpr int LM = decltype(lm)::value;
+        constexpr int RB = DK + 4;
+        constexpr int kBD = (kNP * RB > kMQ * kDS * 4) ? kNP * RB : kMQ * kDS * 4;
+        static_assert(kBD >= kMQ * LM * 8, "candidate lists must fit the tile region");
+        const size_t lds = (size_t)kMQ * RB + kBD + (size_t)(kMQ + kNP + kNP + kMQ + kNP + kMQ + kNP) * 4;
+        auto launch = [&](auto f8) -> hipError_t {
+            constexpr bool F8 = decltype(f8)::value;
+            hipError_t e = hipFuncSetAttribute((const void *)k_leaf_mm<DK, LM, F8>,
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb, order,
+                               offset, chunk_off, nprobe, K, knn, nn, dist);
+            return hipGetLastError();
+        };
+        return fp8 ? launch(std::true_type{}) : launch(std::false_type{});
+    });
+}
+
+}  // namespace annf

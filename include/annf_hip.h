@@ -123,6 +123,35 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                    float *dist_out, int *knn_out, float *red_a_out,
                    float *red_b_out);
 
+/* ---- precision variants of the field (no reference counterpart: the
+ *      reference's leaf scan, kdtree.c:305-352, measures distances in the
+ *      projected space) ----------------------------------------------------
+ * Same projection and tree as annf_field_dev (projection: 0 FP32, 1 FP64).
+ * Every A patch is routed to its natural leaf; the points of the leaves
+ * leaf ^ 0 .. leaf ^ (nprobe - 1) (nprobe a power of two: the enclosing
+ * subtree of nprobe leaves) are compared over the full p*p*3 patch as
+ * |a|^2 + |b|^2 - 2 a.b on MFMA with FP32 accumulation:
+ *   ANNF_PREC_BF16: BF16 operands; the distances are exact integers, the
+ *     K list is the K smallest (distance, B index);
+ *   ANNF_PREC_FP8:  FP8-E4M3 operands, patches centred at 128 and scaled
+ *     per patch by 448 / max|x - 128|; the `shortlist` smallest approximate
+ *     distances (n_neighbors <= shortlist <= 32) are re-ranked with exact
+ *     distances into the K list.
+ * Outputs as annf_field_dev: nn_out is the first list entry, dist_out its
+ * exact L2 (correctly rounded), knn_out the K list (-1 past the candidates).
+ * No propagation, no best-of-K pass: the list is already ranked by the full
+ * patch distance.  Stage marks: 0 project A, 1 project B, 2 tree build,
+ * 3 natural leaves + bucketing, 4 MFMA leaf k-NN. */
+#define ANNF_PREC_BF16 0
+#define ANNF_PREC_FP8 1
+int annf_field_mm_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
+                      const uint8_t *img_b, int hb, int wb, int psize,
+                      const float *basis, const double *mean, int d,
+                      int projection, int tree_depth, int n_neighbors,
+                      int nprobe, int precision, int shortlist, int *nn_out,
+                      float *dist_out, int *knn_out, float *red_a_out,
+                      float *red_b_out);
+
 /* ---- exact field (evaluation reference; no reference counterpart) -----
  * For the nq A patches listed in query_idx (device int32 patch indices),
  * the k nearest B patches by exact squared L2 over the full p*p*3 uint8

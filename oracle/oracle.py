@@ -21,6 +21,8 @@ Plus numpy restatements of the Python-side stages:
 * ``exact_field``      -- brute-force k-NN over full uint8 patches with exact
                           integer squared distances (equal to FP64: every
                           value is an integer below 2**25)
+* ``leaf_knn_exact``   -- the same restricted to a query's probed leaves (the
+                          answer of the BF16 full-patch leaf k-NN)
 """
 from __future__ import annotations
 
@@ -112,6 +114,44 @@ def exact_field(pa, pb, k, chunk=2048):
             order = np.lexsort((cand, d2[r, cand]))[:k]
             idx[s + r] = cand[order]
             dst[s + r] = d2[r, cand[order]]
+    return idx, dst
+
+
+def exact_pair_d2(pa, pb, ia, ib):
+    """Exact integer squared L2 between patch rows pa[ia] and pb[ib]."""
+    x = pa[np.asarray(ia, np.int64)].astype(np.int64)
+    y = pb[np.asarray(ib, np.int64)].astype(np.int64)
+    return ((x - y) ** 2).sum(1)
+
+
+def leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k):
+    """Exact k nearest B patches of every A patch among the points of the
+    leaves qleaf ^ 0 .. qleaf ^ (nprobe - 1) (the full-patch leaf k-NN of
+    the precision variants).  qleaf: natural leaf per A patch; leaves:
+    [from, to) tree positions per leaf; orig: B index per tree position.
+    Integer squared L2, ties by lower B index; missing entries are -1.
+    Returns (idx [n_a, k] int64, sqdist [n_a, k] int64).
+
+    No reference counterpart: the reference scans leaves in the projected
+    space (kdtree.c:305-352).  With nprobe = number of leaves it is the
+    exact field (exact_field)."""
+    a = pa.astype(np.int64)
+    b = pb.astype(np.int64)
+    qleaf = np.asarray(qleaf, np.int64)
+    idx = np.full((a.shape[0], k), -1, np.int64)
+    dst = np.full((a.shape[0], k), -1, np.int64)
+    for leaf in np.unique(qleaf):
+        rows = np.nonzero(qleaf == leaf)[0]
+        pos = np.concatenate([np.arange(leaves[2 * (leaf ^ r)], leaves[2 * (leaf ^ r) + 1])
+                              for r in range(nprobe)])
+        cand = np.asarray(orig, np.int64)[pos]
+        x = a[rows]
+        y = b[cand]
+        d2 = (x * x).sum(1)[:, None] + (y * y).sum(1)[None, :] - 2 * (x @ y.T)
+        order = np.lexsort((np.broadcast_to(cand, d2.shape), d2), axis=1)[:, :k]
+        m = order.shape[1]
+        idx[rows, :m] = cand[order]
+        dst[rows, :m] = np.take_along_axis(d2, order, 1)
     return idx, dst
 
 

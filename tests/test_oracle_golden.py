@@ -193,3 +193,27 @@ def test_planted_mistake_is_caught():
     bad = [np.array_equal(scan(q, g["group_leaves"][i], True), g["group_pos"][i]) for i, q in enumerate(g["Q"][:16])]
     assert all(ok)
     assert not all(bad)
+
+
+def test_leaf_knn_exact_reduces_to_exact_field():
+    """Probing every leaf makes the leaf-restricted k-NN the exact field;
+    one leaf restricts it to that leaf's points (brute force by hand)."""
+    rng = np.random.default_rng(11)
+    pa = rng.integers(0, 256, (300, 27)).astype(np.uint8)
+    pb = rng.integers(0, 256, (256, 27)).astype(np.uint8)
+    pb[5] = pb[9]                                   # an exact tie between two B patches
+    pa[0] = pb[9]
+    leaves = np.array([0, 64, 64, 128, 128, 192, 192, 256])
+    orig = rng.permutation(256)
+    qleaf = rng.integers(0, 4, 300)
+    i1, d1 = O.leaf_knn_exact(pa, pb, qleaf, leaves, orig, 4, 6)
+    i2, d2 = O.exact_field(pa, pb, 6)
+    assert np.array_equal(i1, i2) and np.array_equal(d1, d2)
+    assert i1[0, 0] == 5 and d1[0, 0] == 0 and i1[0, 1] == 9     # tie by lower index
+    i3, d3 = O.leaf_knn_exact(pa, pb, qleaf, leaves, orig, 1, 3)
+    for r in range(0, 300, 37):
+        l = qleaf[r]
+        cand = orig[leaves[2 * l]:leaves[2 * l + 1]]
+        dd = [(int(((pa[r].astype(int) - pb[c].astype(int)) ** 2).sum()), int(c)) for c in cand]
+        dd.sort()
+        assert [c for _, c in dd[:3]] == list(i3[r]) and [x for x, _ in dd[:3]] == list(d3[r])
