@@ -24,8 +24,13 @@ EXPORTS = (
     "annf_create", "annf_destroy", "annf_last_error", "annf_stream", "annf_init", "annf_fit", "annf_fit_dev",
     "annf_neighbors", "annf_process_rows", "annf_norm_array", "annf_free_resources", "annf_tree_info",
     "annf_tree_export", "annf_field_dev", "annf_stage_times", "annf_set_timing", "annf_exact_field_dev",
-    "annf_field_mm_dev",
+    "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
+    "annf_set_encoder", "annf_encode_image_dev",
 )
+
+PROJ_PCA32 = 0
+PROJ_PCA64 = 1
+PROJ_ENCODER = 2
 
 PREC_BF16 = 0
 PREC_FP8 = 1
@@ -76,6 +81,12 @@ def _declare(L):
     L.annf_exact_field_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, i, i, _vp, _vp]
     L.annf_field_mm_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, _vp, i, i, i, i, i, i, i, _vp, _vp, _vp,
                                     _vp, _vp]
+    f = C.c_float
+    L.annf_mlp_forward_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp]
+    L.annf_mlp_triplet_grad_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, f, _vp, _vp]
+    L.annf_mlp_adam_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_longlong, f, f, f, f, i]
+    L.annf_set_encoder.argtypes = [_vp, _vp, i, i, i, i]
+    L.annf_encode_image_dev.argtypes = [_vp, _vp, i, i, i, _vp]
     for name in EXPORTS:
         if name not in ("annf_last_error", "annf_stream", "annf_norm_array"):
             getattr(L, name).restype = C.c_int

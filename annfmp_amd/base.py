@@ -139,6 +139,24 @@ class ANNFieldPropKDTreeHIP:
         return self._create_patches(self.image_b)
 
 
+def _projection_args(projection, components, mean, psize, handle, device):
+    """projection "fp32" / "fp64" (PCA components + mean) or a PatchEncoder,
+    installed on the handle.  Returns (basis, mean, d, ANNF_PROJ_*)."""
+    torch = dev.torch()
+    cuda = dev.cuda_device(device)
+    if not isinstance(projection, str):
+        enc = projection
+        if enc.psize != psize:
+            raise ValueError(f"encoder for patch size {enc.psize}, field uses {psize}")
+        enc.install(handle, device)
+        return None, None, enc.d, _lib.PROJ_ENCODER
+    if projection not in ("fp32", "fp64"):
+        raise ValueError(f"projection {projection!r}: 'fp32', 'fp64' or a PatchEncoder")
+    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=numpy.float32)).to(cuda)
+    mu = torch.from_numpy(numpy.ascontiguousarray(mean, dtype=numpy.float64)).to(cuda)
+    return comp, mu, comp.shape[0], _lib.PROJ_PCA64 if projection == "fp64" else _lib.PROJ_PCA32
+
+
 def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_neighbors, propagate=True,
                     select_best_nn=True, projection="fp32", device=0, handle=None, return_reduced=False,
                     timing=True):
@@ -151,9 +169,7 @@ def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_nei
     cuda = dev.cuda_device(device)
     ia = torch.from_numpy(numpy.ascontiguousarray(image_a, dtype=numpy.uint8)).to(cuda)
     ib = torch.from_numpy(numpy.ascontiguousarray(image_b, dtype=numpy.uint8)).to(cuda)
-    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=numpy.float32)).to(cuda)
-    mu = torch.from_numpy(numpy.ascontiguousarray(mean, dtype=numpy.float64)).to(cuda)
-    d = comp.shape[0]
+    comp, mu, d, proj = _projection_args(projection, components, mean, psize, h, device)
     ha, wa = image_a.shape[:2]
     hb, wb = image_b.shape[:2]
     n_a = (ha - psize + 1) * (wa - psize + 1)
@@ -169,7 +185,7 @@ def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_nei
     t0 = time.time()
     _lib.check("annf_field_dev", L.annf_field_dev(
         h.h, dev.dptr(ia), ha, wa, dev.dptr(ib), hb, wb, int(psize), dev.dptr(comp), dev.dptr(mu), int(d),
-        1 if projection == "fp64" else 0, int(tree_depth), K, int(bool(propagate)), int(bool(select_best_nn)),
+        proj, int(tree_depth), K, int(bool(propagate)), int(bool(select_best_nn)),
         dev.dptr(nn), dev.dptr(dist), dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
     wall = time.time() - t0
     st = numpy.zeros(7, numpy.float32)
@@ -196,9 +212,7 @@ def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_
     cuda = dev.cuda_device(device)
     ia = torch.from_numpy(numpy.ascontiguousarray(image_a, dtype=numpy.uint8)).to(cuda)
     ib = torch.from_numpy(numpy.ascontiguousarray(image_b, dtype=numpy.uint8)).to(cuda)
-    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=numpy.float32)).to(cuda)
-    mu = torch.from_numpy(numpy.ascontiguousarray(mean, dtype=numpy.float64)).to(cuda)
-    d = comp.shape[0]
+    comp, mu, d, proj = _projection_args(projection, components, mean, psize, h, device)
     ha, wa = image_a.shape[:2]
     hb, wb = image_b.shape[:2]
     n_a = (ha - psize + 1) * (wa - psize + 1)
@@ -215,7 +229,7 @@ def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_
     t0 = time.time()
     _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
         h.h, dev.dptr(ia), ha, wa, dev.dptr(ib), hb, wb, int(psize), dev.dptr(comp), dev.dptr(mu), int(d),
-        1 if projection == "fp64" else 0, int(tree_depth), K, int(nprobe), prec, int(shortlist), dev.dptr(nn),
+        proj, int(tree_depth), K, int(nprobe), prec, int(shortlist), dev.dptr(nn),
         dev.dptr(dist), dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
     wall = time.time() - t0
     st = numpy.zeros(7, numpy.float32)

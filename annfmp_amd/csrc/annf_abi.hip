@@ -88,6 +88,10 @@ struct annf_ctx {
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
     DevBuf basis32, mean32, basis64, mean64;
     DevBuf ev_am, ev_an, ev_bm, ev_bn, ev_part;
+    // learned encoder: training workspace and the packed inference weights
+    DevBuf mlp_ws, enc_w1, enc_b1, enc_w2, enc_b2, enc_w3, enc_b3;
+    bool has_encoder = false;
+    int enc_p = 0, enc_d = 0;
     // timing
     bool timing = false;
     hipEvent_t ev[kStages + 1] = {};
@@ -121,9 +125,11 @@ struct annf_ctx {
                           &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
-                          &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part};
+                          &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
+                          &enc_b2, &enc_w3, &enc_b3};
         for (DevBuf *b : list) b->release();
         has_tree = false;
+        has_encoder = false;
     }
 };
 
@@ -566,12 +572,18 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                       int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
                       int K, float *red_a_out, float *red_b_out, FieldShape &fs) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
-    if (!img_a || !img_b || !basis || !mean) return fail(ANNF_EINVAL, "null required pointer");
+    if (!img_a || !img_b) return fail(ANNF_EINVAL, "null image pointer");
     RCHK(set_device(c));
     const int p = psize;
     if (p < 1 || ha < p || wa < p || hb < p || wb < p) return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
     if (d < 1 || d > ANNF_MAX_D) return fail(ANNF_EUNSUPPORTED, "dimension %d outside [1, %d]", d, ANNF_MAX_D);
-    if (projection != 0 && projection != 1) return fail(ANNF_EINVAL, "projection %d is not 0 (FP32) or 1 (FP64)", projection);
+    if (projection != ANNF_PROJ_PCA32 && projection != ANNF_PROJ_PCA64 && projection != ANNF_PROJ_ENCODER)
+        return fail(ANNF_EINVAL, "projection %d is not ANNF_PROJ_PCA32, _PCA64 or _ENCODER", projection);
+    const bool enc = projection == ANNF_PROJ_ENCODER;
+    if (!enc && (!basis || !mean)) return fail(ANNF_EINVAL, "null basis or mean");
+    if (enc && (!c->has_encoder || c->enc_p != p || c->enc_d != d))
+        return fail(ANNF_EINVAL, "no encoder for patch size %d and dimension %d on this handle (annf_set_encoder)",
+                    p, d);
     if (!k_supported(K)) return fail(ANNF_EUNSUPPORTED, "n_neighbors %d outside [1, %d]", K, ANNF_MAX_K);
     const int D = p * p * 3;
     const int nr_a = ha - p + 1, nc_a = wa - p + 1, nr_b = hb - p + 1, nc_b = wb - p + 1;
@@ -581,24 +593,38 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     c->tree_depth = tree_depth;
     RCHK(check_tree_params(c, (int)n_b, d, tree_depth));
     const int dpad = dpad_of(d);
+    const PatchGeom ga{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa, 3LL * wa * ha};
+    const PatchGeom gb{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb, 3LL * wb * hb};
     c->reset_marks();
     c->mark(0);
-    RCHK(c->basis32.ensure((size_t)D * dpad * sizeof(float)));
-    RCHK(c->mean32.ensure((size_t)D * sizeof(float)));
-    RCHK(c->basis64.ensure((size_t)D * dpad * sizeof(double)));
-    RCHK(c->mean64.ensure((size_t)D * sizeof(double)));
-    hipLaunchKernelGGL(k_prep_basis, dim3((D * dpad + 255) / 256), dim3(256), 0, c->stream, basis, mean, d, D, dpad,
-                       c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
-                       c->mean64.as<double>());
-    HIPCHK(hipGetLastError());
+    if (!enc) {
+        RCHK(c->basis32.ensure((size_t)D * dpad * sizeof(float)));
+        RCHK(c->mean32.ensure((size_t)D * sizeof(float)));
+        RCHK(c->basis64.ensure((size_t)D * dpad * sizeof(double)));
+        RCHK(c->mean64.ensure((size_t)D * sizeof(double)));
+        hipLaunchKernelGGL(k_prep_basis, dim3((D * dpad + 255) / 256), dim3(256), 0, c->stream, basis, mean, d, D,
+                           dpad, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
+                           c->mean64.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    auto project = [&](const uint8_t *img, int h, int w, const PatchGeom &g, long long n, float *out) -> int {
+        if (enc) {
+            HIPCHK(encode_patches(g, n, c->enc_w1.as<__bf16>(), c->enc_b1.as<float>(), c->enc_w2.as<__bf16>(),
+                                  c->enc_b2.as<float>(), c->enc_w3.as<__bf16>(), c->enc_b3.as<float>(), d, dpad, out,
+                                  c->stream));
+        } else {
+            HIPCHK(project_patches(img, h, w, p, c->basis32.as<float>(), c->mean32.as<float>(),
+                                   c->basis64.as<double>(), c->mean64.as<double>(), d, dpad, projection, out,
+                                   c->stream));
+        }
+        return ANNF_OK;
+    };
     RCHK(c->qbuf.ensure((size_t)n_a * dpad * sizeof(float)));
-    HIPCHK(project_patches(img_a, ha, wa, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
-                           c->mean64.as<double>(), d, dpad, projection, c->qbuf.as<float>(), c->stream));
+    RCHK(project(img_a, ha, wa, ga, n_a, c->qbuf.as<float>()));
     c->mark(1);
     c->has_tree = false;
     RCHK(c->pts_a.ensure((size_t)n_b * dpad * sizeof(float)));
-    HIPCHK(project_patches(img_b, hb, wb, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
-                           c->mean64.as<double>(), d, dpad, projection, c->pts_a.as<float>(), c->stream));
+    RCHK(project(img_b, hb, wb, gb, n_b, c->pts_a.as<float>()));
     if (red_b_out) {
         hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n_b * d + 255) / 256)), dim3(256), 0, c->stream,
                            c->pts_a.as<float>(), n_b, d, dpad, red_b_out);
@@ -616,8 +642,8 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     fs.nc_a = nc_a;
     fs.n_a = n_a;
     fs.n_b = n_b;
-    fs.ga = PatchGeom{img_a, 1, nc_a, wa, p, 3 * p, 3LL * wa, 3LL * wa * ha};
-    fs.gb = PatchGeom{img_b, 1, nc_b, wb, p, 3 * p, 3LL * wb, 3LL * wb * hb};
+    fs.ga = ga;
+    fs.gb = gb;
     return ANNF_OK;
 }
 
@@ -626,14 +652,14 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
 extern "C" {
 
 int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
-                   int psize, const float *basis, const double *mean, int d, int proj_fp64, int tree_depth,
+                   int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
                    int n_neighbors, int propagate, int select_best_nn, int *nn_out, float *dist_out, int *knn_out,
                    float *red_a_out, float *red_b_out) {
     if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
     if (propagate && (long long)(ha - psize + 1) * (wa - psize + 1) > (long long)(hb - psize + 1) * (wb - psize + 1))
         return fail(ANNF_EINVAL, "propagation needs at most as many A patches as B patches");
     FieldShape fs;
-    RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, proj_fp64, tree_depth,
+    RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, projection, tree_depth,
                            n_neighbors, red_a_out, red_b_out, fs));
     RCHK(run_rows(c, (int)fs.n_a, fs.nr_a, fs.nc_a, n_neighbors, propagate, select_best_nn, fs.ga, fs.gb, nn_out,
                   dist_out, knn_out));
@@ -710,6 +736,80 @@ int annf_exact_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, cons
     HIPCHK(exact_sample(c->ev_am.as<uint8_t>(), c->ev_an.as<uint32_t>(), query_idx, nq, c->ev_bm.as<uint8_t>(),
                         c->ev_bn.as<uint32_t>(), n_b, dpad_b, k, c->ev_part.as<uint64_t>(), out_idx, out_d2,
                         c->stream));
+    return sync_stream(c);
+}
+
+int annf_mlp_forward_dev(annf_ctx *c, const uint8_t *x, int n, int D, const float *params, int h1, int h2, int d,
+                         float *out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (n < 0 || D < 1 || h1 < 1 || h2 < 1 || d < 1) return fail(ANNF_EINVAL, "bad MLP shape");
+    if (n > 0 && (!x || !params || !out)) return fail(ANNF_EINVAL, "null pointer");
+    RCHK(set_device(c));
+    if (n == 0) return ANNF_OK;
+    RCHK(c->mlp_ws.ensure(mlp_workspace_floats(n, h1, h2, d) * sizeof(float)));
+    HIPCHK(mlp_forward(x, n, D, params, h1, h2, d, c->mlp_ws.as<float>(), out, c->stream));
+    return sync_stream(c);
+}
+
+int annf_mlp_triplet_grad_dev(annf_ctx *c, const uint8_t *x, int B, int D, const float *params, int h1, int h2,
+                              int d, float margin, float *grads, float *loss) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (B < 1 || D < 1 || h1 < 1 || h2 < 1 || d < 1) return fail(ANNF_EINVAL, "bad MLP shape or empty batch");
+    if (!x || !params || !grads) return fail(ANNF_EINVAL, "null pointer");
+    if (3LL * B > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "batch too large");
+    RCHK(set_device(c));
+    RCHK(c->mlp_ws.ensure(mlp_workspace_floats(3LL * B, h1, h2, d) * sizeof(float)));
+    HIPCHK(mlp_triplet_grad(x, B, D, params, h1, h2, d, margin, grads, loss, c->mlp_ws.as<float>(), c->stream));
+    return sync_stream(c);
+}
+
+int annf_mlp_adam_dev(annf_ctx *c, float *params, const float *grads, float *m, float *v, long long n, float lr,
+                      float beta1, float beta2, float eps, int step) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (n < 0 || step < 1) return fail(ANNF_EINVAL, "bad Adam arguments");
+    if (n > 0 && (!params || !grads || !m || !v)) return fail(ANNF_EINVAL, "null pointer");
+    RCHK(set_device(c));
+    if (n == 0) return ANNF_OK;
+    HIPCHK(mlp_adam(params, grads, m, v, n, lr, beta1, beta2, eps, step, c->stream));
+    return sync_stream(c);
+}
+
+int annf_set_encoder(annf_ctx *c, const float *params, int psize, int h1, int h2, int d) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!params) return fail(ANNF_EINVAL, "null parameters");
+    if (!encoder_supported(psize, h1, h2, d))
+        return fail(ANNF_EUNSUPPORTED, "encoder p=%d %d-%d-%d: fused inference needs 128-128 hidden layers, d <= 32 "
+                    "and a padded patch of at most 512 bytes", psize, h1, h2, d);
+    RCHK(set_device(c));
+    const int DK = encoder_dk(psize);
+    RCHK(c->enc_w1.ensure((size_t)h1 * DK * sizeof(__bf16)));
+    RCHK(c->enc_b1.ensure((size_t)h1 * sizeof(float)));
+    RCHK(c->enc_w2.ensure((size_t)h2 * h1 * sizeof(__bf16)));
+    RCHK(c->enc_b2.ensure((size_t)h2 * sizeof(float)));
+    RCHK(c->enc_w3.ensure((size_t)32 * h2 * sizeof(__bf16)));
+    RCHK(c->enc_b3.ensure((size_t)32 * sizeof(float)));
+    HIPCHK(pack_encoder(params, psize, h1, h2, d, c->enc_w1.as<__bf16>(), c->enc_b1.as<float>(),
+                        c->enc_w2.as<__bf16>(), c->enc_b2.as<float>(), c->enc_w3.as<__bf16>(), c->enc_b3.as<float>(),
+                        c->stream));
+    RCHK(sync_stream(c));
+    c->has_encoder = true;
+    c->enc_p = psize;
+    c->enc_d = d;
+    return ANNF_OK;
+}
+
+int annf_encode_image_dev(annf_ctx *c, const uint8_t *img, int h, int w, int psize, float *out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!img || !out) return fail(ANNF_EINVAL, "null pointer");
+    if (!c->has_encoder || c->enc_p != psize) return fail(ANNF_EINVAL, "no encoder for patch size %d", psize);
+    if (psize < 1 || h < psize || w < psize) return fail(ANNF_EINVAL, "patch size %d larger than the image", psize);
+    RCHK(set_device(c));
+    const int nc = w - psize + 1;
+    const long long n = (long long)(h - psize + 1) * nc;
+    const PatchGeom g{img, 1, nc, w, psize, 3 * psize, 3LL * w, 3LL * w * h};
+    HIPCHK(encode_patches(g, n, c->enc_w1.as<__bf16>(), c->enc_b1.as<float>(), c->enc_w2.as<__bf16>(),
+                          c->enc_b2.as<float>(), c->enc_w3.as<__bf16>(), c->enc_b3.as<float>(), c->enc_d, c->enc_d,
+                          out, c->stream));
     return sync_stream(c);
 }
 

@@ -105,6 +105,24 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
                    const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int *knn, int *nn,
                    float *dist, hipStream_t st);
 
+// learned encoder (annf_mlp.hip): parameters W1 (h1 x D), b1, W2 (h2 x h1),
+// b2, W3 (d x h2), b3 in one FP32 buffer; inputs (x - 128) / 128
+long long mlp_param_count(int D, int h1, int h2, int d);
+size_t mlp_workspace_floats(long long n, int h1, int h2, int d);
+hipError_t mlp_forward(const uint8_t *x, long long n, int D, const float *params, int h1, int h2, int d,
+                       float *ws, float *out, hipStream_t st);
+hipError_t mlp_triplet_grad(const uint8_t *x, int B, int D, const float *params, int h1, int h2, int d,
+                            float margin, float *grads, float *loss, float *ws, hipStream_t st);
+hipError_t mlp_adam(float *p, const float *g, float *m, float *v, long long n, float lr, float b1, float b2,
+                    float eps, int step, hipStream_t st);
+int encoder_dk(int p);
+bool encoder_supported(int p, int h1, int h2, int d);
+hipError_t pack_encoder(const float *params, int p, int h1, int h2, int d, __bf16 *W1p, float *b1, __bf16 *W2,
+                        float *b2, __bf16 *W3p, float *b3p, hipStream_t st);
+hipError_t encode_patches(const PatchGeom &g, long long npatch, const __bf16 *W1p, const float *b1,
+                          const __bf16 *W2, const float *b2, const __bf16 *W3p, const float *b3p, int d, int dpad,
+                          float *out, hipStream_t st);
+
 // exact field (annf_eval.hip)
 hipError_t patch_matrix(const uint8_t *img, int h, int w, int p, uint8_t *out, int dpad_b, hipStream_t st);
 hipError_t patch_norms(const uint8_t *m, long long n, int dpad_b, uint32_t *nrm, hipStream_t st);

@@ -1,14 +1,27 @@
 """Throughput of the nearest-neighbour-field hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--nprobe 1]
+                    [--encoder annfmp_amd/data/encoder_p11_d16.npz]
 
-A step is one full field on the 1920x1080 synthetic pair (p = 11 -> 363-dim
-patches, PCA-16, leaf 64, K = 5, propagation on, best-of-K on): fused patch
-extraction + FP32 projection of both images, tree build over B, row 0 exact
-search, natural-leaf scan, propagation sweep, index conversion and best-of-K,
-from device-resident uint8 images to the int32 field.  The PCA basis is
-fitted once on 50,000 sampled A patches (FP64 covariance, outside the timed
-region).  Prints one JSON line.
+Workload: the 1920x1080 synthetic warp pair (seed 0), p = 11 (363-byte
+patches), d = 16, leaf 64 (depth 14), K = 5.  A step is one whole field,
+from device-resident uint8 images to the int32 field, for one of two paths:
+
+* learned + BF16 (the measured `value`): fused extraction + MLP encoder
+  (363-128-128-16, BF16 on MFMA) for A and B, tree build over B's
+  embeddings, natural leaf of every A patch, full-patch k-NN over that leaf
+  on MFMA (BF16 operands, exact integer distances).  PCA-16 replaces the
+  encoder when no trained encoder is given or when the encoder's recall@5
+  falls below PCA's on this pair (the fallback);
+* plain FP32 (reported beside it): the reference's method -- PCA-16
+  projection in FP32, tree, row 0 exact search, natural-leaf scan,
+  propagation sweep, best-of-K -- bit-exact to the openmp backend.
+
+W untimed steps, then K timed steps bracketed by device synchronisation,
+for each path.  Quality (recall@5 and mean distance ratio against the
+exact field on 8192 sampled A patches) is measured outside the timed
+region.  The PCA basis is fitted once on 50,000 sampled A patches (FP64).
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -24,9 +37,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
-STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_scan", "row0_exact", "row_sweep", "best_of_k"]
-KERNEL_OF_STAGE = {"project_a": "k_project", "project_b": "k_project", "row0_exact": "k_exact",
-                   "row_sweep": "k_sweep", "best_of_k": "k_best", "natural_leaf_scan": "k_leafscan"}
+BF16_PEAK_TFS = 2500.0         # MI355X dense BF16 MFMA, 2.5 PFLOP/s
+PLAIN_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_scan", "row0_exact", "row_sweep",
+                "best_of_k"]
+MM_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_bucket", "leaf_knn_mfma"]
+DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz")
 
 
 def fit_basis(a, p, d, n_samples, seed):
@@ -46,27 +61,7 @@ def fit_basis(a, p, d, n_samples, seed):
     return comp, mu
 
 
-def algorithmic_bytes(stage, n_a, n_b, H, W, p, d, dpad, K, ppl):
-    """Compulsory bytes of one launch of a stage's kernel (each input read
-    once, each output written once)."""
-    if stage in ("project_a", "project_b"):
-        n = n_a if stage == "project_a" else n_b
-        return H * W * 3 + n * dpad * 4 + (p * p * 3) * (dpad + 1) * 4
-    if stage == "row_sweep":
-        # per query: its vector, the natural-leaf list (dist, pos, leaf),
-        # the K propagated leaves' points with their leaf links once each
-        # (first touch), and the list written back
-        return n_a * (4 * dpad + 12 * K + 4 * K) + n_b * (4 * dpad + 4)
-    if stage == "natural_leaf_scan":
-        return n_a * (4 * dpad + 4 + 12 * K) + n_b * (4 * dpad)
-    if stage == "best_of_k":
-        return n_a * (4 * K + 4 + 4) + n_b * 4 + 2 * H * W * 3
-    if stage == "row0_exact":
-        return 0
-    return n_b * (4 * dpad + 4) * 2
-
-
-def cpu_baseline(a, b, p, comp, mu, red_a, red_b, depth, K, gpu_nn):
+def cpu_baseline(a, b, p, red_a, red_b, depth, K, gpu_nn):
     """The oracle (C restatement of the openmp backend, OpenMP) on the same
     float32 reduced arrays: tree build + rows + conversion + best-of-K."""
     from oracle import oracle as O
@@ -81,10 +76,28 @@ def cpu_baseline(a, b, p, comp, mu, red_a, red_b, depth, K, gpu_nn):
     t2 = time.time()
     n = red_a.shape[0]
     return {
-        "value": n / (t2 - t0), "unit": "query patches/s", "cores": threads, "kind": "port",
-        "sample": f"full 1920x1080 pair, {n} queries: kdtree_fit {t1 - t0:.2f} s (serial, as the reference) + "
-                  f"rows/conversion/best-of-K {t2 - t1:.2f} s on {threads} threads, same float32 reduced arrays",
+        "value": round(n / (t2 - t0), 1), "unit": "query patches/s", "cores": threads, "kind": "port",
+        "sample": f"plain path on the full 1920x1080 pair, {n} queries: kdtree_fit {t1 - t0:.2f} s (serial, as "
+                  f"the reference) + rows/conversion/best-of-K {t2 - t1:.2f} s on {threads} threads, same float32 "
+                  f"reduced arrays as the GPU run",
     }, bool(np.array_equal(final, gpu_nn))
+
+
+def timed(fn, steps, warmup, torch, lib, h, nst):
+    """Mean ms per step over `steps` timed calls and the mean stage times."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    st = np.zeros(8, np.float32)
+    acc = np.zeros(nst)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+        lib.lib().annf_stage_times(h.h, lib.ptr(st, lib._fp), nst)
+        acc += st[:nst]
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    return (t1 - t0) * 1000.0 / steps, acc / steps
 
 
 def main():
@@ -94,6 +107,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--nprobe", type=int, default=1, help="leaves probed per query on the BF16 path")
+    ap.add_argument("--encoder", default=DEFAULT_ENCODER, help="trained encoder (.npz); 'none' forces PCA")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -101,8 +116,9 @@ def main():
         sys.exit(2)
 
     import torch
-    from annfmp_amd import synth, _lib, device as dev
+    from annfmp_amd import synth, _lib, metrics, device as dev
     from annfmp_amd.kdtree import tree_depth_for
+    from annfmp_amd.encoder import PatchEncoder
 
     p, d, leaf, K = 11, 16, 64, 5
     a, b = synth.make_pair("1080p-warp", seed=args.seed)
@@ -111,7 +127,6 @@ def main():
     n_a = (H - p + 1) * (W - p + 1)
     n_b = n_a
     depth = tree_depth_for(n_b, leaf)
-    dpad = (d + 3) // 4 * 4
 
     cuda = dev.cuda_device(0)
     ia = torch.from_numpy(a).to(cuda)
@@ -119,70 +134,93 @@ def main():
     tc = torch.from_numpy(comp.astype(np.float32)).to(cuda)
     tm = torch.from_numpy(mu.astype(np.float64)).to(cuda)
     nn = torch.empty(n_a, dtype=torch.int32, device=cuda)
+    knn = torch.empty((n_a, K), dtype=torch.int32, device=cuda)
     ra = torch.empty((n_a, d), dtype=torch.float32, device=cuda)
     rb = torch.empty((n_b, d), dtype=torch.float32, device=cuda)
     h = _lib.Handle(0)
     L = _lib.lib()
     _lib.check("annf_set_timing", L.annf_set_timing(h.h, 1))
 
-    def step(red=False):
+    encoder = None
+    if args.encoder != "none" and os.path.exists(args.encoder):
+        encoder = PatchEncoder.load(args.encoder)
+        encoder.install(h)
+
+    def plain(knn_out=None, red=False):
         _lib.check("annf_field_dev", L.annf_field_dev(
-            h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, 0, depth, K, 1, 1,
-            dev.dptr(nn), None, None, dev.dptr(ra) if red else None, dev.dptr(rb) if red else None))
+            h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, _lib.PROJ_PCA32,
+            depth, K, 1, 1, dev.dptr(nn), None, dev.dptr(knn_out), dev.dptr(ra) if red else None,
+            dev.dptr(rb) if red else None))
 
-    torch.cuda.synchronize()
-    for _ in range(args.warmup):
-        step()
-    stage_sum = np.zeros(7)
-    st = np.zeros(7, np.float32)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-        L.annf_stage_times(h.h, _lib.ptr(st, _lib._fp), 7)
-        stage_sum += st
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / args.steps
-    value = n_a * args.steps / (t1 - t0)
-    stage_ms = stage_sum / args.steps
+    def mm(proj, knn_out=None):
+        _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
+            h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, proj, depth, K,
+            args.nprobe, _lib.PREC_BF16, K, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
 
-    # one more step that also returns the reduced arrays and K lists
-    knn = torch.empty((n_a, K), dtype=torch.int32, device=cuda)
-    _lib.check("annf_field_dev", L.annf_field_dev(
-        h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, 0, depth, K, 1, 1,
-        dev.dptr(nn), None, dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
-    torch.cuda.synchronize()
-    gpu_nn = nn.cpu().numpy()
-    from annfmp_amd import metrics
-    tq = time.perf_counter()
-    quality = metrics.field_quality(a, b, p, gpu_nn, knn.cpu().numpy(), k=5, n_sample=8192, handle=h)
-    quality["eval_s"] = round(time.perf_counter() - tq, 2)
+    q = metrics.sample_queries(n_a, 8192)
+    exact = metrics.exact_knn(a, b, p, q, 5, handle=h)
 
-    dom = int(np.argmax([stage_ms[i] if STAGES[i] != "tree_build" else 0 for i in range(7)]))
-    dom_name = STAGES[dom]
-    ppl = n_b >> depth
-    abytes = algorithmic_bytes(dom_name, n_a, n_b, H, W, p, d, dpad, K, ppl)
-    achieved = abytes / (stage_ms[dom] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": KERNEL_OF_STAGE.get(dom_name, dom_name), "achieved": round(achieved, 2),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": None, "algorithmic_bytes": int(abytes), "avg_ms": round(float(stage_ms[dom]), 4)}
+    def quality():
+        torch.cuda.synchronize()
+        return metrics.field_quality(a, b, p, nn.cpu().numpy(), knn.cpu().numpy(), k=5, handle=h, exact=exact)
+
+    paths = {}
+    # plain FP32 reference method
+    ms, stage = timed(plain, args.steps, args.warmup, torch, _lib, h, 7)
+    plain(knn, red=True)
+    plain_nn = nn.cpu().numpy()
+    paths["plain_fp32"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                           "stage_ms": {s: round(float(v), 4) for s, v in zip(PLAIN_STAGES, stage)}, **quality()}
+    # PCA + BF16 full-patch leaf k-NN
+    ms, stage = timed(lambda: mm(_lib.PROJ_PCA32), args.steps, args.warmup, torch, _lib, h, 5)
+    mm(_lib.PROJ_PCA32, knn)
+    paths["pca_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                         "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)}, **quality()}
+    chosen = "pca_bf16"
+    if encoder is not None:
+        ms, stage = timed(lambda: mm(_lib.PROJ_ENCODER), args.steps, args.warmup, torch, _lib, h, 5)
+        mm(_lib.PROJ_ENCODER, knn)
+        paths["learned_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                                 "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)},
+                                 **quality()}
+        if paths["learned_bf16"]["recall@5"] >= paths["pca_bf16"]["recall@5"]:
+            chosen = "learned_bf16"
+    main_path = paths[chosen]
+    ms_per_step = main_path["ms_per_step"]
+    value = n_a / ms_per_step * 1e3
+
+    # roofline of the dominant kernel of the measured path: the MFMA leaf
+    # k-NN (stage marks around its one launch); algorithmic work = 2 D flops
+    # per (query, leaf point) pair of the probed leaves
+    leaf_ms = main_path["stage_ms"]["leaf_knn_mfma"]
+    flops = 2.0 * (p * p * 3) * n_a * args.nprobe * (n_b / (1 << depth))
+    achieved = flops / (leaf_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "k_leaf_mm (BF16)", "achieved": round(achieved, 3),
+                "peak": BF16_PEAK_TFS, "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFS, 6),
+                "traffic": None, "algorithmic_flops": flops, "avg_ms": round(float(leaf_ms), 4)}
 
     cpu = None
     parity = None
     if not args.no_cpu:
-        cpu, parity = cpu_baseline(a, b, p, comp, mu, ra.cpu().numpy(), rb.cpu().numpy(), depth, K, gpu_nn)
+        cpu, parity = cpu_baseline(a, b, p, ra.cpu().numpy(), rb.cpu().numpy(), depth, K, plain_nn)
 
+    plain_rec = paths["plain_fp32"]["recall@5"]
     out = {
         "metric": "query patches/sec, 1080p pair: learned+BF16 vs plain FP32 at equal recall@5",
         "value": round(value, 1), "unit": "query patches/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": "plain FP32 path: 1920x1080 warp pair, p=11 (363-dim), PCA-16 FP32 projection, "
-                               "leaf 64 (depth 14), K=5, propagation + best-of-K",
-                   "queries_per_step": n_a, "seed": args.seed},
-        "stage_ms": {s: round(float(v), 4) for s, v in zip(STAGES, stage_ms)},
-        "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity, "quality": quality,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16", "data": "synthetic",
+        "config": {"workload": ("1920x1080 warp pair (seed %d), p=11 (363-dim), %s + BF16 full-patch leaf k-NN on "
+                                "MFMA, leaf 64 (depth 14), %d leaf probed, K=5; plain FP32 path (PCA-16, "
+                                "propagation + best-of-K) timed beside it")
+                               % (args.seed, "learned encoder 363-128-128-16" if chosen == "learned_bf16"
+                                  else "PCA-16", args.nprobe),
+                   "queries_per_step": n_a, "seed": args.seed, "path": chosen,
+                   "encoder": os.path.relpath(args.encoder, ROOT) if encoder is not None else None},
+        "speedup_vs_plain_fp32": round(value / paths["plain_fp32"]["query_patches_per_s"], 3),
+        "recall@5_vs_plain_fp32": [round(main_path["recall@5"], 4), round(plain_rec, 4)],
+        "recall_at_least_plain": bool(main_path["recall@5"] >= plain_rec),
+        "paths": paths, "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity,
     }
     print(json.dumps(out), flush=True)
 

@@ -103,13 +103,19 @@ int annf_tree_info(annf_ctx *ctx, int *n, int *d, int *depth);
 int annf_tree_export(annf_ctx *ctx, int *axis, float *split, int *leaves,
                      int *xi_orig, float *xi_pts, int *pli);
 
+#define ANNF_PROJ_PCA32 0
+#define ANNF_PROJ_PCA64 1
+#define ANNF_PROJ_ENCODER 2
+
 /* ---- image-level fused field (the openmp backend's fit,
  *      annfmp/openmp/base.py:53-150, from projection onwards) -----------
  * All pointers are device pointers.  img_a is ha x wa x 3, img_b is
  * hb x wb x 3 (uint8, row-major).  basis is d x (p*p*3) float32 (PCA
  * components, sklearn's components_ layout), mean is p*p*3 float64.
  * Patches are projected as sum_f (x_f - mean_f) * basis[k][f], accumulated
- * in FP32 (proj_fp64 = 0) or FP64 (proj_fp64 = 1, then rounded to FP32).
+ * in FP32 (projection = ANNF_PROJ_PCA32) or FP64 (ANNF_PROJ_PCA64, then
+ * rounded to FP32), or by the handle's learned encoder (ANNF_PROJ_ENCODER,
+ * set with annf_set_encoder; basis and mean may then be NULL).
  * The tree over B's projected patches is built at depth tree_depth.
  * Outputs: nn_out (n_a int32, original B indices), optionally dist_out
  * (n_a float32: full-patch L2 of the chosen match), knn_out (n_a x K int32
@@ -118,7 +124,7 @@ int annf_tree_export(annf_ctx *ctx, int *axis, float *split, int *leaves,
 int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                    const uint8_t *img_b, int hb, int wb, int psize,
                    const float *basis, const double *mean, int d,
-                   int proj_fp64, int tree_depth, int n_neighbors,
+                   int projection, int tree_depth, int n_neighbors,
                    int propagate, int select_best_nn, int *nn_out,
                    float *dist_out, int *knn_out, float *red_a_out,
                    float *red_b_out);
@@ -126,7 +132,7 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
 /* ---- precision variants of the field (no reference counterpart: the
  *      reference's leaf scan, kdtree.c:305-352, measures distances in the
  *      projected space) ----------------------------------------------------
- * Same projection and tree as annf_field_dev (projection: 0 FP32, 1 FP64).
+ * Same projection (an ANNF_PROJ_* value) and tree as annf_field_dev.
  * Every A patch is routed to its natural leaf; the points of the leaves
  * leaf ^ 0 .. leaf ^ (nprobe - 1) (nprobe a power of two: the enclosing
  * subtree of nprobe leaves) are compared over the full p*p*3 patch as
@@ -151,6 +157,38 @@ int annf_field_mm_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                       int nprobe, int precision, int shortlist, int *nn_out,
                       float *dist_out, int *knn_out, float *red_a_out,
                       float *red_b_out);
+
+/* ---- learned patch encoder (no reference counterpart; replaces the PCA
+ *      projection of openmp/base.py:193-200 as the tree's feature map) ----
+ * An MLP D -> h1 -> h2 -> d (D = p*p*3), GELU (tanh form) after the hidden
+ * layers, BF16 operands on MFMA with FP32 accumulation.  A patch byte x
+ * enters as (x - 128) / 128.  params is one FP32 device buffer:
+ *   W1 (h1 x D), b1 (h1), W2 (h2 x h1), b2 (h2), W3 (d x h2), b3 (d),
+ * rows of W the output units, so layer l computes W_l h + b_l.
+ * x: device uint8 patch rows (n x D, sklearn extract_patches_2d order). */
+int annf_mlp_forward_dev(annf_ctx *ctx, const uint8_t *x, int n, int D,
+                         const float *params, int h1, int h2, int d,
+                         float *out);
+/* Triplet margin loss (PyTorch TripletMarginLoss(margin, p=2, eps=1e-6),
+ * mean over the batch) of the B triplets whose patches are the rows of x:
+ * B queries, then their B positives, then B negatives (3B x D).  Writes
+ * the gradient of the loss w.r.t. params into grads (same layout) and, if
+ * loss is not NULL, the loss (one device float). */
+int annf_mlp_triplet_grad_dev(annf_ctx *ctx, const uint8_t *x, int B, int D,
+                              const float *params, int h1, int h2, int d,
+                              float margin, float *grads, float *loss);
+/* One Adam step (bias-corrected, step >= 1) on n parameters in place. */
+int annf_mlp_adam_dev(annf_ctx *ctx, float *params, const float *grads,
+                      float *m, float *v, long long n, float lr, float beta1,
+                      float beta2, float eps, int step);
+/* Installs params (copied, BF16) as the handle's encoder for patch size
+ * psize; the fused kernel needs h1 = h2 = 128 and d <= 32. */
+int annf_set_encoder(annf_ctx *ctx, const float *params, int psize, int h1,
+                     int h2, int d);
+/* Fused extraction + encoding of every p x p patch of a device image
+ * (h x w x 3 uint8) into out (n x d float32, patch order of a3). */
+int annf_encode_image_dev(annf_ctx *ctx, const uint8_t *img, int h, int w,
+                          int psize, float *out);
 
 /* ---- exact field (evaluation reference; no reference counterpart) -----
  * For the nq A patches listed in query_idx (device int32 patch indices),
