@@ -150,7 +150,6 @@ def train(encoder, triplets, epochs=2, batch=1024, lr=1e-3, margin=0.1, beta1=0.
         host = np.ascontiguousarray(triplets[perm].reshape(nb, batch, 3, D).transpose(0, 2, 1, 3))
         xb = torch.from_numpy(host).to(cuda)
         torch.cuda.synchronize()
-        run = 0.0
         for i in range(nb):
             _lib.check("annf_mlp_triplet_grad_dev", L.annf_mlp_triplet_grad_dev(
                 h.h, dev.dptr(xb[i]), batch, D, dev.dptr(params), h1, h2, d, float(margin), dev.dptr(grads),
@@ -169,6 +168,54 @@ def train(encoder, triplets, epochs=2, batch=1024, lr=1e-3, margin=0.1, beta1=0.
     return encoder
 
 
+def fit_pca(a, p, d, n_samples=50000, seed=0):
+    """PCA basis (d x D) and mean from n_samples uniformly sampled A patches:
+    FP64 covariance and eigensolve, sklearn's sign rule (largest |loading|
+    of each component positive)."""
+    rng = np.random.default_rng(seed + 1000)
+    H, W, _ = a.shape
+    ys = rng.integers(0, H - p + 1, n_samples)
+    xs = rng.integers(0, W - p + 1, n_samples)
+    X = np.stack([a[y:y + p, x:x + p, :].reshape(-1) for y, x in zip(ys, xs)]).astype(np.float64)
+    mu = X.mean(0)
+    C = (X - mu).T @ (X - mu) / (n_samples - 1)
+    _, V = np.linalg.eigh(C)
+    comp = V[:, ::-1][:, :d].T.copy()
+    comp *= np.sign(comp[np.arange(d), np.abs(comp).argmax(1)])[:, None]
+    return comp, mu
+
+
+def compare_with_pca(encoder, seeds, spec="1080p-warp", leaf=64, K=5, nprobe=1, n_sample=8192, handle=None,
+                     device=0, log=None):
+    """recall@5 and mean distance ratio of the BF16 full-patch leaf k-NN with
+    the encoder's tree against PCA-d's tree on the same pairs, at equal leaf
+    visits (nprobe leaves of the same size per query).  Returns a list of
+    per-pair dicts."""
+    from .base import field_mm_on_device
+    from .kdtree import tree_depth_for
+    h = handle or _lib.Handle(device)
+    p = encoder.psize
+    out = []
+    for s in seeds:
+        a, b = synth.make_pair(spec, seed=s)
+        comp, mu = fit_pca(a, p, encoder.d, seed=s)
+        n_a = (a.shape[0] - p + 1) * (a.shape[1] - p + 1)
+        n_b = (b.shape[0] - p + 1) * (b.shape[1] - p + 1)
+        depth = tree_depth_for(n_b, leaf)
+        q = metrics.sample_queries(n_a, n_sample)
+        exact = metrics.exact_knn(a, b, p, q, 5, device=device, handle=h)
+        row = {"seed": s}
+        for name, proj in (("pca", "fp32"), ("encoder", encoder)):
+            r = field_mm_on_device(a, b, p, comp, mu, depth, K, nprobe=nprobe, precision="bf16", projection=proj,
+                                   handle=h, device=device)
+            qual = metrics.field_quality(a, b, p, r["nn"], r["knn"], k=5, n_sample=n_sample, handle=h, exact=exact)
+            row[name] = {"recall@5": qual["recall@5"], "distance_ratio": qual["distance_ratio"]}
+        out.append(row)
+        if log:
+            log(f"pair seed {s}: {row}")
+    return out
+
+
 def train_default(p=11, d=16, n_pairs=64, per_pair=31250, epochs=2, batch=1024, lr=1e-3, seed=1000, handle=None,
                   device=0, log=None):
     """The encoder of the learned workload: 64 seeded 512x512 pairs, 2M
@@ -181,3 +228,34 @@ def train_default(p=11, d=16, n_pairs=64, per_pair=31250, epochs=2, batch=1024, 
     enc = PatchEncoder(p, d, seed=seed)
     train(enc, trip, epochs=epochs, batch=batch, lr=lr, seed=seed, handle=h, device=device, log=log)
     return enc, t1 - t0, time.time() - t1
+
+
+def main(argv=None):
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description="train the learned patch encoder on the GPU")
+    ap.add_argument("--psize", type=int, default=11)
+    ap.add_argument("--d", type=int, default=16)
+    ap.add_argument("--pairs", type=int, default=64, help="seeded 512x512 training pairs")
+    ap.add_argument("--per-pair", type=int, default=31250, help="triplets per pair (64 x 31250 = 2M)")
+    ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--batch", type=int, default=1024)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=1000, help="first training pair seed (and init seed)")
+    ap.add_argument("--eval-seeds", default="0,1,2,3", help="held-out 1080p pairs for the PCA comparison")
+    ap.add_argument("--out", required=True)
+    args = ap.parse_args(argv)
+    log = lambda m: print(m, flush=True)
+    h = _lib.Handle(0)
+    enc, t_trip, t_train = train_default(args.psize, args.d, args.pairs, args.per_pair, args.epochs, args.batch,
+                                         args.lr, args.seed, handle=h, log=log)
+    seeds = [int(x) for x in args.eval_seeds.split(",") if x != ""]
+    cmp = compare_with_pca(enc, seeds, handle=h, log=log) if seeds else []
+    enc.save(args.out, triplet_seconds=t_trip, train_seconds=t_train, pairs=args.pairs, per_pair=args.per_pair,
+             epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed,
+             eval_json=json.dumps(cmp))
+    log(json.dumps({"out": args.out, "triplet_s": round(t_trip, 2), "train_s": round(t_train, 2), "eval": cmp}))
+
+
+if __name__ == "__main__":
+    main()

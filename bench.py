@@ -46,19 +46,8 @@ DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz"
 
 def fit_basis(a, p, d, n_samples, seed):
     """PCA basis from n_samples uniformly sampled A patches (FP64 covariance)."""
-    rng = np.random.default_rng(seed + 1000)
-    H, W, _ = a.shape
-    ys = rng.integers(0, H - p + 1, n_samples)
-    xs = rng.integers(0, W - p + 1, n_samples)
-    X = np.stack([a[y:y + p, x:x + p, :].reshape(-1) for y, x in zip(ys, xs)]).astype(np.float64)
-    mu = X.mean(0)
-    C = (X - mu).T @ (X - mu) / (n_samples - 1)
-    w, V = np.linalg.eigh(C)
-    comp = V[:, ::-1][:, :d].T.copy()
-    # sklearn's svd_flip sign rule: largest |loading| positive
-    sgn = np.sign(comp[np.arange(d), np.abs(comp).argmax(1)])
-    comp *= sgn[:, None]
-    return comp, mu
+    from annfmp_amd.encoder import fit_pca
+    return fit_pca(a, p, d, n_samples, seed)
 
 
 def cpu_baseline(a, b, p, red_a, red_b, depth, K, gpu_nn):
