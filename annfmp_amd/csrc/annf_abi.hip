@@ -118,7 +118,7 @@ struct annf_ctx {
         have_stages = true;
     }
     // host copies of the level bounds (kept alive for the async upload)
-    std::vector<int> h_bnd, h_off, h_cpn;
+    std::vector<int> h_bnd, h_off, h_cpn, h_maxc;
 
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
@@ -159,10 +159,11 @@ int set_device(annf_ctx *c) {
 
 // Segment bounds of every level (kdtree.c:114-115: left child gets c/2).
 void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off, std::vector<int> &cpn,
-                  int &max_leaf) {
+                  std::vector<int> &maxc_out, int &max_leaf) {
     bnd.clear();
     off.assign(depth + 1, 0);
     cpn.assign(depth + 1, 1);
+    maxc_out.assign(depth + 1, 0);
     std::vector<int> cur = {0, n}, nxt;
     for (int l = 0; l <= depth; ++l) {
         off[l] = (int)bnd.size();
@@ -170,6 +171,7 @@ void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off
         int maxc = 0;
         for (size_t j = 0; j + 1 < cur.size(); ++j) maxc = std::max(maxc, cur[j + 1] - cur[j]);
         cpn[l] = std::max(1, (maxc + 2047) / 2048);
+        maxc_out[l] = maxc;
         if (l == depth) {
             max_leaf = maxc;
             break;
@@ -186,9 +188,9 @@ void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off
 
 // Builds the tree from n x dpad padded points already in c->pts_a.
 int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
-    std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn;
+    std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn, &maxc = c->h_maxc;
     int max_leaf = 0;
-    level_bounds(n, depth, bnd, off, cpn, max_leaf);
+    level_bounds(n, depth, bnd, off, cpn, maxc, max_leaf);
     const size_t nodes_max = depth > 0 ? ((size_t)1 << (depth - 1)) : 1;
     size_t chunks_max = 1;
     for (int l = 0; l < depth; ++l) chunks_max = std::max(chunks_max, ((size_t)1 << l) * cpn[l]);
@@ -231,7 +233,7 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     s.bnd = c->bnd.as<int>();
     s.err = c->errflag.as<int>();
     TreeView t;
-    HIPCHK(build_tree(n, d, dpad, depth, c->splitting_type, bnd.data(), off.data(), cpn.data(), s,
+    HIPCHK(build_tree(n, d, dpad, depth, c->splitting_type, bnd.data(), off.data(), cpn.data(), maxc.data(), s,
                       c->axis.as<int>(), c->split.as<float>(), c->leaves.as<int>(), c->pli.as<int>(), t,
                       c->stream));
     t.max_leaf = max_leaf;

@@ -61,9 +61,10 @@ hipError_t iota(int *dst, int n, hipStream_t st);
 
 // Builds the tree; `t` receives the device pointers of the result (inside
 // scratch ping-pong buffers and the given node/leaf arrays).
+// host_maxc[l]: the largest node population of level l.
 hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
-                      BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
+                      const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
                       TreeView &t, hipStream_t st);
 
 // Natural leaf of every query (leaf), optional per-leaf counts, and optional

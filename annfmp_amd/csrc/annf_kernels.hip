@@ -509,6 +509,290 @@ __global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// the last levels of the build, one subtree per block
+// ---------------------------------------------------------------------------
+// Once every node of a level holds at most kSubMax points, one 1024-thread
+// block builds the remaining (at most kSubLevels) levels of each node's
+// subtree with the same rules as the level kernels above -- data box of
+// ordered keys, first strictly longest FP32 width (or level % d), radix
+// select of the rank-c/2 key in four 8-bit passes, the closed-form partition
+// order -- on a permutation of the node's points held in LDS.  Point data
+// are read from the level's arrays (L2-resident) through the permutation and
+// written once, in final order, at the end.
+constexpr int kSubMax = 4096;
+constexpr int kSubLevels = 5;
+constexpr int kSubTPB = 1024;
+constexpr int kSubNodes = 1 << (kSubLevels - 1);   // sub-nodes of the last level done in a block
+static_assert(kSubNodes * 256 == kSubMax, "radix histograms alias the eqoff array");
+static_assert(kSubMax == 4 * kSubTPB, "four contiguous points per thread in the partition scan");
+
+__device__ __forceinline__ int sub_of(const int *sb, int nsub, int i) {
+    int a = 0, b = nsub - 1;
+    while (a < b) {
+        const int m = (a + b + 1) >> 1;
+        if (sb[m] <= i) a = m; else b = m - 1;
+    }
+    return a;
+}
+
+template <int DP>
+__global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
+                                                     const int *__restrict__ bnd0, int L0, int R, int d,
+                                                     int splitting_type, float *__restrict__ pts_out,
+                                                     int *__restrict__ orig_out, int *__restrict__ axis_out,
+                                                     float *__restrict__ split_out, int *err) {
+    __shared__ int perm[2][kSubMax];
+    __shared__ uint32_t keys[kSubMax];
+    __shared__ int hist_eq[kSubMax];                 // radix histograms, then eqoff
+    __shared__ uint32_t sbox[kSubNodes][DP][2];
+    __shared__ int sb[kSubNodes + 1];                // sub-node bounds (local offsets)
+    __shared__ int sax[kSubNodes];
+    __shared__ uint32_t spre[kSubNodes];
+    __shared__ int srank[kSubNodes];
+    __shared__ int PL[kSubNodes + 1], PE[kSubNodes + 1];
+    __shared__ int wlt[kSubTPB / kWave], weq[kSubTPB / kWave];
+
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
+    const int node = blockIdx.x;
+    const int lo = bnd0[node], c0 = bnd0[node + 1] - lo;
+    for (int i = tid; i < c0; i += kSubTPB) perm[0][i] = i;
+    if (tid == 0) {
+        sb[0] = 0;
+        sb[1] = c0;
+    }
+    int cur = 0;
+    for (int r = 0; r < R; ++r) {
+        const int nsub = 1 << r;
+        const int lvl = L0 + r;
+        const int heap0 = (1 << lvl) - 1 + node * nsub;
+        if (tid < nsub * DP) {
+            sbox[tid / DP][tid % DP][0] = 0xFFFFFFFFu;
+            sbox[tid / DP][tid % DP][1] = 0u;
+        }
+        __syncthreads();
+        // (A) data boxes: chunks of 64 points of one sub-node per wavefront
+        if (splitting_type != 0) {
+            int nch = 0;
+            for (int s2 = 0; s2 < nsub; ++s2) nch += (sb[s2 + 1] - sb[s2] + kWave - 1) / kWave;
+            for (int q = wid; q < nch; q += kSubTPB / kWave) {
+                int s2 = 0, base = 0, acc = 0;
+                for (; s2 < nsub; ++s2) {
+                    const int ch = (sb[s2 + 1] - sb[s2] + kWave - 1) / kWave;
+                    if (q < acc + ch) {
+                        base = sb[s2] + (q - acc) * kWave;
+                        break;
+                    }
+                    acc += ch;
+                }
+                const int i = base + lane;
+                const bool valid = i < sb[s2 + 1];
+                float v[DP];
+                if (valid) {
+                    load_vec<DP>(pts_in + (size_t)(lo + perm[cur][i]) * DP, v);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < DP; ++j) v[j] = 0.0f;
+                }
+#pragma unroll
+                for (int j = 0; j < DP; ++j) {
+                    if (j >= d) break;
+                    const uint32_t k = f2key(v[j]);
+                    uint32_t mn = valid ? k : 0xFFFFFFFFu;
+                    uint32_t mx = valid ? k : 0u;
+#pragma unroll
+                    for (int m = kWave / 2; m >= 1; m >>= 1) {
+                        mn = min(mn, (uint32_t)__shfl_xor((int)mn, m, kWave));
+                        mx = max(mx, (uint32_t)__shfl_xor((int)mx, m, kWave));
+                    }
+                    if (lane == 0) {
+                        atomicMin(&sbox[s2][j][0], mn);
+                        atomicMax(&sbox[s2][j][1], mx);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        // (B) split axis per sub-node
+        if (tid < nsub) {
+            int ax = 0;
+            if (splitting_type == 0) {
+                ax = lvl % d;
+            } else {
+                float longest = 0.0f;
+                for (int k = 0; k < d; ++k) {
+                    const float w = key2f(sbox[tid][k][1]) - key2f(sbox[tid][k][0]);
+                    if (w > longest) { longest = w; ax = k; }
+                }
+            }
+            sax[tid] = ax;
+            axis_out[heap0 + tid] = ax;
+            spre[tid] = 0u;
+            srank[tid] = (sb[tid + 1] - sb[tid]) / 2;
+        }
+        __syncthreads();
+        // (C) split-axis keys
+        for (int i = tid; i < c0; i += kSubTPB) {
+            const int s2 = sub_of(sb, nsub, i);
+            keys[i] = f2key(pts_in[(size_t)(lo + perm[cur][i]) * DP + sax[s2]]);
+        }
+        // (D) radix select of the rank-c/2 key of every sub-node
+        for (int pass = 0; pass < 4; ++pass) {
+            for (int i = tid; i < nsub * 256; i += kSubTPB) hist_eq[i] = 0;
+            __syncthreads();
+            const int dsh = 24 - 8 * pass;
+            for (int i = tid; i < c0; i += kSubTPB) {
+                const int s2 = sub_of(sb, nsub, i);
+                const uint32_t k = keys[i];
+                if (pass == 0 || ((k ^ spre[s2]) >> (dsh + 8)) == 0u)
+                    atomicAdd(&hist_eq[s2 * 256 + ((k >> dsh) & 255u)], 1);
+            }
+            __syncthreads();
+            if (wid < nsub) {
+                const int *h = hist_eq + wid * 256;
+                const uint32_t c_0 = h[lane * 4 + 0], c_1 = h[lane * 4 + 1], c_2 = h[lane * 4 + 2],
+                               c_3 = h[lane * 4 + 3];
+                const uint32_t tot = c_0 + c_1 + c_2 + c_3;
+                uint32_t inc = tot;
+#pragma unroll
+                for (int o = 1; o < kWave; o <<= 1) {
+                    const uint32_t t = (uint32_t)__shfl_up((int)inc, o, kWave);
+                    if (lane >= o) inc += t;
+                }
+                const uint32_t exc = inc - tot;
+                const uint32_t rr = (uint32_t)srank[wid];
+                const bool mine = rr >= exc && rr < inc;
+                int digit = 0;
+                uint32_t before = exc;
+                if (mine) {
+                    if (rr < exc + c_0) { digit = lane * 4 + 0; before = exc; }
+                    else if (rr < exc + c_0 + c_1) { digit = lane * 4 + 1; before = exc + c_0; }
+                    else if (rr < exc + c_0 + c_1 + c_2) { digit = lane * 4 + 2; before = exc + c_0 + c_1; }
+                    else { digit = lane * 4 + 3; before = exc + c_0 + c_1 + c_2; }
+                }
+                const unsigned long long bal = __ballot(mine);
+                const int owner = __ffsll(bal) - 1;
+                digit = readlane_i(digit, owner);
+                before = (uint32_t)readlane_i((int)before, owner);
+                if (lane == 0) {
+                    spre[wid] |= (uint32_t)digit << dsh;
+                    srank[wid] = (int)(rr - before);
+                }
+            }
+            __syncthreads();
+        }
+        // (E) '<' and '==' counts in segment order: four contiguous points
+        // per thread, block-wide exclusive scan over the concatenated
+        // sub-node segments
+        int fl[4], fe[4];
+        int tl = 0, te = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = 4 * tid + u;
+            fl[u] = fe[u] = 0;
+            if (i < c0) {
+                const uint32_t v = spre[sub_of(sb, nsub, i)];
+                fl[u] = keys[i] < v;
+                fe[u] = keys[i] == v;
+            }
+            tl += fl[u];
+            te += fe[u];
+        }
+        int il = tl, ie = te;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int a = __shfl_up(il, o, kWave), b2 = __shfl_up(ie, o, kWave);
+            if (lane >= o) { il += a; ie += b2; }
+        }
+        if (lane == kWave - 1) { wlt[wid] = il; weq[wid] = ie; }
+        __syncthreads();
+        int ol = 0, oe = 0;
+        for (int w = 0; w < wid; ++w) { ol += wlt[w]; oe += weq[w]; }
+        int xl = il - tl + ol, xe = ie - te + oe;   // exclusive prefix of point 4*tid
+        int pl[4], pe[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            pl[u] = xl;
+            pe[u] = xe;
+            const int i = 4 * tid + u;
+            for (int s2 = 0; s2 <= nsub; ++s2)
+                if (sb[s2] == i && i < c0) { PL[s2] = xl; PE[s2] = xe; }
+            xl += fl[u];
+            xe += fe[u];
+        }
+        if (tid == kSubTPB - 1) { PL[nsub] = xl; PE[nsub] = xe; }
+        __syncthreads();
+        // (F) eqoff[t] = G-offset of the t-th equal point in G order
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = 4 * tid + u;
+            if (i >= c0 || !fe[u]) continue;
+            const int s2 = sub_of(sb, nsub, i);
+            const int c = sb[s2 + 1] - sb[s2], off = i - sb[s2];
+            const int Ls = PL[s2 + 1] - PL[s2], Es = PE[s2 + 1] - PE[s2];
+            const int rl = pl[u] - PL[s2], re = pe[u] - PE[s2];
+            const int go = (c - Ls - 1) - (off - rl);
+            const int t = (Es - 1) - re;
+            if (t < 0 || t >= c) { atomicOr(err, 2); continue; }
+            hist_eq[sb[s2] + t] = go;
+        }
+        __syncthreads();
+        // (G) slots
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int i = 4 * tid + u;
+            if (i >= c0) continue;
+            const int s2 = sub_of(sb, nsub, i);
+            const int c = sb[s2 + 1] - sb[s2], off = i - sb[s2];
+            const int Ls = PL[s2 + 1] - PL[s2], Es = PE[s2 + 1] - PE[s2];
+            const int rl = pl[u] - PL[s2], re = pe[u] - PE[s2];
+            int slot;
+            if (fl[u]) {
+                slot = rl;
+            } else {
+                const int go = (c - Ls - 1) - (off - rl);
+                if (fe[u]) {
+                    slot = Ls + (Es - 1) - re;
+                } else {
+                    int q = go;
+                    for (int it = 0; q < Es && it <= Es; ++it) q = hist_eq[sb[s2] + q];
+                    slot = Ls + q;
+                }
+            }
+            if (slot < 0 || slot >= c) { atomicOr(err, 1); continue; }
+            const int pidx = perm[cur][i];
+            perm[cur ^ 1][sb[s2] + slot] = pidx;
+            if (slot == c / 2) split_out[heap0 + s2] = pts_in[(size_t)(lo + pidx) * DP + sax[s2]];
+        }
+        __syncthreads();
+        // children bounds for the next level: the left child gets c/2 points
+        // (kdtree.c:114-115); 2 * nsub <= kSubNodes there
+        if (r + 1 < R) {
+            int b_lo = 0, b_hi = 0;
+            if (tid < nsub) {
+                b_lo = sb[tid];
+                b_hi = sb[tid + 1];
+            }
+            __syncthreads();
+            if (tid < nsub) {
+                sb[2 * tid] = b_lo;
+                sb[2 * tid + 1] = b_lo + (b_hi - b_lo) / 2;
+            }
+            if (tid == 0) sb[2 * nsub] = c0;
+        }
+        cur ^= 1;
+        __syncthreads();
+    }
+    for (int i = tid; i < c0; i += kSubTPB) {
+        const int pidx = perm[cur][i];
+        const float4 *src = reinterpret_cast<const float4 *>(pts_in + (size_t)(lo + pidx) * DP);
+        float4 *dst = reinterpret_cast<float4 *>(pts_out + (size_t)(lo + i) * DP);
+#pragma unroll
+        for (int q = 0; q < DP / 4; ++q) dst[q] = src[q];
+        orig_out[lo + i] = orig_in[lo + pidx];
+    }
+}
+
 // Leaf bounds and point -> leaf map (kdtree.c:86-100).
 __global__ void k_leaves(const int *bnd, int nleaves, const int *orig, int n, int *leaves, int *pli) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -525,13 +809,18 @@ __global__ void k_leaves(const int *bnd, int nleaves, const int *orig, int n, in
 
 hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
-                      BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
+                      const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
                       TreeView &t, hipStream_t st) {
     (void)host_bnd;
     float *pin = s.pts_a, *pout = s.pts_b;
     int *oin = s.orig_a, *oout = s.orig_b;
     hipError_t err = hipSuccess;
-    for (int lvl = 0; lvl < depth; ++lvl) {
+    // first level whose nodes all fit one subtree block, at most kSubLevels
+    // above the leaves
+    int L0 = depth;
+    for (int lvl = std::max(0, depth - kSubLevels); lvl < depth; ++lvl)
+        if (host_maxc[lvl] <= kSubMax) { L0 = lvl; break; }
+    for (int lvl = 0; lvl < L0; ++lvl) {
         LevelArgs L;
         L.nodes = 1 << lvl;
         L.cpn = host_cpn[lvl];
@@ -559,6 +848,18 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                                s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
             hipLaunchKernelGGL((k_scatter<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
                                s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
+            return hipGetLastError();
+        });
+        if (err != hipSuccess) return err;
+        std::swap(pin, pout);
+        std::swap(oin, oout);
+    }
+    if (L0 < depth) {
+        err = with_dp(dpad, [&](auto dp) {
+            constexpr int DP = decltype(dp)::value;
+            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin, oin,
+                               s.bnd + host_bnd_off[L0], L0, depth - L0, d, splitting_type, pout, oout, axis, split,
+                               s.err);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
