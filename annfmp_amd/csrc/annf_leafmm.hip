@@ -3,12 +3,12 @@
 // every point of the probed leaves is |a|^2 + |b|^2 - 2 a.b over the full
 // p*p*3 uint8 patch, the dot products on MFMA with FP32 accumulation:
 //
-//  * BF16: A patches centred (a - 128, in [-128, 127]) and B patches as they
-//    are (0..255) are exact in BF16; every product and every partial sum of
-//    (a - 128).b stays below 2^24 in magnitude for D <= 512, so FP32
-//    accumulation is exact in any order and the distance is the exact
-//    integer.  The K smallest (distance, B index) keys are the answer; no
-//    re-rank can change them.
+//  * BF16: bytes 0..255 are exact in BF16 and so is every product; the
+//    k-steps alternate between two FP32 accumulators, each summing at most
+//    16 steps of 16 products below 65,025 (D <= 512), so no partial sum
+//    reaches 2^24, both halves are exact integers and so is the distance.
+//    The K smallest (distance, B index) keys are the answer; no re-rank can
+//    change them.
 //  * FP8-E4M3: both patches centred and scaled per patch by 448 / max|x-128|,
 //    dot products de-scaled in FP32.  The S smallest approximate keys form a
 //    shortlist that is re-ranked with exact integer distances into the K
@@ -88,17 +88,15 @@ __device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
     return (__builtin_bit_cast(uint32_t, hi) & 0xFFFF0000u) | (__builtin_bit_cast(uint32_t, lo) >> 16);
 }
 
-// eight bytes (two dwords at p) as BF16; CENTER subtracts 128 first
-template <bool CENTER>
+// eight bytes (two dwords at p) as BF16
 __device__ __forceinline__ bf16x8 frag_bf16(const uint8_t *p) {
     const uint32_t w0 = *reinterpret_cast<const uint32_t *>(p);
     const uint32_t w1 = *reinterpret_cast<const uint32_t *>(p + 4);
-    const float c = CENTER ? 128.0f : 0.0f;
     uint4 r;
-    r.x = bf16_pair((float)(w0 & 255u) - c, (float)((w0 >> 8) & 255u) - c);
-    r.y = bf16_pair((float)((w0 >> 16) & 255u) - c, (float)(w0 >> 24) - c);
-    r.z = bf16_pair((float)(w1 & 255u) - c, (float)((w1 >> 8) & 255u) - c);
-    r.w = bf16_pair((float)((w1 >> 16) & 255u) - c, (float)(w1 >> 24) - c);
+    r.x = bf16_pair((float)(w0 & 255u), (float)((w0 >> 8) & 255u));
+    r.y = bf16_pair((float)((w0 >> 16) & 255u), (float)(w0 >> 24));
+    r.z = bf16_pair((float)(w1 & 255u), (float)((w1 >> 8) & 255u));
+    r.w = bf16_pair((float)((w1 >> 16) & 255u), (float)(w1 >> 24));
     return __builtin_bit_cast(bf16x8, r);
 }
 
@@ -136,25 +134,73 @@ __device__ __forceinline__ void klist_insert(uint64_t (&l)[LM], uint64_t key) {
         if (s == r) l[s] = key;
 }
 
-// exact squared distance of A patch i and B patch j, both read from HBM
-__device__ __forceinline__ uint32_t exact_d2(const PatchGeom &ga, long long i, const PatchGeom &gb, long long j,
-                                             int rw, uint32_t tmask) {
-    uint32_t saa = 0, sbb = 0, sab = 0;
-    const long long oa = pofs(ga, i), ob = pofs(gb, j);
-    for (int r = 0; r < ga.rows; ++r) {
-        for (int k = 0; k < rw; ++k) {
-            // dword k of row r of both patches (unaligned reads, masked tail)
-            uint32_t wa = 0, wb = 0;
-            load_row(ga, oa + r * ga.rstride + 4 * k, 1, k == rw - 1 ? tmask : ~0u,
-                     [&](int, uint32_t v) { wa = v; });
-            load_row(gb, ob + r * gb.rstride + 4 * k, 1, k == rw - 1 ? tmask : ~0u,
-                     [&](int, uint32_t v) { wb = v; });
-            saa = __builtin_amdgcn_udot4(wa, wa, saa, false);
-            sbb = __builtin_amdgcn_udot4(wb, wb, sbb, false);
-            sab = __builtin_amdgcn_udot4(wa, wb, sab, false);
+constexpr int kMaxRW = 10;   // dwords per padded patch row (p <= 12)
+
+// Row r of a patch as rw (<= kMaxRW) dwords, tail masked, into w[]; every
+// index is static (the loop runs kMaxRW times, guarded by rw).
+__device__ __forceinline__ void row_words(const PatchGeom &g, long long off, int rw, uint32_t tmask,
+                                          uint32_t (&w)[kMaxRW]) {
+    const long long al = off & ~3LL;
+    const int sh = (int)(off & 3);
+    if (al + 4LL * (rw + 1) <= g.nbytes) {
+        const uint32_t *src = reinterpret_cast<const uint32_t *>(g.base + al);
+        uint32_t prev = src[0];
+#pragma unroll
+        for (int k = 0; k < kMaxRW; ++k) {
+            if (k < rw) {
+                const uint32_t next = src[k + 1];
+                w[k] = __builtin_amdgcn_alignbyte(next, prev, sh);
+                prev = next;
+            } else {
+                w[k] = 0u;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kMaxRW; ++k) {
+            uint32_t v = 0;
+            if (k < rw)
+                for (int b = 0; b < 4; ++b) {
+                    const long long o = off + 4 * k + b;
+                    if (o < g.nbytes) v |= (uint32_t)g.base[o] << (8 * b);
+                }
+            w[k] = v;
         }
     }
-    return saa + sbb - 2u * sab;
+#pragma unroll
+    for (int k = 0; k < kMaxRW; ++k)
+        if (k == rw - 1) w[k] &= tmask;
+}
+
+// Exact squared distances of A patch i to the B patches j[0..nc) (NC >= nc,
+// static), row by row: each A row is read once for all candidates.
+template <int NC>
+__device__ __forceinline__ void exact_d2_rows(const PatchGeom &ga, long long i, const PatchGeom &gb,
+                                              const uint32_t (&j)[NC], int nc, int rw, uint32_t tmask,
+                                              uint32_t (&out)[NC]) {
+    uint32_t saa = 0, sab[NC], sbb[NC];
+    const long long oa = pofs(ga, i);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) sab[c] = sbb[c] = 0u;
+    for (int r = 0; r < ga.rows; ++r) {
+        uint32_t wa[kMaxRW];
+        row_words(ga, oa + r * ga.rstride, rw, tmask, wa);
+#pragma unroll
+        for (int k = 0; k < kMaxRW; ++k) saa = __builtin_amdgcn_udot4(wa[k], wa[k], saa, false);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            if (c >= nc) break;
+            uint32_t wb[kMaxRW];
+            row_words(gb, pofs(gb, j[c]) + r * gb.rstride, rw, tmask, wb);
+#pragma unroll
+            for (int k = 0; k < kMaxRW; ++k) {
+                sbb[c] = __builtin_amdgcn_udot4(wb[k], wb[k], sbb[c], false);
+                sab[c] = __builtin_amdgcn_udot4(wa[k], wb[k], sab[c], false);
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = saa + sbb[c] - 2u * sab[c];
 }
 
 // Stages patch idx into an LDS row (rows of rw dwords); the 4 lanes of a
@@ -317,38 +363,44 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
                         nb[i] = ss - 256u * sm + 16384u * (uint32_t)D;
                     }
                 } else {
-                    if (q4 == 0) {
-                        nb[i] = ss;
-                        nbsum[i] = (int)sm;
-                    }
+                    if (q4 == 0) nb[i] = ss;
                 }
                 if (q4 == 0) borig[i] = bj;
             }
             __syncthreads();
             // 64 x 32 block of dot products per wavefront
-            floatx16 acc0 = {}, acc1 = {};
+            // acc0 / acc1: query rows 0-31 / 32-63; BF16 splits each over
+            // even (x0) and odd (x1) k-steps
+            floatx16 acc0 = {}, acc1 = {}, acc0x = {}, acc1x = {};
             {
                 const uint8_t *ar0 = Aq + (lane & 31) * RB + 8 * (lane >> 5);
                 const uint8_t *ar1 = ar0 + 32 * RB;
                 const uint8_t *br = BD + (32 * wid + (lane & 31)) * RB + 8 * (lane >> 5);
+                if constexpr (FP8) {
 #pragma unroll 5
-                for (int k0 = 0; k0 < DK; k0 += 16) {
-                    if constexpr (FP8) {
+                    for (int k0 = 0; k0 < DK; k0 += 16) {
                         const long fb = frag_fp8(br + k0);
                         acc0 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(frag_fp8(ar0 + k0), fb, acc0, 0, 0, 0);
                         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(frag_fp8(ar1 + k0), fb, acc1, 0, 0, 0);
-                    } else {
-                        const bf16x8 fb = frag_bf16<false>(br + k0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16<true>(ar0 + k0), fb, acc0, 0, 0,
-                                                                       0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16<true>(ar1 + k0), fb, acc1, 0, 0,
-                                                                       0);
+                    }
+                } else {
+#pragma unroll 3
+                    for (int k0 = 0; k0 < DK; k0 += 32) {
+                        const bf16x8 fb = frag_bf16(br + k0);
+                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar0 + k0), fb, acc0, 0, 0, 0);
+                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar1 + k0), fb, acc1, 0, 0, 0);
+                        if (k0 + 16 < DK) {
+                            const bf16x8 fc = frag_bf16(br + k0 + 16);
+                            acc0x = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar0 + k0 + 16), fc, acc0x, 0,
+                                                                            0, 0);
+                            acc1x = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar1 + k0 + 16), fc, acc1x, 0,
+                                                                            0, 0);
+                        }
                     }
                 }
             }
             const int pcol = 32 * wid + (lane & 31);
             const uint32_t nbp = nb[pcol];
-            const int sump = nbsum[pcol];
             const float sbp = sb[pcol];
             __syncthreads();   // every wavefront is done with the staged points
             {
@@ -365,8 +417,9 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
                             const float dd = ((float)na[qq] + (float)nbp) - 2.0f * (acc / (sa[qq] * sbp));
                             d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
                         } else {
-                            // a.b = (a - 128).b + 128 sum(b); the dot is an exact integer
-                            d = na[qq] + nbp - 2u * ((uint32_t)(int)acc + 128u * (uint32_t)sump);
+                            // a.b as the sum of two exact integer halves
+                            const float accx = h ? acc1x[r] : acc0x[r];
+                            d = na[qq] + nbp - 2u * ((uint32_t)(int)acc + (uint32_t)(int)accx);
                         }
                         Dm[qq * kDS + pcol] = pv ? d : 0xFFFFFFFFu;
                     }
@@ -403,12 +456,21 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
         __syncthreads();
         // exact re-rank of the shortlist
         if (q < nqb) {
-            for (int s = q4; s < LM; s += 4) {
-                const uint64_t key = cand[q * LM + s];
-                if (key == kNoKey) continue;
-                const uint32_t j = (uint32_t)key;
-                cand[q * LM + s] = ((uint64_t)exact_d2(ga, qid[q], gb, j, rw, tmask) << 32) | j;
+            // this lane's entries q4, q4 + 4, ... of the shortlist
+            constexpr int NC = (LM + 3) / 4;
+            uint32_t jj[NC], d2[NC];
+            int nc = 0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const int s = q4 + 4 * c;
+                const uint64_t key = s < LM ? cand[q * LM + s] : kNoKey;
+                jj[c] = (uint32_t)key;
+                if (key != kNoKey) nc = c + 1;   // live entries come first
             }
+            exact_d2_rows(ga, qid[q], gb, jj, nc, rw, tmask, d2);
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                if (c < nc) cand[q * LM + q4 + 4 * c] = ((uint64_t)d2[c] << 32) | jj[c];
         }
         __syncthreads();
         if (q < nqb && q4 == 0) {
