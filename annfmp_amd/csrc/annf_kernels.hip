@@ -151,6 +151,24 @@ __device__ __forceinline__ int wave_add1(int *ctr, int key) {
     return res;
 }
 
+// ctr[key] += 1 per lane: lanes sharing the wave's first two keys add once
+// per key, the rest on their own; no return value, so nothing waits.
+__device__ __forceinline__ void wave_count1(int *ctr, int key) {
+    bool done = false;
+    for (int round = 0; round < 2; ++round) {
+        if (!done) {
+            const int lead = __builtin_amdgcn_readfirstlane(key);
+            const bool mine = key == lead;
+            const unsigned long long m = __ballot(mine);
+            if (mine) {
+                if (lane_rank(m) == 0) atomicAdd(&ctr[lead], __popcll(m));
+                done = true;
+            }
+        }
+    }
+    if (!done) atomicAdd(&ctr[key], 1);
+}
+
 template <typename Fn>
 static hipError_t with_dp(int dpad, Fn &&fn) {
     switch (dpad) {
@@ -512,20 +530,23 @@ __global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__
 // ---------------------------------------------------------------------------
 // the last levels of the build, one subtree per block
 // ---------------------------------------------------------------------------
-// Once every node of a level holds at most kSubMax points, one 1024-thread
-// block builds the remaining (at most kSubLevels) levels of each node's
-// subtree with the same rules as the level kernels above -- data box of
-// ordered keys, first strictly longest FP32 width (or level % d), radix
+// Once every node of a level holds at most sub_max<DP> points, one
+// 1024-thread block builds the remaining (at most kSubLevels) levels of each
+// node's subtree with the same rules as the level kernels above -- data box
+// of ordered keys, first strictly longest FP32 width (or level % d), radix
 // select of the rank-c/2 key in four 8-bit passes, the closed-form partition
-// order -- on a permutation of the node's points held in LDS.  Point data
-// are read from the level's arrays (L2-resident) through the permutation and
-// written once, in final order, at the end.
-constexpr int kSubMax = 4096;
-constexpr int kSubLevels = 5;
+// order -- with the node's points resident in LDS (read once) and a 16-bit
+// index permutation rearranged per level; the points are written once, in
+// final order, at the end.
+constexpr int kSubLevels = 4;
 constexpr int kSubTPB = 1024;
 constexpr int kSubNodes = 1 << (kSubLevels - 1);   // sub-nodes of the last level done in a block
-static_assert(kSubNodes * 256 == kSubMax, "radix histograms alias the eqoff array");
-static_assert(kSubMax == 4 * kSubTPB, "four contiguous points per thread in the partition scan");
+constexpr int kSubMaxAll = 2048;
+template <int DP>
+constexpr int sub_max() { return DP <= 16 ? 2048 : 1024; }   // 128 KiB of points
+static_assert(kSubNodes * 256 == kSubMaxAll, "radix histograms alias the eqoff array");
+
+int subtree_max(int dpad) { return dpad <= 16 ? 2048 : 1024; }
 
 __device__ __forceinline__ int sub_of(const int *sb, int nsub, int i) {
     int a = 0, b = nsub - 1;
@@ -542,26 +563,35 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
                                                      int splitting_type, float *__restrict__ pts_out,
                                                      int *__restrict__ orig_out, int *__restrict__ axis_out,
                                                      float *__restrict__ split_out, int *err) {
-    __shared__ int perm[2][kSubMax];
-    __shared__ uint32_t keys[kSubMax];
-    __shared__ int hist_eq[kSubMax];                 // radix histograms, then eqoff
+    constexpr int SM = sub_max<DP>();
+    constexpr int EPT = (SM + kSubTPB - 1) / kSubTPB;   // points per thread in the partition scan
+    __shared__ float4 data[SM * DP / 4];                // the node's points, level-L0 order
+    __shared__ uint16_t perm[2][SM];
+    __shared__ uint32_t keys[SM];
+    __shared__ int hist_eq[kSubMaxAll];                 // radix histograms, then eqoff
     __shared__ uint32_t sbox[kSubNodes][DP][2];
-    __shared__ int sb[kSubNodes + 1];                // sub-node bounds (local offsets)
+    __shared__ int sb[kSubNodes + 1];                   // sub-node bounds (local offsets)
     __shared__ int sax[kSubNodes];
     __shared__ uint32_t spre[kSubNodes];
     __shared__ int srank[kSubNodes];
     __shared__ int PL[kSubNodes + 1], PE[kSubNodes + 1];
     __shared__ int wlt[kSubTPB / kWave], weq[kSubTPB / kWave];
+    const float *dataf = reinterpret_cast<const float *>(data);
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int node = blockIdx.x;
     const int lo = bnd0[node], c0 = bnd0[node + 1] - lo;
-    for (int i = tid; i < c0; i += kSubTPB) perm[0][i] = i;
+    {
+        const float4 *src = reinterpret_cast<const float4 *>(pts_in + (size_t)lo * DP);
+        for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) data[e] = src[e];
+    }
+    for (int i = tid; i < c0; i += kSubTPB) perm[0][i] = (uint16_t)i;
     if (tid == 0) {
         sb[0] = 0;
         sb[1] = c0;
     }
     int cur = 0;
+    __syncthreads();
     for (int r = 0; r < R; ++r) {
         const int nsub = 1 << r;
         const int lvl = L0 + r;
@@ -571,45 +601,31 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
             sbox[tid / DP][tid % DP][1] = 0u;
         }
         __syncthreads();
-        // (A) data boxes: chunks of 64 points of one sub-node per wavefront
+        // (A) data boxes: every (sub-node, dimension) pair gets T threads
+        // (a power of two, T | 64) that scan its points; no atomics
         if (splitting_type != 0) {
-            int nch = 0;
-            for (int s2 = 0; s2 < nsub; ++s2) nch += (sb[s2 + 1] - sb[s2] + kWave - 1) / kWave;
-            for (int q = wid; q < nch; q += kSubTPB / kWave) {
-                int s2 = 0, base = 0, acc = 0;
-                for (; s2 < nsub; ++s2) {
-                    const int ch = (sb[s2 + 1] - sb[s2] + kWave - 1) / kWave;
-                    if (q < acc + ch) {
-                        base = sb[s2] + (q - acc) * kWave;
-                        break;
-                    }
-                    acc += ch;
+            const int P = nsub * d;
+            int T = 64;
+            while (T > 1 && T * P > kSubTPB) T >>= 1;
+            const int pair = tid / T, lip = tid % T;
+            uint32_t mn = 0xFFFFFFFFu, mx = 0u;
+            int s2 = 0, j = 0;
+            if (pair < P) {
+                s2 = pair / d;
+                j = pair - s2 * d;
+                for (int i = sb[s2] + lip; i < sb[s2 + 1]; i += T) {
+                    const uint32_t k = f2key(dataf[(size_t)perm[cur][i] * DP + j]);
+                    mn = min(mn, k);
+                    mx = max(mx, k);
                 }
-                const int i = base + lane;
-                const bool valid = i < sb[s2 + 1];
-                float v[DP];
-                if (valid) {
-                    load_vec<DP>(pts_in + (size_t)(lo + perm[cur][i]) * DP, v);
-                } else {
-#pragma unroll
-                    for (int j = 0; j < DP; ++j) v[j] = 0.0f;
-                }
-#pragma unroll
-                for (int j = 0; j < DP; ++j) {
-                    if (j >= d) break;
-                    const uint32_t k = f2key(v[j]);
-                    uint32_t mn = valid ? k : 0xFFFFFFFFu;
-                    uint32_t mx = valid ? k : 0u;
-#pragma unroll
-                    for (int m = kWave / 2; m >= 1; m >>= 1) {
-                        mn = min(mn, (uint32_t)__shfl_xor((int)mn, m, kWave));
-                        mx = max(mx, (uint32_t)__shfl_xor((int)mx, m, kWave));
-                    }
-                    if (lane == 0) {
-                        atomicMin(&sbox[s2][j][0], mn);
-                        atomicMax(&sbox[s2][j][1], mx);
-                    }
-                }
+            }
+            for (int m = T >> 1; m >= 1; m >>= 1) {
+                mn = min(mn, (uint32_t)__shfl_xor((int)mn, m, kWave));
+                mx = max(mx, (uint32_t)__shfl_xor((int)mx, m, kWave));
+            }
+            if (pair < P && lip == 0) {
+                sbox[s2][j][0] = mn;
+                sbox[s2][j][1] = mx;
             }
         }
         __syncthreads();
@@ -632,10 +648,8 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         }
         __syncthreads();
         // (C) split-axis keys
-        for (int i = tid; i < c0; i += kSubTPB) {
-            const int s2 = sub_of(sb, nsub, i);
-            keys[i] = f2key(pts_in[(size_t)(lo + perm[cur][i]) * DP + sax[s2]]);
-        }
+        for (int i = tid; i < c0; i += kSubTPB)
+            keys[i] = f2key(dataf[(size_t)perm[cur][i] * DP + sax[sub_of(sb, nsub, i)]]);
         // (D) radix select of the rank-c/2 key of every sub-node
         for (int pass = 0; pass < 4; ++pass) {
             for (int i = tid; i < nsub * 256; i += kSubTPB) hist_eq[i] = 0;
@@ -681,14 +695,13 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
             }
             __syncthreads();
         }
-        // (E) '<' and '==' counts in segment order: four contiguous points
-        // per thread, block-wide exclusive scan over the concatenated
-        // sub-node segments
-        int fl[4], fe[4];
+        // (E) '<' and '==' counts in segment order: EPT contiguous points per
+        // thread, block-wide exclusive scan over the concatenated segments
+        int fl[EPT], fe[EPT];
         int tl = 0, te = 0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = 4 * tid + u;
+        for (int u = 0; u < EPT; ++u) {
+            const int i = EPT * tid + u;
             fl[u] = fe[u] = 0;
             if (i < c0) {
                 const uint32_t v = spre[sub_of(sb, nsub, i)];
@@ -708,13 +721,13 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         __syncthreads();
         int ol = 0, oe = 0;
         for (int w = 0; w < wid; ++w) { ol += wlt[w]; oe += weq[w]; }
-        int xl = il - tl + ol, xe = ie - te + oe;   // exclusive prefix of point 4*tid
-        int pl[4], pe[4];
+        int xl = il - tl + ol, xe = ie - te + oe;   // exclusive prefix of point EPT*tid
+        int pl[EPT], pe[EPT];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < EPT; ++u) {
             pl[u] = xl;
             pe[u] = xe;
-            const int i = 4 * tid + u;
+            const int i = EPT * tid + u;
             for (int s2 = 0; s2 <= nsub; ++s2)
                 if (sb[s2] == i && i < c0) { PL[s2] = xl; PE[s2] = xe; }
             xl += fl[u];
@@ -724,8 +737,8 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         __syncthreads();
         // (F) eqoff[t] = G-offset of the t-th equal point in G order
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = 4 * tid + u;
+        for (int u = 0; u < EPT; ++u) {
+            const int i = EPT * tid + u;
             if (i >= c0 || !fe[u]) continue;
             const int s2 = sub_of(sb, nsub, i);
             const int c = sb[s2 + 1] - sb[s2], off = i - sb[s2];
@@ -739,8 +752,8 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         __syncthreads();
         // (G) slots
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int i = 4 * tid + u;
+        for (int u = 0; u < EPT; ++u) {
+            const int i = EPT * tid + u;
             if (i >= c0) continue;
             const int s2 = sub_of(sb, nsub, i);
             const int c = sb[s2 + 1] - sb[s2], off = i - sb[s2];
@@ -761,8 +774,8 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
             }
             if (slot < 0 || slot >= c) { atomicOr(err, 1); continue; }
             const int pidx = perm[cur][i];
-            perm[cur ^ 1][sb[s2] + slot] = pidx;
-            if (slot == c / 2) split_out[heap0 + s2] = pts_in[(size_t)(lo + pidx) * DP + sax[s2]];
+            perm[cur ^ 1][sb[s2] + slot] = (uint16_t)pidx;
+            if (slot == c / 2) split_out[heap0 + s2] = dataf[(size_t)pidx * DP + sax[s2]];
         }
         __syncthreads();
         // children bounds for the next level: the left child gets c/2 points
@@ -783,14 +796,12 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         cur ^= 1;
         __syncthreads();
     }
-    for (int i = tid; i < c0; i += kSubTPB) {
-        const int pidx = perm[cur][i];
-        const float4 *src = reinterpret_cast<const float4 *>(pts_in + (size_t)(lo + pidx) * DP);
-        float4 *dst = reinterpret_cast<float4 *>(pts_out + (size_t)(lo + i) * DP);
-#pragma unroll
-        for (int q = 0; q < DP / 4; ++q) dst[q] = src[q];
-        orig_out[lo + i] = orig_in[lo + pidx];
+    float4 *dst = reinterpret_cast<float4 *>(pts_out + (size_t)lo * DP);
+    for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) {
+        const int i = e / (DP / 4), c = e - i * (DP / 4);
+        dst[e] = data[(size_t)perm[cur][i] * (DP / 4) + c];
     }
+    for (int i = tid; i < c0; i += kSubTPB) orig_out[lo + i] = orig_in[lo + perm[cur][i]];
 }
 
 // Leaf bounds and point -> leaf map (kdtree.c:86-100).
@@ -819,7 +830,7 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
     // above the leaves
     int L0 = depth;
     for (int lvl = std::max(0, depth - kSubLevels); lvl < depth; ++lvl)
-        if (host_maxc[lvl] <= kSubMax) { L0 = lvl; break; }
+        if (host_maxc[lvl] <= subtree_max(dpad)) { L0 = lvl; break; }
     for (int lvl = 0; lvl < L0; ++lvl) {
         LevelArgs L;
         L.nodes = 1 << lvl;
@@ -907,7 +918,7 @@ __global__ __launch_bounds__(256) void k_natural(const float *__restrict__ q, in
         const int from = leaves[2 * l];
         desc[i] = (from << 1) | (leaves[2 * l + 1] - from - m);
     }
-    if (count) (void)wave_add1(count, l);
+    if (count) wave_count1(count, l);
 }
 
 hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, int *desc,

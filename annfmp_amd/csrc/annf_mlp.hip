@@ -22,15 +22,20 @@ constexpr float kG1 = 0.07135481627f;        // 2 sqrt(2 / pi) * 0.044715
 inline unsigned nblk(long long n, int tpb) { return (unsigned)((n + tpb - 1) / tpb); }
 
 // GELU, tanh form: 0.5 x (1 + tanh(u)), u = sqrt(2/pi) (x + 0.044715 x^3),
-// written as x * sigmoid(2u).
+// written as x * sigmoid(2u) with the hardware exp2 and reciprocal (about
+// 1 ulp each; the network rounds activations to BF16 anyway).
+__device__ __forceinline__ float sigmoid_f(float t) {
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(-t * 1.4426950408889634f));
+}
+
 __device__ __forceinline__ float gelu_f(float x) {
     const float t = x * (kG0 + kG1 * (x * x));
-    return x / (1.0f + __expf(-t));
+    return x * sigmoid_f(t);
 }
 
 __device__ __forceinline__ float gelu_grad(float x) {
     const float t = x * (kG0 + kG1 * (x * x));
-    const float s = 1.0f / (1.0f + __expf(-t));
+    const float s = sigmoid_f(t);
     return s + x * (s * (1.0f - s)) * (kG0 + 3.0f * kG1 * (x * x));
 }
 

@@ -98,7 +98,22 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nprobe", type=int, default=1, help="leaves probed per query on the BF16 path")
     ap.add_argument("--encoder", default=DEFAULT_ENCODER, help="trained encoder (.npz); 'none' forces PCA")
+    ap.add_argument("--paths", default="plain,pca,learned",
+                    help="paths to run (profiling runs select one); the value needs pca and learned")
+    ap.add_argument("--no-quality", action="store_true", help="skip the exact-field quality evaluation")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="only run 5 device-to-device copies of MB MiB (scales the HBM byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.empty(args.copy_calibration << 20, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": args.copy_calibration << 20, "copies": 5}))
+        return
+    run_paths = set(args.paths.split(","))
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the field would shard by image columns; see the design notes)", file=sys.stderr)
@@ -146,34 +161,45 @@ def main():
             h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, proj, depth, K,
             args.nprobe, _lib.PREC_BF16, K, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
 
-    q = metrics.sample_queries(n_a, 8192)
-    exact = metrics.exact_knn(a, b, p, q, 5, handle=h)
+    if not args.no_quality:
+        q = metrics.sample_queries(n_a, 8192)
+        exact = metrics.exact_knn(a, b, p, q, 5, handle=h)
 
     def quality():
         torch.cuda.synchronize()
+        if args.no_quality:
+            return {"recall@5": float("nan"), "distance_ratio": float("nan")}
         return metrics.field_quality(a, b, p, nn.cpu().numpy(), knn.cpu().numpy(), k=5, handle=h, exact=exact)
 
     paths = {}
-    # plain FP32 reference method
-    ms, stage = timed(plain, args.steps, args.warmup, torch, _lib, h, 7)
-    plain(knn, red=True)
-    plain_nn = nn.cpu().numpy()
-    paths["plain_fp32"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
-                           "stage_ms": {s: round(float(v), 4) for s, v in zip(PLAIN_STAGES, stage)}, **quality()}
-    # PCA + BF16 full-patch leaf k-NN
-    ms, stage = timed(lambda: mm(_lib.PROJ_PCA32), args.steps, args.warmup, torch, _lib, h, 5)
-    mm(_lib.PROJ_PCA32, knn)
-    paths["pca_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
-                         "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)}, **quality()}
-    chosen = "pca_bf16"
-    if encoder is not None:
+    plain_nn = None
+    if "plain" in run_paths:
+        # plain FP32 reference method
+        ms, stage = timed(plain, args.steps, args.warmup, torch, _lib, h, 7)
+        plain(knn, red=True)
+        plain_nn = nn.cpu().numpy()
+        paths["plain_fp32"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                               "stage_ms": {s: round(float(v), 4) for s, v in zip(PLAIN_STAGES, stage)},
+                               **quality()}
+    if "pca" in run_paths:
+        # PCA + BF16 full-patch leaf k-NN
+        ms, stage = timed(lambda: mm(_lib.PROJ_PCA32), args.steps, args.warmup, torch, _lib, h, 5)
+        mm(_lib.PROJ_PCA32, knn)
+        paths["pca_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                             "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)}, **quality()}
+    chosen = "pca_bf16" if "pca_bf16" in paths else None
+    if encoder is not None and "learned" in run_paths:
         ms, stage = timed(lambda: mm(_lib.PROJ_ENCODER), args.steps, args.warmup, torch, _lib, h, 5)
         mm(_lib.PROJ_ENCODER, knn)
         paths["learned_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
                                  "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)},
                                  **quality()}
-        if paths["learned_bf16"]["recall@5"] >= paths["pca_bf16"]["recall@5"]:
+        if chosen is None or args.no_quality or \
+                paths["learned_bf16"]["recall@5"] >= paths["pca_bf16"]["recall@5"]:
             chosen = "learned_bf16"
+    if chosen is None:
+        print(json.dumps({"paths": paths}), flush=True)
+        return
     main_path = paths[chosen]
     ms_per_step = main_path["ms_per_step"]
     value = n_a / ms_per_step * 1e3
@@ -190,10 +216,11 @@ def main():
 
     cpu = None
     parity = None
-    if not args.no_cpu:
+    if not args.no_cpu and plain_nn is not None:
         cpu, parity = cpu_baseline(a, b, p, ra.cpu().numpy(), rb.cpu().numpy(), depth, K, plain_nn)
 
-    plain_rec = paths["plain_fp32"]["recall@5"]
+    plain_res = paths.get("plain_fp32", {"recall@5": float("nan"), "query_patches_per_s": float("nan")})
+    plain_rec = plain_res["recall@5"]
     out = {
         "metric": "query patches/sec, 1080p pair: learned+BF16 vs plain FP32 at equal recall@5",
         "value": round(value, 1), "unit": "query patches/s", "n_gpus": 1, "steps": args.steps,
@@ -206,7 +233,7 @@ def main():
                                   else "PCA-16", args.nprobe),
                    "queries_per_step": n_a, "seed": args.seed, "path": chosen,
                    "encoder": os.path.relpath(args.encoder, ROOT) if encoder is not None else None},
-        "speedup_vs_plain_fp32": round(value / paths["plain_fp32"]["query_patches_per_s"], 3),
+        "speedup_vs_plain_fp32": round(value / plain_res["query_patches_per_s"], 3),
         "recall@5_vs_plain_fp32": [round(main_path["recall@5"], 4), round(plain_rec, 4)],
         "recall_at_least_plain": bool(main_path["recall@5"] >= plain_rec),
         "paths": paths, "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity,
