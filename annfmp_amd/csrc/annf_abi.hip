@@ -82,7 +82,7 @@ struct annf_ctx {
     // tree storage
     DevBuf pts_a, pts_b, orig_a, orig_b, axis, split, leaves, pli;
     // build scratch
-    DevBuf keys, eqoff, box, nax, prefix, rank, hist, cnt, tot, bnd, errflag;
+    DevBuf keys, eqoff, box, box2, nax, prefix, rank, hist, cnt, tot, bnd, errflag;
     // query workspaces
     DevBuf qbuf, natleaf, natdesc, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
@@ -121,7 +121,7 @@ struct annf_ctx {
     std::vector<int> h_bnd, h_off, h_cpn, h_maxc;
 
     void release_all() {
-        DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box,
+        DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box, &box2,
                           &nax, &prefix, &rank, &hist, &cnt, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
@@ -204,7 +204,8 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     RCHK(c->pli.ensure((size_t)n * sizeof(int)));
     RCHK(c->keys.ensure((size_t)n * sizeof(uint32_t)));
     RCHK(c->eqoff.ensure((size_t)n * sizeof(int)));
-    RCHK(c->box.ensure(nodes_max * dpad * 2 * sizeof(uint32_t)));
+    RCHK(c->box.ensure(2 * nodes_max * dpad * 2 * sizeof(uint32_t)));
+    RCHK(c->box2.ensure(2 * nodes_max * dpad * 2 * sizeof(uint32_t)));
     RCHK(c->nax.ensure(nodes_max * sizeof(int)));
     RCHK(c->prefix.ensure(nodes_max * sizeof(uint32_t)));
     RCHK(c->rank.ensure(nodes_max * sizeof(int)));
@@ -224,6 +225,7 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     s.keys = c->keys.as<uint32_t>();
     s.eqoff = c->eqoff.as<int>();
     s.box = c->box.as<uint32_t>();
+    s.box2 = c->box2.as<uint32_t>();
     s.nax = c->nax.as<int>();
     s.prefix = c->prefix.as<uint32_t>();
     s.rank = c->rank.as<int>();

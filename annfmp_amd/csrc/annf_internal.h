@@ -30,7 +30,7 @@ struct BuildScratch {
     int *orig_a, *orig_b;             // n each
     uint32_t *keys;                   // n
     int *eqoff;                       // n
-    uint32_t *box;                    // nodes_max * dpad * 2
+    uint32_t *box, *box2;             // 2 * nodes_max * dpad * 2 each (this / next level)
     int *nax;                         // nodes_max
     uint32_t *prefix;                 // nodes_max
     int *rank;                        // nodes_max

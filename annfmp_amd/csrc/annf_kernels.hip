@@ -249,6 +249,30 @@ __device__ __forceinline__ int2 block_scan_int2(int2 v, int2 *wsum /* __shared__
     return make_int2(inc.x - v.x + off.x, inc.y - v.y + off.y);
 }
 
+// Block-wide exclusive scan of an int2 over 256 threads, also returning the
+// block total.
+__device__ __forceinline__ int2 block_scan_int2_tot(int2 v, int2 *wsum /* __shared__ [4] */, int2 &total) {
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int2 inc = v;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        int tx = __shfl_up(inc.x, o, kWave);
+        int ty = __shfl_up(inc.y, o, kWave);
+        if (lane >= o) { inc.x += tx; inc.y += ty; }
+    }
+    if (lane == kWave - 1) wsum[wid] = inc;
+    __syncthreads();
+    int2 off = make_int2(0, 0), tot = make_int2(0, 0);
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+        if (w < wid) { off.x += wsum[w].x; off.y += wsum[w].y; }
+        tot.x += wsum[w].x;
+        tot.y += wsum[w].y;
+    }
+    __syncthreads();
+    total = tot;
+    return make_int2(inc.x - v.x + off.x, inc.y - v.y + off.y);
+}
+
 // ---------------------------------------------------------------------------
 // tree build (level-synchronous)
 // ---------------------------------------------------------------------------
@@ -274,9 +298,13 @@ __device__ __forceinline__ void chunk_range(const LevelArgs &L, int &node, int &
     e = min(s + kChunk, hi);
 }
 
-__global__ void k_level_init(LevelArgs L, int dpad, uint32_t *box, uint32_t *prefix, int *rank, uint32_t *hist) {
+// Resets the level's per-node state; box (this level's nodes) and nbox (the
+// next level's, filled by k_move) when given.
+__global__ void k_level_init(LevelArgs L, int dpad, uint32_t *box, uint32_t *nbox, uint32_t *prefix, int *rank,
+                             uint32_t *hist) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < L.nodes * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
+    if (box && i < L.nodes * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
+    if (nbox && i < 2 * L.nodes * dpad) { nbox[2 * i] = 0xFFFFFFFFu; nbox[2 * i + 1] = 0u; }
     if (i < L.nodes) { prefix[i] = 0u; rank[i] = (L.bnd[i + 1] - L.bnd[i]) / 2; }
     if (i < L.nodes * 256) hist[i] = 0u;
 }
@@ -524,6 +552,129 @@ __global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__
             orig_out[dst] = orig_in[i];
             if (slot == c / 2) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
         }
+    }
+}
+
+// The moving pass of the partition (the slots of k_scatter), coalesced: the
+// block first computes every point's slot (points s + u*256 + tid, so the
+// segment-order ranks come from eight block scans), then copies the records
+// one float4 per lane.  With nbox, the same copy accumulates the two
+// children's data boxes (the next level's k_box), for DP/4 a power of two.
+template <int DP>
+__global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
+                                                    LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
+                                                    const int2 *cnt_off, const int2 *tot, const int *eqoff,
+                                                    float *__restrict__ pts_out, int *__restrict__ orig_out,
+                                                    const int *nax, float *split, int *err, uint32_t *nbox, int d) {
+    constexpr int P4 = DP / 4;
+    constexpr bool kBoxOk = (P4 & (P4 - 1)) == 0;
+    __shared__ int2 wsum[kBuildTPB / kWave];
+    __shared__ int slots[kChunk];
+    __shared__ uint32_t sbox[2][DP][2];
+    int node, lo, hi, s, e;
+    chunk_range(L, node, lo, hi, s, e);
+    if (s >= e) return;
+    const int c = hi - lo, half = c / 2;
+    const int Lt = tot[node].x, E = tot[node].y, g = c - Lt;
+    const uint32_t vk = vkey[node];
+    const int ax = nax[node];
+    const bool boxes = kBoxOk && nbox != nullptr;
+    if (boxes && threadIdx.x < 2 * DP) {
+        sbox[threadIdx.x / DP][threadIdx.x % DP][0] = 0xFFFFFFFFu;
+        sbox[threadIdx.x / DP][threadIdx.x % DP][1] = 0u;
+    }
+    int2 run = cnt_off[blockIdx.x];
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) {
+        const int i = s + u * kBuildTPB + threadIdx.x;
+        const bool valid = i < e;
+        const uint32_t k = valid ? keys[i] : 0u;
+        const int lt = valid && k < vk, eq = valid && k == vk;
+        int2 sum;
+        const int2 ex = block_scan_int2_tot(make_int2(lt, eq), wsum, sum);
+        const int r_lt = run.x + ex.x, r_eq = run.y + ex.y;
+        run.x += sum.x;
+        run.y += sum.y;
+        int slot = -1;
+        if (valid) {
+            const int off = i - lo;
+            if (lt) {
+                slot = r_lt;
+            } else if (eq) {
+                slot = Lt + (E - 1) - r_eq;
+            } else {
+                int p = (g - 1) - (off - r_lt);
+                // the chain strictly increases, so it ends within E steps
+                for (int it = 0; p < E && it <= E; ++it) p = eqoff[lo + p];
+                slot = Lt + p;
+            }
+            if (slot < 0 || slot >= c) {   // cannot happen for a correct partition
+                atomicOr(err, 1);
+                slot = -1;
+            } else {
+                orig_out[lo + slot] = orig_in[i];
+                if (slot == half) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
+            }
+        }
+        slots[u * kBuildTPB + threadIdx.x] = slot;
+    }
+    __syncthreads();
+    const float4 *src4 = reinterpret_cast<const float4 *>(pts_in) + (size_t)s * P4;
+    float4 *dst4 = reinterpret_cast<float4 *>(pts_out) + (size_t)lo * P4;
+    uint32_t mn0[4], mx0[4], mn1[4], mx1[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { mn0[q] = mn1[q] = 0xFFFFFFFFu; mx0[q] = mx1[q] = 0u; }
+    const int nf = (e - s) * P4;
+    for (int f = threadIdx.x; f < nf; f += kBuildTPB) {
+        const int el = f / P4, part = f - el * P4;
+        const int slot = slots[el];
+        if (slot < 0) continue;
+        const float4 v = src4[f];
+        dst4[(size_t)slot * P4 + part] = v;
+        if (boxes) {
+            const uint32_t k0 = f2key(v.x), k1 = f2key(v.y), k2 = f2key(v.z), k3 = f2key(v.w);
+            if (slot < half) {
+                mn0[0] = min(mn0[0], k0); mx0[0] = max(mx0[0], k0);
+                mn0[1] = min(mn0[1], k1); mx0[1] = max(mx0[1], k1);
+                mn0[2] = min(mn0[2], k2); mx0[2] = max(mx0[2], k2);
+                mn0[3] = min(mn0[3], k3); mx0[3] = max(mx0[3], k3);
+            } else {
+                mn1[0] = min(mn1[0], k0); mx1[0] = max(mx1[0], k0);
+                mn1[1] = min(mn1[1], k1); mx1[1] = max(mx1[1], k1);
+                mn1[2] = min(mn1[2], k2); mx1[2] = max(mx1[2], k2);
+                mn1[3] = min(mn1[3], k3); mx1[3] = max(mx1[3], k3);
+            }
+        }
+    }
+    if (!boxes) return;
+    // lanes with the same part (lane % P4) hold the same four dimensions
+    const int lane = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int m = P4; m < kWave; m <<= 1) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            mn0[q] = min(mn0[q], (uint32_t)__shfl_xor((int)mn0[q], m, kWave));
+            mx0[q] = max(mx0[q], (uint32_t)__shfl_xor((int)mx0[q], m, kWave));
+            mn1[q] = min(mn1[q], (uint32_t)__shfl_xor((int)mn1[q], m, kWave));
+            mx1[q] = max(mx1[q], (uint32_t)__shfl_xor((int)mx1[q], m, kWave));
+        }
+    }
+    if (lane < P4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = 4 * lane + q;
+            atomicMin(&sbox[0][j][0], mn0[q]);
+            atomicMax(&sbox[0][j][1], mx0[q]);
+            atomicMin(&sbox[1][j][0], mn1[q]);
+            atomicMax(&sbox[1][j][1], mx1[q]);
+        }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 2 * d) {
+        const int ch = threadIdx.x / d, j = threadIdx.x - ch * d;
+        const size_t o = ((size_t)(2 * node + ch) * DP + j) * 2;
+        atomicMin(&nbox[o], sbox[ch][j][0]);
+        atomicMax(&nbox[o + 1], sbox[ch][j][1]);
     }
 }
 
@@ -831,6 +982,9 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
     int L0 = depth;
     for (int lvl = std::max(0, depth - kSubLevels); lvl < depth; ++lvl)
         if (host_maxc[lvl] <= subtree_max(dpad)) { L0 = lvl; break; }
+    // the moving pass computes the next level's boxes when DP / 4 is a power
+    // of two; boxes alternate between box and box2
+    const bool fused_box = splitting_type != 0 && (dpad == 4 || dpad == 8 || dpad == 16 || dpad == 32);
     for (int lvl = 0; lvl < L0; ++lvl) {
         LevelArgs L;
         L.nodes = 1 << lvl;
@@ -839,13 +993,17 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
         L.node0 = L.nodes - 1;
         L.lvl = lvl;
         const unsigned nblk = (unsigned)(L.nodes * L.cpn);
-        hipLaunchKernelGGL(k_level_init, dim3(blocks_for((long long)L.nodes * 256, 256)), dim3(256), 0, st, L, dpad,
-                           s.box, s.prefix, s.rank, s.hist);
+        uint32_t *box = (lvl & 1) ? s.box2 : s.box;
+        uint32_t *nbox = (lvl & 1) ? s.box : s.box2;
+        const bool need_kbox = splitting_type != 0 && (!fused_box || lvl == 0);
+        if (!(fused_box && lvl + 1 < L0)) nbox = nullptr;
+        hipLaunchKernelGGL(k_level_init, dim3(blocks_for((long long)L.nodes * 256 * 2, 256)), dim3(256), 0, st, L,
+                           dpad, need_kbox ? box : (uint32_t *)nullptr, nbox, s.prefix, s.rank, s.hist);
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
-            if (splitting_type != 0)
-                hipLaunchKernelGGL(k_box<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, d, s.box);
-            hipLaunchKernelGGL(k_axis, dim3(blocks_for(L.nodes, 256)), dim3(256), 0, st, L, d, dpad, s.box,
+            if (need_kbox)
+                hipLaunchKernelGGL(k_box<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, d, box);
+            hipLaunchKernelGGL(k_axis, dim3(blocks_for(L.nodes, 256)), dim3(256), 0, st, L, d, dpad, box,
                                splitting_type, s.nax, axis);
             for (int pass = 0; pass < 4; ++pass) {
                 hipLaunchKernelGGL(k_hist<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, s.nax, s.prefix, pass,
@@ -857,8 +1015,8 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
             hipLaunchKernelGGL(k_scan_nodes, dim3(L.nodes), dim3(kBuildTPB), 0, st, L, s.cnt, s.tot);
             hipLaunchKernelGGL((k_scatter<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
                                s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
-            hipLaunchKernelGGL((k_scatter<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
-                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
+            hipLaunchKernelGGL(k_move<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, s.prefix, s.cnt,
+                               s.tot, s.eqoff, pout, oout, s.nax, split, s.err, nbox, d);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
