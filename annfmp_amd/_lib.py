@@ -79,8 +79,8 @@ def _declare(L):
     L.annf_stage_times.argtypes = [_vp, _fp, i]
     L.annf_set_timing.argtypes = [_vp, i]
     L.annf_exact_field_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, i, i, _vp, _vp]
-    L.annf_field_mm_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, _vp, i, i, i, i, i, i, i, _vp, _vp, _vp,
-                                    _vp, _vp]
+    L.annf_field_mm_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, _vp, i, i, i, i, i, i, i, i, _vp, _vp,
+                                    _vp, _vp, _vp]
     f = C.c_float
     L.annf_mlp_forward_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp]
     L.annf_mlp_triplet_grad_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, f, _vp, _vp]

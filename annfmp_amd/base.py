@@ -199,13 +199,14 @@ def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_nei
 
 
 def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_neighbors, nprobe=1,
-                       precision="bf16", shortlist=20, projection="fp32", device=0, handle=None,
+                       precision="bf16", shortlist=20, projection="fp32", propagate=False, device=0, handle=None,
                        return_reduced=False, timing=True):
     """One call of annf_field_mm_dev (full-patch leaf k-NN on MFMA) on host
     images; returns numpy results like field_on_device.
 
     precision: "bf16" (exact distances) or "fp8" (shortlist re-ranked
-    exactly); nprobe: leaves per query (a power of two)."""
+    exactly); nprobe: leaves per query (a power of two); propagate: False,
+    True (all K entries of the list above propagate) or a count."""
     torch = dev.torch()
     h = handle or _lib.Handle(device)
     L = _lib.lib()
@@ -229,13 +230,14 @@ def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_
     t0 = time.time()
     _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
         h.h, dev.dptr(ia), ha, wa, dev.dptr(ib), hb, wb, int(psize), dev.dptr(comp), dev.dptr(mu), int(d),
-        proj, int(tree_depth), K, int(nprobe), prec, int(shortlist), dev.dptr(nn),
+        proj, int(tree_depth), K, int(nprobe), prec, int(shortlist),
+        (K if propagate is True else int(propagate or 0)), dev.dptr(nn),
         dev.dptr(dist), dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
     wall = time.time() - t0
     st = numpy.zeros(7, numpy.float32)
     L.annf_stage_times(h.h, _lib.ptr(st, _lib._fp), 7)
     out = {"nn": nn.cpu().numpy(), "dist": dist.cpu().numpy(), "knn": knn.cpu().numpy(),
-           "stages_ms": st[:5].tolist(), "wall_s": wall}
+           "stages_ms": st[:6].tolist(), "wall_s": wall}
     if return_reduced:
         out["red_a"] = ra.cpu().numpy()
         out["red_b"] = rb.cpu().numpy()

@@ -3,6 +3,7 @@
 #include "annf_internal.h"
 #include "../../include/annf_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -90,6 +91,8 @@ struct annf_ctx {
     DevBuf ev_am, ev_an, ev_bm, ev_bn, ev_part;
     // learned encoder: training workspace and the packed inference weights
     DevBuf mlp_ws, enc_w1, enc_b1, enc_w2, enc_b2, enc_w3, enc_b3;
+    // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
+    DevBuf mm_keys1, mm_keys2, mm_pairleaf;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // timing
@@ -126,7 +129,7 @@ struct annf_ctx {
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
-                          &enc_b2, &enc_w3, &enc_b3};
+                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf};
         for (DevBuf *b : list) b->release();
         has_tree = false;
         has_encoder = false;
@@ -675,8 +678,8 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
 
 int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
                       int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
-                      int n_neighbors, int nprobe, int precision, int shortlist, int *nn_out, float *dist_out,
-                      int *knn_out, float *red_a_out, float *red_b_out) {
+                      int n_neighbors, int nprobe, int precision, int shortlist, int propagate, int *nn_out,
+                      float *dist_out, int *knn_out, float *red_a_out, float *red_b_out) {
     if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
     if (!leafmm_supported(psize)) return fail(ANNF_EUNSUPPORTED, "patch size %d: padded patch above 512 bytes", psize);
     if (precision != ANNF_PREC_BF16 && precision != ANNF_PREC_FP8)
@@ -686,19 +689,28 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         return fail(ANNF_EUNSUPPORTED, "shortlist %d outside [n_neighbors, 32]", shortlist);
     if (tree_depth < 0 || tree_depth > 30 || nprobe < 1 || nprobe > (1 << tree_depth) || (nprobe & (nprobe - 1)))
         return fail(ANNF_EINVAL, "nprobe %d is not a power of two in [1, 2^tree_depth]", nprobe);
+    if (propagate > 0 && wa != wb)
+        return fail(ANNF_EINVAL, "propagation needs images of equal width (base.c:271)");
     FieldShape fs;
     RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, projection, tree_depth, K,
                            red_a_out, red_b_out, fs));
     const TreeView &t = c->tree;
     const int nleaves = 1 << t.depth;
     const int n_a = (int)fs.n_a;
+    if (propagate < 0) return fail(ANNF_EINVAL, "propagate %d is negative", propagate);
+    const bool prop = propagate > 0 && fs.nr_a > 1;
+    // the first prop_k entries of the list above propagate (K: all, base.c:255-282)
+    const int prop_k = std::min(propagate, K);
+    const size_t n_pairs = prop ? (size_t)n_a * prop_k : 0;
+    const int fp8 = precision == ANNF_PREC_FP8;
     c->mark(3);
     RCHK(c->natleaf.ensure((size_t)n_a * sizeof(int)));
     RCHK(c->bcount.ensure((size_t)nleaves * sizeof(int)));
     RCHK(c->boffset.ensure(((size_t)nleaves + 1) * sizeof(int)));
     RCHK(c->bcursor.ensure((size_t)nleaves * sizeof(int)));
     RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
-    RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
+    RCHK(c->order.ensure(std::max((size_t)n_a, n_pairs) * sizeof(int)));
+    RCHK(c->mm_keys1.ensure((size_t)n_a * K * sizeof(uint64_t)));
     HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
     HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), nullptr,
                           c->stream));
@@ -706,8 +718,27 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                           c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), leafmm_chunk(), c->stream));
     c->mark(4);
     HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), n_a, nprobe,
-                   shortlist, K, precision == ANNF_PREC_FP8, knn_out, nn_out, dist_out, c->stream));
+                   shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), c->stream));
     c->mark(5);
+    uint64_t *keys2 = nullptr;
+    if (prop) {
+        // the propagated leaves, bucketed like the queries, scanned by the same kernel
+        RCHK(c->mm_pairleaf.ensure(n_pairs * sizeof(int)));
+        RCHK(c->mm_keys2.ensure(n_pairs * K * sizeof(uint64_t)));
+        keys2 = c->mm_keys2.as<uint64_t>();
+        HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
+        HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, c->natleaf.as<int>(), t.pli, fs.nc_a,
+                          (int)fs.n_b, c->mm_pairleaf.as<int>(), c->bcount.as<int>(), c->stream));
+        HIPCHK(bucket_queries(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, c->bcount.as<int>(),
+                              c->boffset.as<int>(), c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(),
+                              leafmm_chunk(), c->stream));
+        HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
+        HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), (int)n_pairs,
+                       1, shortlist, K, fp8, prop_k, keys2, c->stream));
+    }
+    HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,
+                        c->stream));
+    c->mark(6);
     RCHK(sync_stream(c));
     RCHK(check_build_flag(c));
     c->collect_marks();

@@ -9,6 +9,29 @@ namespace annf {
 constexpr int kWave = 64;
 constexpr float kFltMax = 3.402823466e+38f;  // list sentinel, float.h:24
 
+// rank of this lane among the lanes set in m
+__device__ __forceinline__ int lane_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// ctr[key] += 1 per active lane: lanes sharing the wave's first two keys add
+// once per key, the rest on their own; no return value, so nothing waits.
+__device__ __forceinline__ void wave_count1(int *ctr, int key) {
+    bool done = false;
+    for (int round = 0; round < 2; ++round) {
+        if (!done) {
+            const int lead = __builtin_amdgcn_readfirstlane(key);
+            const bool mine = key == lead;
+            const unsigned long long m = __ballot(mine);
+            if (mine) {
+                if (lane_rank(m) == 0) atomicAdd(&ctr[lead], __popcll(m));
+                done = true;
+            }
+        }
+    }
+    if (!done) atomicAdd(&ctr[key], 1);
+}
+
 // A k-d tree resident in HBM.  Tree positions are the reference's XI order
 // (kdtree.c:434-451 then the in-place partitions): pts holds the reduced
 // vectors in that order, zero-padded from d to dpad (a multiple of 4) so a
@@ -102,9 +125,17 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
 int leafmm_chunk();
 int leafmm_dk(int p);
 bool leafmm_supported(int p);
+// order holds output slots; slot s is A patch s / slot_div and receives its
+// K exact keys ((d2 << 32) | B index, ascending) in keys_out[s * K ...].
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
-                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int *knn, int *nn,
-                   float *dist, hipStream_t st);
+                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int slot_div,
+                   uint64_t *keys_out, hipStream_t st);
+// propagated (query, leaf) pairs: pairleaf[q * prop_k + k] (-1: none), per-leaf counts
+hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, const int *natleaf, const int *pli,
+                      int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st);
+hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st);
+hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, int K, int prop_k, int *knn, int *nn,
+                        float *dist, hipStream_t st);
 
 // learned encoder (annf_mlp.hip): parameters W1 (h1 x D), b1, W2 (h2 x h1),
 // b2, W3 (d x h2), b3 in one FP32 buffer; inputs (x - 128) / 128

@@ -121,9 +121,6 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ __forceinline__ int lane_rank(unsigned long long m) {
-    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
 
 // atomicAdd(&ctr[key], 1) for every active lane; lanes of the wavefront that
 // share a key are combined into one atomic (two rounds, then per-lane), so a
@@ -149,24 +146,6 @@ __device__ __forceinline__ int wave_add1(int *ctr, int key) {
     }
     if (!done) res = atomicAdd(&ctr[key], 1);
     return res;
-}
-
-// ctr[key] += 1 per lane: lanes sharing the wave's first two keys add once
-// per key, the rest on their own; no return value, so nothing waits.
-__device__ __forceinline__ void wave_count1(int *ctr, int key) {
-    bool done = false;
-    for (int round = 0; round < 2; ++round) {
-        if (!done) {
-            const int lead = __builtin_amdgcn_readfirstlane(key);
-            const bool mine = key == lead;
-            const unsigned long long m = __ballot(mine);
-            if (mine) {
-                if (lane_rank(m) == 0) atomicAdd(&ctr[lead], __popcll(m));
-                done = true;
-            }
-        }
-    }
-    if (!done) atomicAdd(&ctr[key], 1);
 }
 
 template <typename Fn>
@@ -1118,10 +1097,13 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int *count, int n, i
     if (threadIdx.x == blockDim.x - 1) offset[n] = run;
 }
 
+// order[cursor[leaf[i]]++] = i; entries with leaf < 0 are left out
 __global__ void k_bucket(const int *leaf, int nq, int *cursor, int *order) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nq) return;
-    const int p = wave_add1(cursor, leaf[i]);
+    const int l = leaf[i];
+    if (l < 0) return;
+    const int p = wave_add1(cursor, l);
     order[p] = i;
 }
 

@@ -275,8 +275,8 @@ template <int DK, int LM, bool FP8>
 __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, PatchGeom gb,
                                                          const int *__restrict__ order,
                                                          const int *__restrict__ offset,
-                                                         const int *__restrict__ chunk_off, int nprobe, int K, int *__restrict__ knn, int *__restrict__ nn,
-                                                         float *__restrict__ dist) {
+                                                         const int *__restrict__ chunk_off, int nprobe, int K,
+                                                         int slot_div, uint64_t *__restrict__ keys_out) {
     constexpr int RB = DK + 4;                              // LDS patch stride (bytes, odd dword count)
     constexpr int kBD = (kNP * RB > kMQ * kDS * 4) ? kNP * RB : kMQ * kDS * 4;
     extern __shared__ __align__(16) uint8_t smem[];
@@ -315,7 +315,8 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
     // queries: one quad per query
     {
         const bool v = q < nqb;
-        const int qa = v ? order[qb + q] : 0;
+        const int slot = v ? order[qb + q] : 0;   // output slot; its A patch is slot / slot_div
+        const int qa = slot / slot_div;
         uint32_t *row = reinterpret_cast<uint32_t *>(Aq + q * RB);
         uint32_t ss, sm;
         stage_patch(ga, qa, v, row, rw, tmask, dkw, q4, ss, sm);
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
         } else {
             if (q4 == 0) na[q] = ss;
         }
-        if (q4 == 0) qid[q] = qa;
+        if (q4 == 0) qid[q] = slot;
     }
 
     // this lane's LM smallest keys (every slot live)
@@ -467,7 +468,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
                 jj[c] = (uint32_t)key;
                 if (key != kNoKey) nc = c + 1;   // live entries come first
             }
-            exact_d2_rows(ga, qid[q], gb, jj, nc, rw, tmask, d2);
+            exact_d2_rows(ga, qid[q] / slot_div, gb, jj, nc, rw, tmask, d2);
 #pragma unroll
             for (int c = 0; c < NC; ++c)
                 if (c < nc) cand[q * LM + q4 + 4 * c] = ((uint64_t)d2[c] << 32) | jj[c];
@@ -488,15 +489,75 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
     }
     __syncthreads();
     if (q < nqb) {
-        const int qa = qid[q];
-        for (int s = q4; s < K; s += 4) {
-            const uint64_t key = cand[q * LM + s];
-            const int j = key == kNoKey ? -1 : (int)(uint32_t)key;
-            if (knn) knn[(size_t)qa * K + s] = j;
-            if (s == 0) {
-                nn[qa] = j;
-                if (dist) dist[qa] = key == kNoKey ? kFltMax : sqrt_f((uint32_t)(key >> 32));
-            }
+        const size_t o = (size_t)qid[q] * K;
+        for (int s = q4; s < K; s += 4) keys_out[o + s] = cand[q * LM + s];
+    }
+}
+
+// Propagated leaves (the parallel form of base.c:255-282): for A patch q
+// below the first row and k < prop_k, the B patch one row below the k-th
+// match of the patch above (first-pass keys), its leaf pli[.]; leaves equal
+// to q's natural leaf or to an earlier k are dropped (-1).  Counts the kept
+// (q, k) pairs per leaf.
+__global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K, int prop_k,
+                             const int *__restrict__ natleaf, const int *__restrict__ pli, int n_cols, int n_b,
+                             int *__restrict__ pairleaf, int *__restrict__ count) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)n_a * prop_k) return;
+    const int q = (int)(t / prop_k), k = (int)(t - (long long)q * prop_k);
+    int leaf = -1;
+    if (q >= n_cols) {
+        auto leaf_of = [&](int kk) -> int {
+            const uint64_t key = keys1[(size_t)(q - n_cols) * K + kk];
+            if (key == kNoKey) return -1;
+            int o = (int)(uint32_t)key;
+            if (o + n_cols < n_b) o += n_cols;
+            return pli[o];
+        };
+        leaf = leaf_of(k);
+        if (leaf == natleaf[q]) leaf = -1;
+        for (int kk = 0; kk < k && leaf >= 0; ++kk)
+            if (leaf_of(kk) == leaf) leaf = -1;
+    }
+    pairleaf[t] = leaf;
+    if (leaf >= 0) wave_count1(count, leaf);
+}
+
+__global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) keys[i] = kNoKey;
+}
+
+// The K smallest of a query's first-pass keys and its propagated pairs'
+// keys (disjoint leaves, so no duplicates) -> knn, nn, dist.
+template <int KM>
+__global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__restrict__ keys2, int n_a, int K,
+                         int prop_k, int *__restrict__ knn, int *__restrict__ nn, float *__restrict__ dist) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n_a) return;
+    uint64_t l[KM];
+    klist_init(l, K);
+    for (int s = 0; s < K; ++s) {
+        const uint64_t key = keys1[(size_t)q * K + s];
+        if (key < l[KM - 1]) klist_insert(l, key);
+    }
+    if (keys2) {
+        const uint64_t *p2 = keys2 + (size_t)q * prop_k * K;
+        for (int s = 0; s < prop_k * K; ++s) {
+            const uint64_t key = p2[s];
+            if (key < l[KM - 1]) klist_insert(l, key);
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < KM; ++s) {
+        if (s < KM - K) continue;
+        const uint64_t key = l[s];
+        const int j = key == kNoKey ? -1 : (int)(uint32_t)key;
+        const int r = s - (KM - K);
+        if (knn) knn[(size_t)q * K + r] = j;
+        if (r == 0) {
+            nn[q] = j;
+            if (dist) dist[q] = key == kNoKey ? kFltMax : sqrt_f((uint32_t)(key >> 32));
         }
     }
 }
@@ -529,14 +590,40 @@ int leafmm_dk(int p) {
 
 bool leafmm_supported(int p) { return p >= 1 && leafmm_dk(p) <= 512; }
 
+hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, const int *natleaf, const int *pli,
+                      int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st) {
+    const long long n = (long long)n_a * prop_k;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prop_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys1, n_a, K, prop_k,
+                       natleaf, pli, n_cols, n_b, pairleaf, count);
+    return hipGetLastError();
+}
+
+hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n);
+    return hipGetLastError();
+}
+
+hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, int K, int prop_k, int *knn, int *nn,
+                        float *dist, hipStream_t st) {
+    if (n_a == 0) return hipSuccess;
+    const dim3 g((unsigned)((n_a + 255) / 256)), b(256);
+    if (K <= 8)
+        hipLaunchKernelGGL(k_finish<8>, g, b, 0, st, keys1, keys2, n_a, K, prop_k, knn, nn, dist);
+    else
+        hipLaunchKernelGGL(k_finish<32>, g, b, 0, st, keys1, keys2, n_a, K, prop_k, knn, nn, dist);
+    return hipGetLastError();
+}
+
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
-                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int *knn, int *nn,
-                   float *dist, hipStream_t st) {
+                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int slot_div,
+                   uint64_t *keys_out, hipStream_t st) {
     if (nq == 0) return hipSuccess;
     const int p = ga.rows;
     if (!leafmm_supported(p) || gb.rows != p || ga.rowlen != 3 * p || gb.rowlen != 3 * p) return hipErrorInvalidValue;
     const int nleaves = 1 << t.depth;
-    if (nprobe < 1 || nprobe > nleaves || K < 1 || K > 32) return hipErrorInvalidValue;
+    if (nprobe < 1 || nprobe > nleaves || K < 1 || K > 32 || slot_div < 1) return hipErrorInvalidValue;
     // list length: K for BF16 (exact keys), the shortlist for FP8; rounded
     // up to an instantiated length (a longer list only adds candidates)
     const int L = fp8 ? shortlist : K;
@@ -555,7 +642,7 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb, order,
-                               offset, chunk_off, nprobe, K, knn, nn, dist);
+                               offset, chunk_off, nprobe, K, slot_div, keys_out);
             return hipGetLastError();
         };
         return fp8 ? launch(std::true_type{}) : launch(std::false_type{});

@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 BF16_PEAK_TFS = 2500.0         # MI355X dense BF16 MFMA, 2.5 PFLOP/s
 PLAIN_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_scan", "row0_exact", "row_sweep",
                 "best_of_k"]
-MM_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_bucket", "leaf_knn_mfma"]
+MM_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_bucket", "leaf_knn_mfma", "propagated_pass"]
 DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz")
 
 
@@ -159,7 +159,7 @@ def main():
     def mm(proj, knn_out=None):
         _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
             h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, proj, depth, K,
-            args.nprobe, _lib.PREC_BF16, K, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
+            args.nprobe, _lib.PREC_BF16, K, 0, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
 
     if not args.no_quality:
         q = metrics.sample_queries(n_a, 8192)
@@ -183,13 +183,13 @@ def main():
                                **quality()}
     if "pca" in run_paths:
         # PCA + BF16 full-patch leaf k-NN
-        ms, stage = timed(lambda: mm(_lib.PROJ_PCA32), args.steps, args.warmup, torch, _lib, h, 5)
+        ms, stage = timed(lambda: mm(_lib.PROJ_PCA32), args.steps, args.warmup, torch, _lib, h, 6)
         mm(_lib.PROJ_PCA32, knn)
         paths["pca_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
                              "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)}, **quality()}
     chosen = "pca_bf16" if "pca_bf16" in paths else None
     if encoder is not None and "learned" in run_paths:
-        ms, stage = timed(lambda: mm(_lib.PROJ_ENCODER), args.steps, args.warmup, torch, _lib, h, 5)
+        ms, stage = timed(lambda: mm(_lib.PROJ_ENCODER), args.steps, args.warmup, torch, _lib, h, 6)
         mm(_lib.PROJ_ENCODER, knn)
         paths["learned_bf16"] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
                                  "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)},

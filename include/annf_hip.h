@@ -143,20 +143,28 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
  *     per patch by 448 / max|x - 128|; the `shortlist` smallest approximate
  *     distances (n_neighbors <= shortlist <= 32) are re-ranked with exact
  *     distances into the K list.
+ * propagate = m > 0 adds the reference's propagated leaves (base.c:255-282)
+ * as a second, parallel pass: for a patch below the first row, the leaves of
+ * the B patches one row below the first min(m, K) first-pass matches of the
+ * patch above (m >= K: all of them, the reference's rule; dropping its
+ * natural leaf and repeats) are scanned the same way, and the K smallest
+ * keys of both passes are kept.  The candidates come from the
+ * first pass of the row above, not from its propagated result (the
+ * reference sweeps rows in order); this needs wa == wb.
  * Outputs as annf_field_dev: nn_out is the first list entry, dist_out its
  * exact L2 (correctly rounded), knn_out the K list (-1 past the candidates).
- * No propagation, no best-of-K pass: the list is already ranked by the full
- * patch distance.  Stage marks: 0 project A, 1 project B, 2 tree build,
- * 3 natural leaves + bucketing, 4 MFMA leaf k-NN. */
+ * No best-of-K pass: the list is already ranked by the full patch distance.
+ * Stage marks: 0 project A, 1 project B, 2 tree build, 3 natural leaves +
+ * bucketing, 4 MFMA leaf k-NN, 5 propagated pass + merge. */
 #define ANNF_PREC_BF16 0
 #define ANNF_PREC_FP8 1
 int annf_field_mm_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                       const uint8_t *img_b, int hb, int wb, int psize,
                       const float *basis, const double *mean, int d,
                       int projection, int tree_depth, int n_neighbors,
-                      int nprobe, int precision, int shortlist, int *nn_out,
-                      float *dist_out, int *knn_out, float *red_a_out,
-                      float *red_b_out);
+                      int nprobe, int precision, int shortlist, int propagate,
+                      int *nn_out, float *dist_out, int *knn_out,
+                      float *red_a_out, float *red_b_out);
 
 /* ---- learned patch encoder (no reference counterpart; replaces the PCA
  *      projection of openmp/base.py:193-200 as the tree's feature map) ----

@@ -155,6 +155,41 @@ def leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k):
     return idx, dst
 
 
+def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k):
+    """The full-patch leaf k-NN with the parallel propagated pass: pass one is
+    leaf_knn_exact over the natural leaf; a patch below the first row then
+    also scans the leaves pli[o'] of its row-above neighbour's pass-one
+    matches o, shifted one row down (o' = o + n_cols when that is a B patch,
+    base.c:265-276), leaving out its natural leaf and repeats, and keeps the
+    k smallest (squared distance, B index) over all scanned points.
+    Returns (idx [n_a, k], sqdist [n_a, k]) int64, -1 past the candidates."""
+    i1, d1 = leaf_knn_exact(pa, pb, qleaf, leaves, orig, 1, k)
+    n_a, n_b = pa.shape[0], pb.shape[0]
+    out_i, out_d = i1.copy(), d1.copy()
+    A = pa.astype(np.int64)
+    B = pb.astype(np.int64)
+    orig = np.asarray(orig, np.int64)
+    for q in range(n_cols, n_a):
+        ls = [int(qleaf[q])]
+        for j in range(k):
+            o = int(i1[q - n_cols, j])
+            if o < 0:
+                continue
+            if o + n_cols < n_b:
+                o += n_cols
+            lf = int(pli[o])
+            if lf not in ls:
+                ls.append(lf)
+        if len(ls) == 1:
+            continue
+        cand = orig[np.concatenate([np.arange(leaves[2 * lf], leaves[2 * lf + 1]) for lf in ls])]
+        d2 = ((A[q][None, :] - B[cand]) ** 2).sum(1)
+        order = np.lexsort((cand, d2))[:k]
+        out_i[q, :order.size] = cand[order]
+        out_d[q, :order.size] = d2[order]
+    return out_i, out_d
+
+
 # --------------------------------------------------------------------------
 # the C restatement (liboracle.so)
 # --------------------------------------------------------------------------

@@ -134,3 +134,48 @@ def test_variant_arguments_rejected(gpu_available):
         field_mm_on_device(a, b, 5, comp, mu, 4, 5, precision="fp8", shortlist=4)   # shortlist < K
     with pytest.raises(ANNFError):
         field_mm_on_device(a, b, 5, comp, mu, 4, 5, nprobe=32)         # more leaves than the tree has
+
+
+@pytest.mark.parametrize("p,K,seed", [(7, 5, 41), (5, 3, 42), (9, 8, 43)])
+def test_bf16_propagation_matches_oracle(p, K, seed, gpu_available):
+    """The propagated pass: the K best exact keys over the natural leaf and
+    the leaves named by the row above's first-pass matches."""
+    from annfmp_amd import synth, field_mm_on_device
+    a, b = synth.make_pair(synth.small_spec(96, 128), seed=seed)
+    pa, pb = O.extract_patches(a, p), O.extract_patches(b, p)
+    comp, mu = _basis(pa, pb, 8)
+    depth = O.tree_depth(pb.shape[0], 32)
+    res = field_mm_on_device(a, b, p, comp, mu, depth, K, nprobe=1, precision="bf16", propagate=True,
+                             return_reduced=True)
+    tree = O.OracleTree(res["red_b"], depth)
+    qleaf = tree.natural_leaves(res["red_a"])
+    n_cols = a.shape[1] - p + 1
+    wi, wd = O.leaf_knn_propagated(pa, pb, qleaf, tree.leaves, tree.orig, tree.pli, n_cols, K)
+    rows = np.arange(pa.shape[0])
+    _check_bf16(res, rows, wi, wd)
+    # propagation only adds candidates: never worse than the first pass
+    i1, d1 = O.leaf_knn_exact(pa, pb, qleaf, tree.leaves, tree.orig, 1, K)
+    assert (wd[:, 0] <= d1[:, 0]).all() and (wd[:, 0] < d1[:, 0]).any()
+
+
+def test_fp8_propagation_properties(gpu_available):
+    from annfmp_amd import synth, field_mm_on_device
+    a, b = synth.make_pair(synth.small_spec(96, 128), seed=44)
+    p, K = 9, 8
+    pa, pb = O.extract_patches(a, p), O.extract_patches(b, p)
+    comp, mu = _basis(pa, pb, 8)
+    depth = O.tree_depth(pb.shape[0], 32)
+    res = field_mm_on_device(a, b, p, comp, mu, depth, K, nprobe=1, precision="fp8", shortlist=32,
+                             propagate=True, return_reduced=True)
+    tree = O.OracleTree(res["red_b"], depth)
+    qleaf = tree.natural_leaves(res["red_a"])
+    n_cols = a.shape[1] - p + 1
+    wi, wd = O.leaf_knn_propagated(pa, pb, qleaf, tree.leaves, tree.orig, tree.pli, n_cols, K)
+    knn = res["knn"].astype(np.int64)
+    rows = np.arange(pa.shape[0])
+    d2 = O.exact_pair_d2(pa, pb, np.repeat(rows, K), knn.ravel()).reshape(-1, K)
+    assert (np.diff(d2 * (1 << 32) + knn, axis=1) > 0).all()
+    assert (d2 >= wd).all()
+    hit = (knn[:, 0] == wi[:, 0]).mean()
+    print(f"fp8 + propagation: exact propagated nearest found on {hit:.4f} of queries")
+    assert hit >= 0.8

@@ -217,3 +217,32 @@ def test_leaf_knn_exact_reduces_to_exact_field():
         dd = [(int(((pa[r].astype(int) - pb[c].astype(int)) ** 2).sum()), int(c)) for c in cand]
         dd.sort()
         assert [c for _, c in dd[:3]] == list(i3[r]) and [x for x, _ in dd[:3]] == list(d3[r])
+
+
+def test_leaf_knn_propagated_is_the_best_of_its_leaves():
+    """Every query's list is exactly ordered, and nothing among the points of
+    its natural leaf and its propagated leaves (collected here row by row, as
+    base.c:255-282 does) is closer than its k-th entry."""
+    rng = np.random.default_rng(5)
+    n_rows, n_cols, D, k = 9, 11, 12, 3
+    n = n_rows * n_cols
+    pa = rng.integers(0, 256, (n, D)).astype(np.uint8)
+    pb = rng.integers(0, 256, (n, D)).astype(np.uint8)
+    leaves = np.array([0, 25, 25, 50, 50, 74, 74, 99])
+    orig = rng.permutation(n)
+    pli = np.empty(n, np.int64)
+    for lf in range(4):
+        pli[orig[leaves[2 * lf]:leaves[2 * lf + 1]]] = lf
+    qleaf = rng.integers(0, 4, n)
+    i1, _ = O.leaf_knn_exact(pa, pb, qleaf, leaves, orig, 1, k)
+    ip, dp = O.leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k)
+    for q in range(n):
+        allowed = {int(qleaf[q])}
+        if q >= n_cols:
+            for o in i1[q - n_cols]:
+                o2 = o + n_cols if o + n_cols < n else o
+                allowed.add(int(pli[o2]))
+        pts = np.concatenate([orig[leaves[2 * lf]:leaves[2 * lf + 1]] for lf in sorted(allowed)])
+        d_all = sorted((int(((pa[q].astype(int) - pb[j].astype(int)) ** 2).sum()), int(j)) for j in pts)
+        assert [j for _, j in d_all[:k]] == list(ip[q]) and [x for x, _ in d_all[:k]] == list(dp[q])
+    assert (ip[:n_cols] == i1[:n_cols]).all()
