@@ -101,6 +101,14 @@ def lib(path=None):
             if not os.path.exists(p):
                 raise HipLibraryMissing(
                     f"{p} is missing: build it with `python -m annfmp_amd.build` (no CPU fallback exists)")
+            # let torch bring up the device first: a HIP runtime started by
+            # this library alone leaves torch without a visible GPU
+            try:
+                import torch
+                if torch.cuda.is_available():
+                    torch.cuda.init()
+            except ImportError:
+                pass
             L = C.CDLL(p)
             _declare(L)
             _lib = L

@@ -42,6 +42,8 @@ PLAIN_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_scan", "ro
                 "best_of_k"]
 MM_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_bucket", "leaf_knn_mfma", "propagated_pass"]
 DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz")
+DEFAULT_ENCODER_4K = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p9_d16.npz")
+FP8_PEAK_TFS = 5000.0          # MI355X dense FP8 MFMA, 5.0 PFLOP/s
 
 
 def fit_basis(a, p, d, n_samples, seed):
@@ -89,6 +91,96 @@ def timed(fn, steps, warmup, torch, lib, h, nst):
     return (t1 - t0) * 1000.0 / steps, acc / steps
 
 
+def run_4k(args):
+    """The combined form at 4K: 3840x2160 warp pair, p = 9 (243-dim), d = 16,
+    leaf 128 (depth 15), K = 8.  Times the plain FP32 path (PCA-16,
+    propagation, best-of-K) and the learned encoder with FP8 full-patch leaf
+    distances (top-32 exact re-rank) and the propagated pass, plus the same
+    with BF16; recall@8 against the exact field.  One JSON line."""
+    import torch
+    from annfmp_amd import synth, _lib, metrics, device as dev
+    from annfmp_amd.kdtree import tree_depth_for
+    from annfmp_amd.encoder import PatchEncoder
+
+    p, d, leaf, K, S = 9, 16, 128, 8, 32
+    prop = K if args.propagate is None else args.propagate
+    a, b = synth.make_pair("4k-warp", seed=args.seed)
+    H, W, _ = a.shape
+    comp, mu = fit_basis(a, p, d, 50000, args.seed)
+    n_a = (H - p + 1) * (W - p + 1)
+    n_b = n_a
+    depth = tree_depth_for(n_b, leaf)
+    cuda = dev.cuda_device(0)
+    ia, ib = torch.from_numpy(a).to(cuda), torch.from_numpy(b).to(cuda)
+    tc = torch.from_numpy(comp.astype(np.float32)).to(cuda)
+    tm = torch.from_numpy(mu.astype(np.float64)).to(cuda)
+    nn = torch.empty(n_a, dtype=torch.int32, device=cuda)
+    knn = torch.empty((n_a, K), dtype=torch.int32, device=cuda)
+    h = _lib.Handle(0)
+    L = _lib.lib()
+    _lib.check("annf_set_timing", L.annf_set_timing(h.h, 1))
+    enc_path = args.encoder if args.encoder != DEFAULT_ENCODER else DEFAULT_ENCODER_4K
+    encoder = None
+    if args.encoder != "none" and os.path.exists(enc_path):
+        encoder = PatchEncoder.load(enc_path)
+        encoder.install(h)
+    proj = _lib.PROJ_ENCODER if encoder is not None else _lib.PROJ_PCA32
+
+    def plain(knn_out=None):
+        _lib.check("annf_field_dev", L.annf_field_dev(
+            h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, _lib.PROJ_PCA32,
+            depth, K, 1, 1, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
+
+    def mm(prec, shortlist, propagate, knn_out=None):
+        _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
+            h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, proj, depth, K, 1,
+            prec, shortlist, propagate, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
+
+    q = metrics.sample_queries(n_a, 8192)
+    exact = metrics.exact_knn(a, b, p, q, K, handle=h)
+
+    def quality():
+        torch.cuda.synchronize()
+        return metrics.field_quality(a, b, p, nn.cpu().numpy(), knn.cpu().numpy(), k=K, handle=h, exact=exact)
+
+    paths = {}
+    ms, stage = timed(plain, args.steps, args.warmup, torch, _lib, h, 7)
+    plain(knn)
+    paths["plain_fp32"] = {"ms_per_step": round(ms, 3), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                           "stage_ms": {s_: round(float(v), 3) for s_, v in zip(PLAIN_STAGES, stage)}, **quality()}
+    flops_pass1 = 2.0 * (p * p * 3) * n_a * (n_b / (1 << depth))
+    runs = (("learned_fp8_prop", _lib.PREC_FP8, S, prop), ("learned_bf16_prop", _lib.PREC_BF16, K, prop),
+            ("learned_bf16_prop2", _lib.PREC_BF16, K, 2), ("learned_fp8", _lib.PREC_FP8, S, 0),
+            ("learned_bf16", _lib.PREC_BF16, K, 0))
+    for name, prec, sl, pr in runs:
+        ms, stage = timed(lambda: mm(prec, sl, pr), args.steps, args.warmup, torch, _lib, h, 6)
+        mm(prec, sl, pr, knn)
+        peak = FP8_PEAK_TFS if prec == _lib.PREC_FP8 else BF16_PEAK_TFS
+        leaf_ms = float(stage[4])
+        paths[name] = {"ms_per_step": round(ms, 3), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                       "precision": "fp8" if prec == _lib.PREC_FP8 else "bf16", "shortlist": sl,
+                       "propagated_entries": pr,
+                       "stage_ms": {s_: round(float(v), 3) for s_, v in zip(MM_STAGES, stage)},
+                       "leaf_pass_mfma_tflops": round(flops_pass1 / (leaf_ms * 1e-3) / 1e12, 2),
+                       "leaf_pass_mfma_util": round(flops_pass1 / (leaf_ms * 1e-3) / 1e12 / peak, 5),
+                       **quality()}
+    main_p = paths["learned_fp8_prop"]
+    out = {
+        "metric": "query patches/sec, 4K pair: learned+FP8 (top-32 re-rank, propagation) vs plain FP32, recall@8",
+        "value": main_p["query_patches_per_s"], "unit": "query patches/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": main_p["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp8", "data": "synthetic",
+        "config": {"workload": "3840x2160 warp pair (seed %d), p=9 (243-dim), %s, leaf 128 (depth %d), K=8, "
+                               "FP8-E4M3 full-patch leaf distances, top-32 exact re-rank, %d propagated entries"
+                               % (args.seed, "learned encoder 243-128-128-16" if encoder is not None else "PCA-16",
+                                  depth, prop),
+                   "queries_per_step": n_a, "encoder": os.path.relpath(enc_path, ROOT) if encoder else None},
+        "speedup_vs_plain_fp32": round(main_p["query_patches_per_s"] / paths["plain_fp32"]["query_patches_per_s"], 3),
+        "paths": paths,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +193,10 @@ def main():
     ap.add_argument("--paths", default="plain,pca,learned",
                     help="paths to run (profiling runs select one); the value needs pca and learned")
     ap.add_argument("--no-quality", action="store_true", help="skip the exact-field quality evaluation")
+    ap.add_argument("--workload", default="1080p", choices=["1080p", "4k"],
+                    help="1080p: the metric (default); 4k: the combined form on the 3840x2160 pair")
+    ap.add_argument("--propagate", type=int, default=None,
+                    help="4k: list entries that propagate on the MFMA path (default K = 8)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="only run 5 device-to-device copies of MB MiB (scales the HBM byte counters) and exit")
     args = ap.parse_args()
@@ -114,6 +210,12 @@ def main():
         print(json.dumps({"copy_calibration_bytes": args.copy_calibration << 20, "copies": 5}))
         return
     run_paths = set(args.paths.split(","))
+    if args.workload == "4k":
+        if args.gpus != 1:
+            print("bench.py measures one GPU only", file=sys.stderr)
+            sys.exit(2)
+        run_4k(args)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(the field would shard by image columns; see the design notes)", file=sys.stderr)
