@@ -25,7 +25,7 @@ EXPORTS = (
     "annf_neighbors", "annf_process_rows", "annf_norm_array", "annf_free_resources", "annf_tree_info",
     "annf_tree_export", "annf_field_dev", "annf_stage_times", "annf_set_timing", "annf_exact_field_dev",
     "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
-    "annf_set_encoder", "annf_encode_image_dev",
+    "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev",
 )
 
 PROJ_PCA32 = 0
@@ -87,6 +87,7 @@ def _declare(L):
     L.annf_mlp_adam_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_longlong, f, f, f, f, i]
     L.annf_set_encoder.argtypes = [_vp, _vp, i, i, i, i]
     L.annf_encode_image_dev.argtypes = [_vp, _vp, i, i, i, _vp]
+    L.annf_pca_cov_dev.argtypes = [_vp, _vp, i, i, _vp, i, _vp, i, i, _vp, i, i, _vp, _vp]
     for name in EXPORTS:
         if name not in ("annf_last_error", "annf_stream", "annf_norm_array"):
             getattr(L, name).restype = C.c_int

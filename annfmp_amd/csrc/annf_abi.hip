@@ -93,6 +93,7 @@ struct annf_ctx {
     DevBuf mlp_ws, enc_w1, enc_b1, enc_w2, enc_b2, enc_w3, enc_b3;
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
+    DevBuf pca_scratch;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // timing
@@ -129,7 +130,7 @@ struct annf_ctx {
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
-                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf};
+                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch};
         for (DevBuf *b : list) b->release();
         has_tree = false;
         has_encoder = false;
@@ -845,6 +846,27 @@ int annf_encode_image_dev(annf_ctx *c, const uint8_t *img, int h, int w, int psi
     HIPCHK(encode_patches(g, n, c->enc_w1.as<__bf16>(), c->enc_b1.as<float>(), c->enc_w2.as<__bf16>(),
                           c->enc_b2.as<float>(), c->enc_w3.as<__bf16>(), c->enc_b3.as<float>(), c->enc_d, c->enc_d,
                           out, c->stream));
+    return sync_stream(c);
+}
+
+int annf_pca_cov_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const int *idx_a, int n_a,
+                     const uint8_t *img_b, int hb, int wb, const int *idx_b, int n_b, int psize, double *mean_out,
+                     double *cov_out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!mean_out || !cov_out) return fail(ANNF_EINVAL, "null output");
+    if (n_a < 0 || n_b < 0 || n_a + n_b < 2) return fail(ANNF_EINVAL, "need at least two samples");
+    if ((n_a > 0 && (!img_a || !idx_a)) || (n_b > 0 && (!img_b || !idx_b))) return fail(ANNF_EINVAL, "null input");
+    const int p = psize;
+    if (p < 1 || (n_a > 0 && (ha < p || wa < p)) || (n_b > 0 && (hb < p || wb < p)))
+        return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
+    if ((long long)n_a + n_b > (1LL << 24))
+        return fail(ANNF_EUNSUPPORTED, "at most 2^24 samples (exact integer sums in FP64)");
+    RCHK(set_device(c));
+    const int D = p * p * 3;
+    const PatchGeom ga{img_a, 1, n_a > 0 ? wa - p + 1 : 1, wa, p, 3 * p, 3LL * wa, 3LL * wa * ha};
+    const PatchGeom gb{img_b, 1, n_b > 0 ? wb - p + 1 : 1, wb, p, 3 * p, 3LL * wb, 3LL * wb * hb};
+    RCHK(c->pca_scratch.ensure(pca_cov_scratch(D, n_a + n_b)));
+    HIPCHK(pca_cov(ga, idx_a, n_a, gb, idx_b, n_b, D, c->pca_scratch.as<uint8_t>(), mean_out, cov_out, c->stream));
     return sync_stream(c);
 }
 

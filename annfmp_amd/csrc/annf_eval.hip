@@ -203,3 +203,118 @@ hipError_t exact_sample(const uint8_t *am, const uint32_t *an, const int *qidx, 
 }
 
 }  // namespace annf
+
+// ---------------------------------------------------------------------------
+// PCA covariance in FP64 (scikit-learn's covariance_eigh form, _pca.py 1.7.2:
+// C = X^T X; C -= (n * mean_col) * mean_row; C /= n - 1) of sampled patches.
+// On uint8 patches X^T X and the column sums are exact integers (uint64
+// accumulation, any order), so only the three FP64 operations per entry
+// round, in the library's order: the result equals the CPU computation bit
+// for bit.
+// ---------------------------------------------------------------------------
+namespace annf {
+namespace {
+
+// XT[f][s] (D x ns, ns padded to a multiple of 4 with zeros) of the sampled
+// patches: samples [0, na) from image a (indices ia), then nb from image b.
+__global__ void k_gather_t(PatchGeom ga, const int *__restrict__ ia, int na, PatchGeom gb,
+                           const int *__restrict__ ib, int nb, int D, int ns_pad, uint8_t *__restrict__ XT) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)ns_pad * D) return;
+    const int f = (int)(t / ns_pad), s = (int)(t - (long long)f * ns_pad);
+    uint8_t v = 0;
+    if (s < na + nb) {
+        const PatchGeom &g = s < na ? ga : gb;
+        const long long i = s < na ? ia[s] : ib[s - na];
+        const long long y = i / g.ncols, x = i - y * g.ncols;
+        const int r = f / g.rowlen, u = f - r * g.rowlen;
+        v = g.base[(y * g.width + x) * 3 + r * g.rstride + u];
+    }
+    XT[(size_t)f * ns_pad + s] = v;
+}
+
+// column sums S[f] = sum_s x[s][f], exact
+__global__ __launch_bounds__(256) void k_feature_sums(const uint8_t *__restrict__ XT, int ns_pad,
+                                                      unsigned long long *__restrict__ S) {
+    __shared__ unsigned long long part[256];
+    const int f = blockIdx.x;
+    const uint32_t *row = reinterpret_cast<const uint32_t *>(XT + (size_t)f * ns_pad);
+    unsigned long long acc = 0;
+    for (int w = threadIdx.x; w < ns_pad / 4; w += 256) acc += __builtin_amdgcn_udot4(row[w], 0x01010101u, 0u, false);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) S[f] = part[0];
+}
+
+// Gram G[i][j] = sum_s x[s][i] x[s][j], exact: 32 x 32 tiles, 256-sample
+// chunks staged in LDS, dot4 per 4 samples into a per-chunk uint32 (at most
+// 256 * 65025 < 2^32), summed in uint64.
+__global__ __launch_bounds__(1024) void k_gram(const uint8_t *__restrict__ XT, int ns_pad, int D,
+                                               unsigned long long *__restrict__ G) {
+    __shared__ uint32_t ti[32][65], tj[32][65];
+    const int i0 = blockIdx.y * 32, j0 = blockIdx.x * 32;
+    const int ii = threadIdx.x >> 5, jj = threadIdx.x & 31;
+    unsigned long long acc = 0;
+    const int nw = ns_pad / 4;
+    for (int w0 = 0; w0 < nw; w0 += 64) {
+        const int cw = min(64, nw - w0);
+        for (int e = threadIdx.x; e < 32 * 64; e += 1024) {
+            const int r = e >> 6, w = e & 63;
+            const bool in = w < cw;
+            ti[r][w] = (in && i0 + r < D) ? reinterpret_cast<const uint32_t *>(XT + (size_t)(i0 + r) * ns_pad)[w0 + w]
+                                          : 0u;
+            tj[r][w] = (in && j0 + r < D) ? reinterpret_cast<const uint32_t *>(XT + (size_t)(j0 + r) * ns_pad)[w0 + w]
+                                          : 0u;
+        }
+        __syncthreads();
+        uint32_t part = 0;
+        for (int w = 0; w < cw; ++w) part = __builtin_amdgcn_udot4(ti[ii][w], tj[jj][w], part, false);
+        acc += part;
+        __syncthreads();
+    }
+    if (i0 + ii < D && j0 + jj < D) G[(size_t)(i0 + ii) * D + j0 + jj] = acc;
+}
+
+__global__ void k_cov_final(const unsigned long long *__restrict__ S, const unsigned long long *__restrict__ G, int D,
+                            long long n, double *__restrict__ mean, double *__restrict__ C) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < D) mean[t] = (double)S[t] / (double)n;
+    if (t >= (long long)D * D) return;
+    const int i = (int)(t / D), j = (int)(t - (long long)i * D);
+    const double mi = (double)S[i] / (double)n, mj = (double)S[j] / (double)n;
+    const double sub = ((double)n * mi) * mj;
+    C[t] = ((double)G[t] - sub) / (double)(n - 1);
+}
+
+}  // namespace
+
+size_t pca_cov_scratch(int D, int ns) {
+    const size_t ns_pad = (size_t)(ns + 3) / 4 * 4;
+    return (size_t)D * ns_pad + (size_t)D * sizeof(unsigned long long) + (size_t)D * D * sizeof(unsigned long long) +
+           256;
+}
+
+hipError_t pca_cov(const PatchGeom &ga, const int *ia, int na, const PatchGeom &gb, const int *ib, int nb, int D,
+                   uint8_t *scratch, double *mean, double *C, hipStream_t st) {
+    const int ns = na + nb;
+    const int ns_pad = (ns + 3) / 4 * 4;
+    uint8_t *XT = scratch;
+    size_t off = ((size_t)D * ns_pad + 255) / 256 * 256;
+    unsigned long long *S = reinterpret_cast<unsigned long long *>(scratch + off);
+    unsigned long long *G = S + D;
+    const long long tot = (long long)ns_pad * D;
+    hipLaunchKernelGGL(k_gather_t, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, ga, ia, na, gb, ib, nb, D,
+                       ns_pad, XT);
+    hipLaunchKernelGGL(k_feature_sums, dim3(D), dim3(256), 0, st, XT, ns_pad, S);
+    hipLaunchKernelGGL(k_gram, dim3((D + 31) / 32, (D + 31) / 32), dim3(1024), 0, st, XT, ns_pad, D, G);
+    const long long dd = (long long)D * D;
+    hipLaunchKernelGGL(k_cov_final, dim3((unsigned)((dd + 255) / 256)), dim3(256), 0, st, S, G, D, (long long)ns,
+                       mean, C);
+    return hipGetLastError();
+}
+
+}  // namespace annf

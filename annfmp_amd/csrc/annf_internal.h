@@ -163,6 +163,12 @@ hipError_t exact_sample(const uint8_t *am, const uint32_t *an, const int *qidx, 
                         const uint32_t *bn, long long nb, int dpad_b, int k, uint64_t *part, int *idx, int *d2,
                         hipStream_t st);
 
+// FP64 PCA covariance of sampled patches (annf_eval.hip): scratch of
+// pca_cov_scratch(D, na + nb) bytes (256-aligned)
+size_t pca_cov_scratch(int D, int ns);
+hipError_t pca_cov(const PatchGeom &ga, const int *ia, int na, const PatchGeom &gb, const int *ib, int nb, int D,
+                   uint8_t *scratch, double *mean, double *C, hipStream_t st);
+
 int dpad_of(int d);
 bool dpad_supported(int dpad);
 bool k_supported(int K);

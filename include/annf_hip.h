@@ -198,6 +198,19 @@ int annf_set_encoder(annf_ctx *ctx, const float *params, int psize, int h1,
 int annf_encode_image_dev(annf_ctx *ctx, const uint8_t *img, int h, int w,
                           int psize, float *out);
 
+/* ---- PCA covariance in FP64 (the covariance_eigh solver of scikit-learn
+ *      1.7.2 that _fit_pca_model reaches, openmp/base.py:168-191) ---------
+ * Mean (D doubles) and covariance (D x D doubles, row-major) of the patches
+ * idx_a (n_a device int32 indices into img_a's patches) followed by idx_b
+ * (n_b into img_b's), D = p*p*3: mean = sum / n; C = X^T X;
+ * C -= (n * mean_col) * mean_row; C /= n - 1 -- sklearn's expression; the
+ * sums are exact, so the result equals it bit for bit.  The eigensolve stays
+ * with the caller (LAPACK eigh, as sklearn).  n_a + n_b <= 2^24. */
+int annf_pca_cov_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
+                     const int *idx_a, int n_a, const uint8_t *img_b, int hb,
+                     int wb, const int *idx_b, int n_b, int psize,
+                     double *mean_out, double *cov_out);
+
 /* ---- exact field (evaluation reference; no reference counterpart) -----
  * For the nq A patches listed in query_idx (device int32 patch indices),
  * the k nearest B patches by exact squared L2 over the full p*p*3 uint8
