@@ -13,10 +13,16 @@ openmp/base.py:168-191).  Everything after the fit runs on the GPU through
 B, row 0, the propagation sweep and best-of-K.  Patches are never
 materialised on the host unless ``patches_a`` / ``patches_b`` is read.
 
-Extra keyword arguments (not in the reference): ``device`` (GPU ordinal) and
+Extra keyword arguments (not in the reference): ``device`` (GPU ordinal),
 ``projection`` ("fp32": FP32 accumulation; "fp64": FP64 accumulation rounded
 once to float32, which reproduces the reference's FP64 transform up to its
-last-bit rounding).  Extra outputs after ``fit``: ``nn_distances`` (full-patch
+last-bit rounding), ``leaf_distance`` ("reduced": the reference's method,
+distances in the projected space, the default; "bf16" / "fp8": the
+full-patch leaf k-NN on MFMA of ``annf_field_mm_dev``, its list ranked by the
+full-patch distance, propagation as the parallel second pass), ``encoder``
+(a ``PatchEncoder`` or the path of its ``.npz``: the learned projection
+replaces PCA), ``shortlist`` (FP8 re-rank length) and ``nprobe`` (leaves per
+query for "bf16" / "fp8").  Extra outputs after ``fit``: ``nn_distances`` (full-patch
 L2 of each chosen match, float32) and ``knn_indices`` (the K candidates of
 every patch before best-of-K, original indices).
 """
@@ -35,7 +41,7 @@ from .kdtree import tree_depth_for
 class ANNFieldPropKDTreeHIP:
     def __init__(self, n_neighbors=3, psize=8, dim_reduced=5, n_subset=1000, leaf_size=64, n_jobs=-1,
                  propagate=True, select_best_nn=True, float_type="float", verbose=1, seed=0, device=0,
-                 projection="fp32"):
+                 projection="fp32", leaf_distance="reduced", encoder=None, shortlist=20, nprobe=1):
         self.n_neighbors = n_neighbors
         self.psize = psize
         self.dim_reduced = dim_reduced
@@ -49,6 +55,17 @@ class ANNFieldPropKDTreeHIP:
         self.seed = seed
         self.device = device
         self.projection = projection
+        if leaf_distance not in ("reduced", "bf16", "fp8"):
+            raise Exception("leaf_distance must be 'reduced', 'bf16' or 'fp8', not {!r}".format(leaf_distance))
+        self.leaf_distance = leaf_distance
+        if isinstance(encoder, str):
+            from .encoder import PatchEncoder
+            encoder = PatchEncoder.load(encoder)
+        if encoder is not None and encoder.psize != psize:
+            raise Exception("encoder trained for patch size {}, psize is {}".format(encoder.psize, psize))
+        self.encoder = encoder
+        self.shortlist = shortlist
+        self.nprobe = nprobe
         self._handle = None
 
     # ------------------------------------------------------------------
@@ -62,7 +79,7 @@ class ANNFieldPropKDTreeHIP:
         self._n_rows = self.image_a.shape[0] - self.psize + 1
 
         tstart = time.time()
-        self._pca_model = self._fit_pca_model(self.image_a, self.image_b)
+        self._pca_model = None if self.encoder is not None else self._fit_pca_model(self.image_a, self.image_b)
         self._timers["pca_fit"] = time.time() - tstart
 
         # since there is just one leaf visit, this must hold (openmp/base.py:113)
@@ -70,9 +87,18 @@ class ANNFieldPropKDTreeHIP:
 
         n_b = (image_b.shape[0] - self.psize + 1) * (image_b.shape[1] - self.psize + 1)
         depth = tree_depth_for(n_b, self.leaf_size)
-        res = field_on_device(image_a, image_b, self.psize, self._pca_model.components_, self._pca_model.mean_,
-                              depth, self.n_neighbors, self.propagate, self.select_best_nn,
-                              projection=self.projection, device=self.device, handle=self._get_handle())
+        comp = self._pca_model.components_ if self._pca_model is not None else None
+        mean = self._pca_model.mean_ if self._pca_model is not None else None
+        proj = self.encoder if self.encoder is not None else self.projection
+        if self.leaf_distance == "reduced":
+            res = field_on_device(image_a, image_b, self.psize, comp, mean, depth, self.n_neighbors, self.propagate,
+                                  self.select_best_nn, projection=proj, device=self.device,
+                                  handle=self._get_handle())
+        else:
+            res = field_mm_on_device(image_a, image_b, self.psize, comp, mean, depth, self.n_neighbors,
+                                     nprobe=self.nprobe, precision=self.leaf_distance, shortlist=self.shortlist,
+                                     projection=proj, propagate=bool(self.propagate), device=self.device,
+                                     handle=self._get_handle())
         nn_indices = res["nn"]
         self.nn_distances = res["dist"]
         self.knn_indices = res["knn"]

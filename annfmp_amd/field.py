@@ -1,7 +1,9 @@
 """ANNField-compatible front (annfmp/base.py:13-142).
 
 Same constructor, ``get_params`` / ``set_params``, ``fit(image_a, image_b)``
-and ``patches_a`` / ``patches_b``.  The only algorithm provided here is
+and ``patches_a`` / ``patches_b``; extra keyword arguments are passed to the
+backend (``leaf_distance``, ``encoder``, ``shortlist``, ``nprobe``,
+``projection``, ``device``).  The only algorithm provided here is
 ``"hip"`` (the MI355X backend, ``ANNFieldPropKDTreeHIP``); the reference's
 other algorithm names raise, as an unknown name does there (base.py:72-73).
 """
@@ -15,7 +17,7 @@ REFERENCE_ONLY = ["sklearn", "naive", "openmp", "openclopt"]
 
 class ANNField:
     def __init__(self, n_neighbors=1, psize=8, dim_reduced=24, n_subset=1000, algorithm="hip", leaf_size=64,
-                 n_jobs=-1, propagate=True, select_best_nn=True, verbose=1, seed=0):
+                 n_jobs=-1, propagate=True, select_best_nn=True, verbose=1, seed=0, **backend_options):
         self.n_neighbors = n_neighbors
         self.psize = psize
         self.dim_reduced = dim_reduced
@@ -27,6 +29,8 @@ class ANNField:
         self.select_best_nn = select_best_nn
         self.verbose = verbose
         self.seed = seed
+        # options of ANNFieldPropKDTreeHIP beyond the reference's (leaf_distance, encoder, ...)
+        self.backend_options = dict(backend_options)
 
     def get_params(self, deep=True):
         return {
@@ -57,7 +61,7 @@ class ANNField:
         self._model = ANNFieldPropKDTreeHIP(
             n_neighbors=self.n_neighbors, psize=self.psize, dim_reduced=self.dim_reduced, n_subset=self.n_subset,
             leaf_size=self.leaf_size, n_jobs=self.n_jobs, propagate=self.propagate,
-            select_best_nn=self.select_best_nn, verbose=self.verbose, seed=self.seed)
+            select_best_nn=self.select_best_nn, verbose=self.verbose, seed=self.seed, **self.backend_options)
         return self._model.fit(image_a, image_b)
 
     @property

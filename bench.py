@@ -74,6 +74,18 @@ def cpu_baseline(a, b, p, red_a, red_b, depth, K, gpu_nn):
     }, bool(np.array_equal(final, gpu_nn))
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
+    None when the summary has no line for it."""
+    import re
+    path = os.path.join(ROOT, "profiles", "field_1080p_pmc.md")
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch " + re.escape(kernel) + r": (\d+)", open(path).read())
+    return int(m.group(1)) if m else None
+
+
 def timed(fn, steps, warmup, torch, lib, h, nst):
     """Mean ms per step over `steps` timed calls and the mean stage times."""
     for _ in range(warmup):
@@ -314,7 +326,8 @@ def main():
     achieved = flops / (leaf_ms * 1e-3) / 1e12
     roofline = {"bound": "mfma", "kernel": "k_leaf_mm (BF16)", "achieved": round(achieved, 3),
                 "peak": BF16_PEAK_TFS, "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFS, 6),
-                "traffic": None, "algorithmic_flops": flops, "avg_ms": round(float(leaf_ms), 4)}
+                "traffic": measured_traffic("k_leaf_mm<400, 5, false>"), "algorithmic_flops": flops,
+                "avg_ms": round(float(leaf_ms), 4)}
 
     cpu = None
     parity = None

@@ -179,3 +179,20 @@ def test_fp8_propagation_properties(gpu_available):
     hit = (knn[:, 0] == wi[:, 0]).mean()
     print(f"fp8 + propagation: exact propagated nearest found on {hit:.4f} of queries")
     assert hit >= 0.8
+
+
+def test_front_runs_the_learned_bf16_form(gpu_available):
+    """ANNField(leaf_distance="bf16", encoder=...) is the same call as the
+    device entry with the encoder's projection."""
+    from annfmp_amd import ANNField, synth, field_mm_on_device
+    from annfmp_amd.encoder import PatchEncoder
+    a, b = synth.make_pair(synth.small_spec(80, 100), seed=50)
+    p, K, leaf = 7, 4, 16
+    enc = PatchEncoder(p, 16, seed=3)
+    f = ANNField(n_neighbors=K, psize=p, dim_reduced=16, algorithm="hip", leaf_size=leaf, propagate=True,
+                 verbose=0, leaf_distance="bf16", encoder=enc)
+    nn = f.fit(a, b)
+    depth = O.tree_depth((b.shape[0] - p + 1) * (b.shape[1] - p + 1), leaf)
+    res = field_mm_on_device(a, b, p, None, None, depth, K, precision="bf16", projection=enc, propagate=True)
+    assert np.array_equal(nn, res["nn"])
+    assert f._model.nn_distances.dtype == np.float32
