@@ -1036,18 +1036,45 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
 // ---------------------------------------------------------------------------
 
 // kdtree.c:198-214 with only_leaf_idx = 1: right iff q[axis] - split >= 0.
+// The top kNatTop levels of the tree are staged in LDS and every query is
+// read once into LDS, so most of the dependent steps of the descent are LDS
+// accesses; the deeper levels read the tree from L2.
+constexpr int kNatTop = 9;
+
 template <int DP>
 __global__ __launch_bounds__(256) void k_natural(const float *__restrict__ q, int nq, const int *__restrict__ axis,
                                                  const float *__restrict__ split, const int *__restrict__ leaves,
                                                  int depth, int m, int *leaf, int *count, int *desc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int s_axis[(1 << kNatTop) - 1];
+    __shared__ float s_split[(1 << kNatTop) - 1];
+    __shared__ float sq[256 * (DP + 1)];
+    const int top = depth < kNatTop ? depth : kNatTop;
+    const int ntop = (1 << top) - 1;
+    for (int k = threadIdx.x; k < ntop; k += 256) {
+        s_axis[k] = axis[k];
+        s_split[k] = split[k];
+    }
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float *qs = sq + threadIdx.x * (DP + 1);
+    if (i < nq) {
+        float v[DP];
+        load_vec<DP>(q + (size_t)i * DP, v);
+#pragma unroll
+        for (int j = 0; j < DP; ++j) qs[j] = v[j];
+    }
+    __syncthreads();
     if (i >= nq) return;
     const float *qr = q + (size_t)i * DP;
     int idx = 0;
-    for (int lvl = 0; lvl < depth; ++lvl) {
-        float diff = qr[axis[idx]] - split[idx];
+    for (int lvl = 0; lvl < top; ++lvl) {
+        const float diff = qs[s_axis[idx]] - s_split[idx];
         idx = diff >= 0.0f ? 2 * idx + 2 : 2 * idx + 1;
     }
+    for (int lvl = top; lvl < depth; ++lvl) {
+        const float diff = qs[axis[idx]] - split[idx];
+        idx = diff >= 0.0f ? 2 * idx + 2 : 2 * idx + 1;
+    }
+    (void)qr;
     int l = idx - ((1 << depth) - 1);
     leaf[i] = l;
     if (desc) {

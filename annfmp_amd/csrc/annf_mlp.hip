@@ -271,14 +271,12 @@ __global__ __launch_bounds__(256) void k_encode(PatchGeom g, long long npatch, c
                         }
                     }
                     if (k == rw - 1) v &= tmask;
-                    const __bf16 x0 = (__bf16)in_scale((uint8_t)(v & 255u));
-                    const __bf16 x1 = (__bf16)in_scale((uint8_t)((v >> 8) & 255u));
-                    const __bf16 x2 = (__bf16)in_scale((uint8_t)((v >> 16) & 255u));
-                    const __bf16 x3 = (__bf16)in_scale((uint8_t)(v >> 24));
-                    const uint32_t lo = (uint32_t)__builtin_bit_cast(uint16_t, x0) |
-                                        ((uint32_t)__builtin_bit_cast(uint16_t, x1) << 16);
-                    const uint32_t hi = (uint32_t)__builtin_bit_cast(uint16_t, x2) |
-                                        ((uint32_t)__builtin_bit_cast(uint16_t, x3) << 16);
+                    // raw bytes (the packed weights carry the input scaling);
+                    // integers 0..255 are exact in BF16: the top half of the FP32
+                    const uint32_t lo = (__builtin_bit_cast(uint32_t, (float)((v >> 8) & 255u)) & 0xFFFF0000u) |
+                                        (__builtin_bit_cast(uint32_t, (float)(v & 255u)) >> 16);
+                    const uint32_t hi = (__builtin_bit_cast(uint32_t, (float)(v >> 24)) & 0xFFFF0000u) |
+                                        (__builtin_bit_cast(uint32_t, (float)((v >> 16) & 255u)) >> 16);
                     dst[2 * k] = lo;
                     dst[2 * k + 1] = hi;
                 }
@@ -355,18 +353,24 @@ __global__ void k_pack_encoder(const float *__restrict__ params, int p, int h1, 
     const float *pW3 = pb2 + h2, *pb3 = pW3 + (size_t)d * h2;
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < (long long)h1 * DK) {
+        // the input scaling (x - 128) / 128 moves into the weights: W1 / 128
+        // (exact in BF16) applied to the raw bytes, minus sum_k W1 in the bias
         const int j = (int)(i / DK), k = (int)(i % DK);
         const int r = k / rw4, u = k % rw4;
         float w = 0.0f;
         if (r < p && u < rowlen) w = pW1[(size_t)j * D + r * rowlen + u];
-        W1p[i] = (__bf16)w;
+        W1p[i] = (__bf16)((float)(__bf16)w * 0.0078125f);
     }
     if (i < (long long)h2 * h1) W2[i] = (__bf16)pW2[i];
     if (i < 32LL * h2) {
         const int j = (int)(i / h2), k = (int)(i % h2);
         W3p[i] = (__bf16)(j < d ? pW3[(size_t)j * h2 + k] : 0.0f);
     }
-    if (i < h1) b1[i] = pb1[i];
+    if (i < h1) {
+        double acc = 0.0;
+        for (int f = 0; f < D; ++f) acc += (double)(float)(__bf16)pW1[(size_t)i * D + f];
+        b1[i] = (float)((double)pb1[i] - acc);
+    }
     if (i < h2) b2[i] = pb2[i];
     if (i < 32) b3p[i] = i < d ? pb3[i] : 0.0f;
 }
