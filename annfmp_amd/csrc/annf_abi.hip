@@ -94,6 +94,7 @@ struct annf_ctx {
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
     DevBuf pca_scratch;
+    DevBuf sort_keys, sort_tmp;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // timing
@@ -130,7 +131,8 @@ struct annf_ctx {
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
-                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch};
+                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch,
+                          &sort_keys, &sort_tmp};
         for (DevBuf *b : list) b->release();
         has_tree = false;
         has_encoder = false;
@@ -712,11 +714,15 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
     RCHK(c->order.ensure(std::max((size_t)n_a, n_pairs) * sizeof(int)));
     RCHK(c->mm_keys1.ensure((size_t)n_a * K * sizeof(uint64_t)));
-    HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
-    HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), nullptr,
-                          c->stream));
-    HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
-                          c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), leafmm_chunk(), c->stream));
+    const size_t n_sort = std::max((size_t)n_a, n_pairs);
+    RCHK(c->sort_keys.ensure(2 * n_sort * sizeof(int)));
+    const size_t sort_bytes = sort_scratch_bytes((int)n_sort, nleaves);
+    RCHK(c->sort_tmp.ensure(sort_bytes));
+    int *skeys = c->sort_keys.as<int>();
+    HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, nullptr, c->stream));
+    HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafmm_chunk(), skeys, skeys + n_sort,
+                         c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(), c->bchunk.as<int>(),
+                         c->sort_tmp.p, sort_bytes, c->stream));
     c->mark(4);
     HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), n_a, nprobe,
                    shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), c->stream));
@@ -727,12 +733,11 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         RCHK(c->mm_pairleaf.ensure(n_pairs * sizeof(int)));
         RCHK(c->mm_keys2.ensure(n_pairs * K * sizeof(uint64_t)));
         keys2 = c->mm_keys2.as<uint64_t>();
-        HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
         HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, c->natleaf.as<int>(), t.pli, fs.nc_a,
-                          (int)fs.n_b, c->mm_pairleaf.as<int>(), c->bcount.as<int>(), c->stream));
-        HIPCHK(bucket_queries(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, c->bcount.as<int>(),
-                              c->boffset.as<int>(), c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(),
-                              leafmm_chunk(), c->stream));
+                          (int)fs.n_b, c->mm_pairleaf.as<int>(), nullptr, c->stream));
+        HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, leafmm_chunk(), skeys,
+                             skeys + n_sort, c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(),
+                             c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
         HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
         HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), (int)n_pairs,
                        1, shortlist, K, fp8, prop_k, keys2, c->stream));

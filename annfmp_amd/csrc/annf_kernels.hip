@@ -1145,15 +1145,19 @@ __global__ void k_chunk_count(const int *offset, int nleaves, int chunk, int *ch
 
 int leafscan_chunk() { return kScanChunk; }
 
+hipError_t chunk_offsets(const int *offset, int nleaves, int chunk, int *chunk_cnt, int *chunk_off, hipStream_t st) {
+    hipLaunchKernelGGL(k_chunk_count, dim3(blocks_for(nleaves, 256)), dim3(256), 0, st, offset, nleaves, chunk,
+                       chunk_cnt);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, chunk_cnt, nleaves, chunk_off, (int *)nullptr);
+    return hipGetLastError();
+}
+
 hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset, int *cursor, int *order,
                           int *chunk_off, int chunk, hipStream_t st) {
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, offset, cursor);
     if (nq > 0) hipLaunchKernelGGL(k_bucket, dim3(blocks_for(nq, 256)), dim3(256), 0, st, leaf, nq, cursor, order);
     // chunks per leaf (into count, no longer needed) and their offsets
-    hipLaunchKernelGGL(k_chunk_count, dim3(blocks_for(nleaves, 256)), dim3(256), 0, st, offset, nleaves, chunk,
-                       count);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, chunk_off, (int *)nullptr);
-    return hipGetLastError();
+    return chunk_offsets(offset, nleaves, chunk, count, chunk_off, st);
 }
 
 // Sorted k-list of runtime length K <= KM held in registers.  The K live

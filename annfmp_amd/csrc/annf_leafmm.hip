@@ -520,7 +520,7 @@ __global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K,
             if (leaf_of(kk) == leaf) leaf = -1;
     }
     pairleaf[t] = leaf;
-    if (leaf >= 0) wave_count1(count, leaf);
+    if (count && leaf >= 0) wave_count1(count, leaf);
 }
 
 __global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
