@@ -30,22 +30,47 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, out=OUT):
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-I", os.path.join(ROOT, "include"), "-I", SRC,
-           "-o", out + ".tmp"] + [os.path.join(SRC, s) for s in SOURCES]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
+COMPILE_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _report(res, what, verbose):
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libannf_hip.so")
+        raise RuntimeError(f"hipcc failed: {what}")
     if verbose and res.stderr.strip():
         # the host pass warns about the device-only target feature; keep other warnings visible
         lines = [ln for ln in res.stderr.splitlines() if "packed-fp32-ops" not in ln]
         if lines:
             sys.stderr.write("\n".join(lines) + "\n")
+
+
+def build(verbose=False, out=OUT, jobs=None):
+    """Compiles every source to an object in parallel (relocatable device
+    code is not needed: no kernel calls across files), then links."""
+    import tempfile
+    from concurrent.futures import ThreadPoolExecutor
+    cc = hipcc()
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", SRC]
+    jobs = jobs or min(len(SOURCES), os.cpu_count() or 1, 16)
+    with tempfile.TemporaryDirectory(prefix="annf_build_") as tmp:
+        objs = [os.path.join(tmp, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+
+        def compile_one(i):
+            cmd = [cc, f"--offload-arch={ARCH}"] + COMPILE_FLAGS + inc + [
+                "-c", os.path.join(SRC, SOURCES[i]), "-o", objs[i]]
+            if verbose:
+                print(" ".join(cmd), flush=True)
+            return subprocess.run(cmd, capture_output=True, text=True)
+
+        with ThreadPoolExecutor(jobs) as pool:
+            results = list(pool.map(compile_one, range(len(SOURCES))))
+        for src, res in zip(SOURCES, results):
+            _report(res, src, verbose)
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp"] + objs
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        _report(subprocess.run(cmd, capture_output=True, text=True), "link", verbose)
     os.replace(out + ".tmp", out)
     return out
 
