@@ -42,35 +42,6 @@ __device__ __forceinline__ long long pofs(const PatchGeom &g, long long i) {
     return (y * g.width + x) * 3;
 }
 
-// One patch row of `rw` dwords starting at byte `off`; the last dword is
-// masked by `tmask` (bytes past the row are zero).  fn(j, word).
-template <typename Fn>
-__device__ __forceinline__ void load_row(const PatchGeom &g, long long off, int rw, uint32_t tmask, Fn &&fn) {
-    const long long al = off & ~3LL;
-    const int sh = (int)(off & 3);
-    if (al + 4LL * (rw + 1) <= g.nbytes) {
-        const uint32_t *src = reinterpret_cast<const uint32_t *>(g.base + al);
-        uint32_t prev = src[0];
-        for (int j = 0; j < rw; ++j) {
-            const uint32_t next = src[j + 1];
-            uint32_t v = __builtin_amdgcn_alignbyte(next, prev, sh);
-            if (j == rw - 1) v &= tmask;
-            fn(j, v);
-            prev = next;
-        }
-    } else {
-        for (int j = 0; j < rw; ++j) {
-            uint32_t v = 0;
-            for (int k = 0; k < 4; ++k) {
-                const long long o = off + 4 * j + k;
-                if (o < g.nbytes) v |= (uint32_t)g.base[o] << (8 * k);
-            }
-            if (j == rw - 1) v &= tmask;
-            fn(j, v);
-        }
-    }
-}
-
 // x -> round-to-nearest E4M3 code of (x - 128) * scale, byte by byte; bytes
 // outside `mask` become code 0.
 __device__ __forceinline__ uint32_t fp8_word(uint32_t w, float scale, uint32_t mask) {
@@ -203,26 +174,73 @@ __device__ __forceinline__ void exact_d2_rows(const PatchGeom &ga, long long i, 
     for (int c = 0; c < NC; ++c) out[c] = saa + sbb[c] - 2u * sab[c];
 }
 
-// Stages patch idx into an LDS row (rows of rw dwords); the 4 lanes of a
-// quad take rows q4, q4+4, ...; dwords from rows*rw to dkw are zero.
-// Returns through ss / sum this lane's share of sum(x^2) and sum(x).
-__device__ __forceinline__ void stage_patch(const PatchGeom &g, long long idx, bool valid, uint32_t *row, int rw,
-                                            uint32_t tmask, int dkw, int q4, uint32_t &ss, uint32_t &sum) {
+// Patch staging in two steps so that every global load of a block is in
+// flight at once: fetch_patch issues the loads of this lane's patch rows
+// (q4, q4 + 4, ...; the 4 lanes of a quad share a patch) into registers,
+// store_patch aligns them into the LDS row (rows of rw dwords, the tail of
+// each row masked, dwords rows*rw..dkw zero) and returns this lane's share of
+// sum(x^2) and sum(x).  RL rows per lane and LW dwords per row (rw + 1, the
+// alignment slack) are compile-time bounds.
+template <int DK>
+struct StageShape {
+    static constexpr int RL = DK <= 176 ? 2 : 3;              // p <= 7 : p <= 12
+    static constexpr int LW = DK <= 176 ? 7 : (DK <= 256 ? 8 : 10);
+};
+
+template <int RL, int LW>
+__device__ __forceinline__ void fetch_patch(const PatchGeom &g, long long o, bool valid, int q4,
+                                            uint32_t (&w)[RL][LW]) {
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+        const int r = q4 + 4 * i;
+        const long long al = (o + (long long)r * g.rstride) & ~3LL;
+#pragma unroll
+        for (int k = 0; k < LW; ++k) w[i][k] = 0u;
+        if (valid && r < g.rows) {
+            if (al + 4LL * LW <= g.nbytes) {
+                const uint32_t *src = reinterpret_cast<const uint32_t *>(g.base + al);
+#pragma unroll
+                for (int k = 0; k < LW; ++k) w[i][k] = src[k];
+            } else {
+                // the last rows of the image: bytes past its end read as zero
+#pragma unroll
+                for (int k = 0; k < LW; ++k) {
+                    uint32_t v = 0;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const long long ob = al + 4 * k + b;
+                        if (ob < g.nbytes) v |= (uint32_t)g.base[ob] << (8 * b);
+                    }
+                    w[i][k] = v;
+                }
+            }
+        }
+    }
+}
+
+template <int RL, int LW>
+__device__ __forceinline__ void store_patch(const PatchGeom &g, long long o, const uint32_t (&w)[RL][LW],
+                                            uint32_t *row, int rw, uint32_t tmask, int dkw, int q4, uint32_t &ss,
+                                            uint32_t &sum) {
     ss = 0;
     sum = 0;
-    if (valid) {
-        const long long o = pofs(g, idx);
-        for (int r = q4; r < g.rows; r += 4) {
+#pragma unroll
+    for (int i = 0; i < RL; ++i) {
+        const int r = q4 + 4 * i;
+        if (r < g.rows) {
+            const int sh = (int)((o + (long long)r * g.rstride) & 3);
             uint32_t *dst = row + r * rw;
-            load_row(g, o + r * g.rstride, rw, tmask, [&](int k, uint32_t v) {
-                dst[k] = v;
-                ss = __builtin_amdgcn_udot4(v, v, ss, false);
-                sum = __builtin_amdgcn_udot4(v, 0x01010101u, sum, false);
-            });
+#pragma unroll
+            for (int k = 0; k < LW - 1; ++k) {
+                if (k < rw) {
+                    uint32_t v = __builtin_amdgcn_alignbyte(w[i][k + 1], w[i][k], sh);
+                    if (k == rw - 1) v &= tmask;
+                    dst[k] = v;
+                    ss = __builtin_amdgcn_udot4(v, v, ss, false);
+                    sum = __builtin_amdgcn_udot4(v, 0x01010101u, sum, false);
+                }
+            }
         }
-    } else {
-        for (int r = q4; r < g.rows; r += 4)
-            for (int k = 0; k < rw; ++k) row[r * rw + k] = 0;
     }
     for (int k = g.rows * rw + q4; k < dkw; k += 4) row[k] = 0;
 }
@@ -272,10 +290,11 @@ __device__ __forceinline__ float sqrt_f(uint32_t d2) {
 }
 
 template <int DK, int LM, bool FP8>
-__global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, PatchGeom gb,
+__global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(TreeView t, PatchGeom ga, PatchGeom gb,
                                                          const int *__restrict__ order,
                                                          const int *__restrict__ offset,
-                                                         const int *__restrict__ chunk_off, int nprobe, int K,
+                                                         const int *__restrict__ chunk_off,
+                                                         const int *__restrict__ blk_leaf, int nprobe, int K,
                                                          int slot_div, uint64_t *__restrict__ keys_out) {
     constexpr int RB = DK + 4;                              // LDS patch stride (bytes, odd dword count)
     constexpr int kBD = (kNP * RB > kMQ * kDS * 4) ? kNP * RB : kMQ * kDS * 4;
@@ -301,74 +320,111 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
     const int D = ga.rows * ga.rowlen;
     constexpr int dkw = DK / 4;
     const int b = blockIdx.x;
+    const int leaf = blk_leaf[b];   // blk_leaf holds gridDim.x entries, the used ones set
     if (b >= chunk_off[nleaves]) return;
-    int lo = 0, hi = nleaves - 1;
-    while (lo < hi) {
-        const int m = (lo + hi + 1) >> 1;
-        if (chunk_off[m] <= b) lo = m; else hi = m - 1;
-    }
-    const int leaf = lo;
     const int qb = offset[leaf] + (b - chunk_off[leaf]) * kMQ;
     const int nqb = min(kMQ, offset[leaf + 1] - qb);
     if (nqb <= 0) return;
 
-    // queries: one quad per query
-    {
-        const bool v = q < nqb;
-        const int slot = v ? order[qb + q] : 0;   // output slot; its A patch is slot / slot_div
-        const int qa = slot / slot_div;
-        uint32_t *row = reinterpret_cast<uint32_t *>(Aq + q * RB);
-        uint32_t ss, sm;
-        stage_patch(ga, qa, v, row, rw, tmask, dkw, q4, ss, sm);
-        ss = quad_sum(ss);
-        sm = quad_sum(sm);
-        if constexpr (FP8) {
-            const uint32_t mx = quad_max(maxabs_rows(row, ga.rows, rw, tmask, q4));
-            const float sc = mx ? 448.0f / (float)mx : 1.0f;
-            fp8_rows(row, ga.rows, rw, tmask, sc, q4);
-            if (q4 == 0) {
-                sa[q] = sc;
-                na[q] = ss - 256u * sm + 16384u * (uint32_t)D;   // sum (a - 128)^2
-            }
-        } else {
-            if (q4 == 0) na[q] = ss;
+    using SS = StageShape<DK>;
+    constexpr int RL = SS::RL, LW = SS::LW;
+    constexpr int kBP = kNP * 4 / kThreads;   // B patches per quad lane and tile
+
+    // Tiles of up to kNP points of the probed leaves, in order.  The first
+    // tile's indices go out before the query rows and its points' rows right
+    // after them, so a block waits on global memory about three times.
+    int pr = 0, t0 = 0;
+    int from = t.leaves[2 * leaf], cnt = t.leaves[2 * leaf + 1] - from;
+    auto seek = [&]() -> bool {   // the next non-empty tile at or after (pr, t0)
+        while (t0 >= cnt) {
+            if (++pr >= nprobe) return false;
+            const int pl = leaf ^ pr;
+            from = t.leaves[2 * pl];
+            cnt = t.leaves[2 * pl + 1] - from;
+            t0 = 0;
         }
-        if (q4 == 0) qid[q] = slot;
-    }
+        return true;
+    };
+    int np = 0, bj[kBP];
+    long long ob[kBP];
+    uint32_t rows_b[kBP][RL][LW];
+    auto tile_index = [&]() {
+        np = min(kNP, cnt - t0);
+#pragma unroll
+        for (int u = 0; u < kBP; ++u) {
+            const int i = (tid >> 2) + u * (kThreads / 4);
+            bj[u] = i < np ? t.orig[from + t0 + i] : 0;
+        }
+    };
+    auto tile_fetch = [&]() {
+#pragma unroll
+        for (int u = 0; u < kBP; ++u) {
+            ob[u] = pofs(gb, bj[u]);
+            fetch_patch(gb, ob[u], (tid >> 2) + u * (kThreads / 4) < np, q4, rows_b[u]);
+        }
+    };
+
+    // queries: one quad per query; stored with the first tile's points
+    const bool qv = q < nqb;
+    const int slot = qv ? order[qb + q] : 0;   // output slot; its A patch is slot / slot_div
+    bool more = seek();
+    if (more) tile_index();
+    const long long oa = pofs(ga, slot / slot_div);
+    uint32_t rows_a[RL][LW];
+    fetch_patch(ga, oa, qv, q4, rows_a);
+    if (q4 == 0) qid[q] = slot;
+    if (more) tile_fetch();
+    bool a_staged = false;
 
     // this lane's LM smallest keys (every slot live)
     uint64_t lst[LM];
     klist_init(lst, LM);
 
-    for (int pr = 0; pr < nprobe; ++pr) {
-        const int pleaf = leaf ^ pr;
-        const int from = t.leaves[2 * pleaf], cnt = t.leaves[2 * pleaf + 1] - from;
-        for (int t0 = 0; t0 < cnt; t0 += kNP) {
-            const int np = min(kNP, cnt - t0);
-            __syncthreads();   // the previous tile's distances are consumed
-            for (int e = tid; e < kNP * 4; e += kThreads) {
-                const int i = e >> 2;
-                const bool v = i < np;
-                const int bj = v ? t.orig[from + t0 + i] : 0;
-                uint32_t *row = reinterpret_cast<uint32_t *>(BD + i * RB);
-                uint32_t ss, sm;
-                stage_patch(gb, bj, v, row, rw, tmask, dkw, q4, ss, sm);
-                ss = quad_sum(ss);
-                sm = quad_sum(sm);
-                if constexpr (FP8) {
-                    const uint32_t mx = quad_max(maxabs_rows(row, gb.rows, rw, tmask, q4));
-                    const float sc = mx ? 448.0f / (float)mx : 1.0f;
-                    fp8_rows(row, gb.rows, rw, tmask, sc, q4);
-                    if (q4 == 0) {
-                        sb[i] = sc;
-                        nb[i] = ss - 256u * sm + 16384u * (uint32_t)D;
-                    }
-                } else {
-                    if (q4 == 0) nb[i] = ss;
+    while (more) {
+        __syncthreads();   // the previous tile's distances are consumed
+        if (!a_staged) {
+            a_staged = true;
+            uint32_t *row = reinterpret_cast<uint32_t *>(Aq + q * RB);
+            uint32_t ss, sm;
+            store_patch(ga, oa, rows_a, row, rw, tmask, dkw, q4, ss, sm);
+            ss = quad_sum(ss);
+            sm = quad_sum(sm);
+            if constexpr (FP8) {
+                const uint32_t mx = quad_max(maxabs_rows(row, ga.rows, rw, tmask, q4));
+                const float sc = mx ? 448.0f / (float)mx : 1.0f;
+                fp8_rows(row, ga.rows, rw, tmask, sc, q4);
+                if (q4 == 0) {
+                    sa[q] = sc;
+                    na[q] = ss - 256u * sm + 16384u * (uint32_t)D;   // sum (a - 128)^2
                 }
-                if (q4 == 0) borig[i] = bj;
+            } else {
+                if (q4 == 0) na[q] = ss;
             }
-            __syncthreads();
+        }
+        const int tnp = np;   // this tile's point count
+#pragma unroll
+        for (int u = 0; u < kBP; ++u) {
+            const int i = (tid >> 2) + u * (kThreads / 4);
+            uint32_t *row = reinterpret_cast<uint32_t *>(BD + i * RB);
+            uint32_t ss, sm;
+            store_patch(gb, ob[u], rows_b[u], row, rw, tmask, dkw, q4, ss, sm);
+            ss = quad_sum(ss);
+            sm = quad_sum(sm);
+            if constexpr (FP8) {
+                const uint32_t mx = quad_max(maxabs_rows(row, gb.rows, rw, tmask, q4));
+                const float sc = mx ? 448.0f / (float)mx : 1.0f;
+                fp8_rows(row, gb.rows, rw, tmask, sc, q4);
+                if (q4 == 0) {
+                    sb[i] = sc;
+                    nb[i] = ss - 256u * sm + 16384u * (uint32_t)D;
+                }
+            } else {
+                if (q4 == 0) nb[i] = ss;
+            }
+            if (q4 == 0) borig[i] = bj[u];
+        }
+        __syncthreads();
+        {
             // 64 x 32 block of dot products per wavefront
             // acc0 / acc1: query rows 0-31 / 32-63; BF16 splits each over
             // even (x0) and odd (x1) k-steps
@@ -405,7 +461,7 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
             const float sbp = sb[pcol];
             __syncthreads();   // every wavefront is done with the staged points
             {
-                const bool pv = pcol < np;
+                const bool pv = pcol < tnp;
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
                     const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
@@ -429,11 +485,18 @@ __global__ __launch_bounds__(kThreads) void k_leaf_mm(TreeView t, PatchGeom ga, 
             __syncthreads();
             if (q < nqb) {
                 const uint32_t *dr = Dm + q * kDS;
-                for (int p = q4; p < np; p += 4) {
+                for (int p = q4; p < tnp; p += 4) {
                     const uint64_t key = ((uint64_t)dr[p] << 32) | (uint32_t)borig[p];
                     if (key < lst[LM - 1]) klist_insert(lst, key);
                 }
             }
+        }
+        // the next tile's loads, in flight across the barrier
+        t0 += kNP;
+        more = seek();
+        if (more) {
+            tile_index();
+            tile_fetch();
         }
     }
     __syncthreads();   // the distance tile is free for the candidate lists
@@ -521,6 +584,13 @@ __global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K,
     }
     pairleaf[t] = leaf;
     if (count && leaf >= 0) wave_count1(count, leaf);
+}
+
+// blk_leaf[c] = l for the chunks c of leaf l (chunk_off[l] <= c < chunk_off[l + 1])
+__global__ void k_block_leaf(const int *__restrict__ chunk_off, int nleaves, int *__restrict__ blk_leaf) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaves) return;
+    for (int c = chunk_off[l], e = chunk_off[l + 1]; c < e; ++c) blk_leaf[c] = l;
 }
 
 __global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
@@ -616,9 +686,11 @@ hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, i
     return hipGetLastError();
 }
 
+size_t leafmm_blocks(int nq, int nleaves) { return (size_t)nleaves + ((size_t)nq + kMQ - 1) / kMQ; }
+
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
-                   const int *chunk_off, int nq, int nprobe, int shortlist, int K, int fp8, int slot_div,
-                   uint64_t *keys_out, hipStream_t st) {
+                   const int *chunk_off, int *blk_leaf, int nq, int nprobe, int shortlist, int K, int fp8,
+                   int slot_div, uint64_t *keys_out, hipStream_t st) {
     if (nq == 0) return hipSuccess;
     const int p = ga.rows;
     if (!leafmm_supported(p) || gb.rows != p || ga.rowlen != 3 * p || gb.rowlen != 3 * p) return hipErrorInvalidValue;
@@ -628,7 +700,9 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
     // up to an instantiated length (a longer list only adds candidates)
     const int L = fp8 ? shortlist : K;
     if (L < K || L > 32) return hipErrorInvalidValue;
-    const unsigned nblk = (unsigned)(nleaves + (nq + kMQ - 1) / kMQ);
+    const unsigned nblk = (unsigned)leafmm_blocks(nq, nleaves);
+    hipLaunchKernelGGL(k_block_leaf, dim3((unsigned)((nleaves + 255) / 256)), dim3(256), 0, st, chunk_off, nleaves,
+                       blk_leaf);
     return with_dk_lm(leafmm_dk(p), L, [&](auto dk, auto lm) -> hipError_t {
         constexpr int DK = decltype(dk)::value;
         constexpr int LM = decltype(lm)::value;
@@ -642,7 +716,7 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb, order,
-                               offset, chunk_off, nprobe, K, slot_div, keys_out);
+                               offset, chunk_off, blk_leaf, nprobe, K, slot_div, keys_out);
             return hipGetLastError();
         };
         return fp8 ? launch(std::true_type{}) : launch(std::false_type{});
