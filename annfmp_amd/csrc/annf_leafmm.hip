@@ -21,6 +21,7 @@
 // against tiles of 128 leaf points (one 32-column block per wavefront).
 // Patches are staged in LDS as bytes, rows padded to whole dwords.
 #include "annf_internal.h"
+#include "annf_patch.h"
 
 #include <type_traits>
 
@@ -37,11 +38,6 @@ constexpr int kThreads = 256;
 constexpr int kDS = kNP + 4;           // distance tile row stride (dwords)
 constexpr uint64_t kNoKey = ~0ull;
 
-__device__ __forceinline__ long long pofs(const PatchGeom &g, long long i) {
-    const long long y = i / g.ncols, x = i - y * g.ncols;
-    return (y * g.width + x) * 3;
-}
-
 // x -> round-to-nearest E4M3 code of (x - 128) * scale, byte by byte; bytes
 // outside `mask` become code 0.
 __device__ __forceinline__ uint32_t fp8_word(uint32_t w, float scale, uint32_t mask) {
@@ -54,21 +50,17 @@ __device__ __forceinline__ uint32_t fp8_word(uint32_t w, float scale, uint32_t m
     return (uint32_t)packed & mask;
 }
 
-__device__ __forceinline__ uint32_t bf16_pair(float lo, float hi) {
-    // the values are small integers: their BF16 is the top half of the FP32
-    return (__builtin_bit_cast(uint32_t, hi) & 0xFFFF0000u) | (__builtin_bit_cast(uint32_t, lo) >> 16);
-}
+typedef float floatx8 __attribute__((ext_vector_type(8)));
 
-// eight bytes (two dwords at p) as BF16
+// eight bytes (two dwords at p) as BF16 (exact: bytes are small integers;
+// the FP32 -> BF16 step is one packed conversion per pair)
 __device__ __forceinline__ bf16x8 frag_bf16(const uint8_t *p) {
     const uint32_t w0 = *reinterpret_cast<const uint32_t *>(p);
     const uint32_t w1 = *reinterpret_cast<const uint32_t *>(p + 4);
-    uint4 r;
-    r.x = bf16_pair((float)(w0 & 255u), (float)((w0 >> 8) & 255u));
-    r.y = bf16_pair((float)((w0 >> 16) & 255u), (float)(w0 >> 24));
-    r.z = bf16_pair((float)(w1 & 255u), (float)((w1 >> 8) & 255u));
-    r.w = bf16_pair((float)((w1 >> 16) & 255u), (float)(w1 >> 24));
-    return __builtin_bit_cast(bf16x8, r);
+    const floatx8 f = {(float)(w0 & 255u), (float)((w0 >> 8) & 255u), (float)((w0 >> 16) & 255u),
+                       (float)(w0 >> 24), (float)(w1 & 255u), (float)((w1 >> 8) & 255u),
+                       (float)((w1 >> 16) & 255u), (float)(w1 >> 24)};
+    return __builtin_convertvector(f, bf16x8);
 }
 
 __device__ __forceinline__ long frag_fp8(const uint8_t *p) {
@@ -150,7 +142,7 @@ __device__ __forceinline__ void exact_d2_rows(const PatchGeom &ga, long long i, 
                                               const uint32_t (&j)[NC], int nc, int rw, uint32_t tmask,
                                               uint32_t (&out)[NC]) {
     uint32_t saa = 0, sab[NC], sbb[NC];
-    const long long oa = pofs(ga, i);
+    const long long oa = patch_origin(ga, i);
 #pragma unroll
     for (int c = 0; c < NC; ++c) sab[c] = sbb[c] = 0u;
     for (int r = 0; r < ga.rows; ++r) {
@@ -162,7 +154,7 @@ __device__ __forceinline__ void exact_d2_rows(const PatchGeom &ga, long long i, 
         for (int c = 0; c < NC; ++c) {
             if (c >= nc) break;
             uint32_t wb[kMaxRW];
-            row_words(gb, pofs(gb, j[c]) + r * gb.rstride, rw, tmask, wb);
+            row_words(gb, patch_origin(gb, j[c]) + r * gb.rstride, rw, tmask, wb);
 #pragma unroll
             for (int k = 0; k < kMaxRW; ++k) {
                 sbb[c] = __builtin_amdgcn_udot4(wb[k], wb[k], sbb[c], false);
@@ -174,74 +166,20 @@ __device__ __forceinline__ void exact_d2_rows(const PatchGeom &ga, long long i, 
     for (int c = 0; c < NC; ++c) out[c] = saa + sbb[c] - 2u * sab[c];
 }
 
-// Patch staging in two steps so that every global load of a block is in
-// flight at once: fetch_patch issues the loads of this lane's patch rows
-// (q4, q4 + 4, ...; the 4 lanes of a quad share a patch) into registers,
-// store_patch aligns them into the LDS row (rows of rw dwords, the tail of
-// each row masked, dwords rows*rw..dkw zero) and returns this lane's share of
-// sum(x^2) and sum(x).  RL rows per lane and LW dwords per row (rw + 1, the
-// alignment slack) are compile-time bounds.
-template <int DK>
-struct StageShape {
-    static constexpr int RL = DK <= 176 ? 2 : 3;              // p <= 7 : p <= 12
-    static constexpr int LW = DK <= 176 ? 7 : (DK <= 256 ? 8 : 10);
-};
-
-template <int RL, int LW>
-__device__ __forceinline__ void fetch_patch(const PatchGeom &g, long long o, bool valid, int q4,
-                                            uint32_t (&w)[RL][LW]) {
-#pragma unroll
-    for (int i = 0; i < RL; ++i) {
-        const int r = q4 + 4 * i;
-        const long long al = (o + (long long)r * g.rstride) & ~3LL;
-#pragma unroll
-        for (int k = 0; k < LW; ++k) w[i][k] = 0u;
-        if (valid && r < g.rows) {
-            if (al + 4LL * LW <= g.nbytes) {
-                const uint32_t *src = reinterpret_cast<const uint32_t *>(g.base + al);
-#pragma unroll
-                for (int k = 0; k < LW; ++k) w[i][k] = src[k];
-            } else {
-                // the last rows of the image: bytes past its end read as zero
-#pragma unroll
-                for (int k = 0; k < LW; ++k) {
-                    uint32_t v = 0;
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) {
-                        const long long ob = al + 4 * k + b;
-                        if (ob < g.nbytes) v |= (uint32_t)g.base[ob] << (8 * b);
-                    }
-                    w[i][k] = v;
-                }
-            }
-        }
-    }
-}
-
+// Patch staging: fetch_rows / for_row_words, a quad of lanes per patch.
+// Writes the fetched rows into the LDS patch row (dwords rows*rw..dkw zero)
+// and returns this lane's share of sum(x^2) and sum(x).
 template <int RL, int LW>
 __device__ __forceinline__ void store_patch(const PatchGeom &g, long long o, const uint32_t (&w)[RL][LW],
                                             uint32_t *row, int rw, uint32_t tmask, int dkw, int q4, uint32_t &ss,
                                             uint32_t &sum) {
     ss = 0;
     sum = 0;
-#pragma unroll
-    for (int i = 0; i < RL; ++i) {
-        const int r = q4 + 4 * i;
-        if (r < g.rows) {
-            const int sh = (int)((o + (long long)r * g.rstride) & 3);
-            uint32_t *dst = row + r * rw;
-#pragma unroll
-            for (int k = 0; k < LW - 1; ++k) {
-                if (k < rw) {
-                    uint32_t v = __builtin_amdgcn_alignbyte(w[i][k + 1], w[i][k], sh);
-                    if (k == rw - 1) v &= tmask;
-                    dst[k] = v;
-                    ss = __builtin_amdgcn_udot4(v, v, ss, false);
-                    sum = __builtin_amdgcn_udot4(v, 0x01010101u, sum, false);
-                }
-            }
-        }
-    }
+    for_row_words(g, o, w, q4, 4, rw, tmask, [&](int r, int k, uint32_t v) {
+        row[r * rw + k] = v;
+        ss = __builtin_amdgcn_udot4(v, v, ss, false);
+        sum = __builtin_amdgcn_udot4(v, 0x01010101u, sum, false);
+    });
     for (int k = g.rows * rw + q4; k < dkw; k += 4) row[k] = 0;
 }
 
@@ -359,8 +297,8 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
     auto tile_fetch = [&]() {
 #pragma unroll
         for (int u = 0; u < kBP; ++u) {
-            ob[u] = pofs(gb, bj[u]);
-            fetch_patch(gb, ob[u], (tid >> 2) + u * (kThreads / 4) < np, q4, rows_b[u]);
+            ob[u] = patch_origin(gb, bj[u]);
+            fetch_rows(gb, ob[u], (tid >> 2) + u * (kThreads / 4) < np, q4, 4, rows_b[u]);
         }
     };
 
@@ -369,9 +307,9 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
     const int slot = qv ? order[qb + q] : 0;   // output slot; its A patch is slot / slot_div
     bool more = seek();
     if (more) tile_index();
-    const long long oa = pofs(ga, slot / slot_div);
+    const long long oa = patch_origin(ga, slot / slot_div);
     uint32_t rows_a[RL][LW];
-    fetch_patch(ga, oa, qv, q4, rows_a);
+    fetch_rows(ga, oa, qv, q4, 4, rows_a);
     if (q4 == 0) qid[q] = slot;
     if (more) tile_fetch();
     bool a_staged = false;

@@ -8,6 +8,7 @@
 // BF16 holds exactly.  Parameters are one FP32 buffer:
 //   W1 (h1 x D), b1 (h1), W2 (h2 x h1), b2 (h2), W3 (d x h2), b3 (d).
 #include "annf_internal.h"
+#include "annf_patch.h"
 
 namespace annf {
 
@@ -15,6 +16,8 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 
 constexpr float kG0 = 1.5957691216057308f;   // 2 sqrt(2 / pi)
 constexpr float kG1 = 0.07135481627f;        // 2 sqrt(2 / pi) * 0.044715
@@ -199,11 +202,6 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 constexpr int kEncTile = 64;    // patches per tile
 constexpr int kHS = 128 + 8;    // hidden tile row stride (bf16)
 
-__device__ __forceinline__ long long enc_pofs(const PatchGeom &g, long long i) {
-    const long long y = i / g.ncols, x = i - y * g.ncols;
-    return (y * g.width + x) * 3;
-}
-
 // Persistent blocks: each wavefront keeps its 32 columns of W1 and W2 (and
 // waves 0-1 all of W3) in registers as MFMA B fragments; per tile of 64
 // patches: stage the patches as BF16 in LDS (rows padded to whole dwords:
@@ -212,7 +210,7 @@ __device__ __forceinline__ long long enc_pofs(const PatchGeom &g, long long i) {
 // layer 2 likewise, layer 3 (64 x d) by waves 0-1, written as FP32 rows of
 // dpad floats (zero past d).
 template <int DK>
-__global__ __launch_bounds__(256) void k_encode(PatchGeom g, long long npatch, const __bf16 *__restrict__ W1p,
+__global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch, const __bf16 *__restrict__ W1p,
                                                 const float *__restrict__ b1, const __bf16 *__restrict__ W2,
                                                 const float *__restrict__ b2, const __bf16 *__restrict__ W3p,
                                                 const float *__restrict__ b3p, int d, int dpad,
@@ -245,50 +243,27 @@ __global__ __launch_bounds__(256) void k_encode(PatchGeom g, long long npatch, c
 
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long long p0 = tile * kEncTile;
-        // stage: one (patch, row) item per thread and step
-        for (int e = tid; e < kEncTile * g.rows; e += 256) {
-            const int i = e / g.rows, r = e - i * g.rows;
-            const long long pi = p0 + i;
-            uint32_t *dst = reinterpret_cast<uint32_t *>(X + (size_t)i * XS + r * rw * 4);
-            if (pi < npatch) {
-                const long long off = enc_pofs(g, pi) + r * g.rstride;
-                const long long al = off & ~3LL;
-                const int sh = (int)(off & 3);
-                const bool fast = al + 4LL * (rw + 1) <= g.nbytes;
-                const uint32_t *src = reinterpret_cast<const uint32_t *>(g.base + al);
-                uint32_t prev = fast ? src[0] : 0u;
-                for (int k = 0; k < rw; ++k) {
-                    uint32_t v;
-                    if (fast) {
-                        const uint32_t next = src[k + 1];
-                        v = __builtin_amdgcn_alignbyte(next, prev, sh);
-                        prev = next;
-                    } else {
-                        v = 0;
-                        for (int b = 0; b < 4; ++b) {
-                            const long long o = off + 4 * k + b;
-                            if (o < g.nbytes) v |= (uint32_t)g.base[o] << (8 * b);
-                        }
-                    }
-                    if (k == rw - 1) v &= tmask;
-                    // raw bytes (the packed weights carry the input scaling);
-                    // integers 0..255 are exact in BF16: the top half of the FP32
-                    const uint32_t lo = (__builtin_bit_cast(uint32_t, (float)((v >> 8) & 255u)) & 0xFFFF0000u) |
-                                        (__builtin_bit_cast(uint32_t, (float)(v & 255u)) >> 16);
-                    const uint32_t hi = (__builtin_bit_cast(uint32_t, (float)(v >> 24)) & 0xFFFF0000u) |
-                                        (__builtin_bit_cast(uint32_t, (float)((v >> 16) & 255u)) >> 16);
-                    dst[2 * k] = lo;
-                    dst[2 * k + 1] = hi;
-                }
-            } else {
-                for (int k = 0; k < 2 * rw; ++k) dst[k] = 0u;
-            }
-        }
-        // features past the padded rows, up to DK, are zero
-        for (int e = tid; e < kEncTile * (DK - used) / 2; e += 256) {
-            const int per = (DK - used) / 2;
-            const int i = e / per, k = e - i * per;
-            reinterpret_cast<uint32_t *>(X + (size_t)i * XS + used)[k] = 0u;
+        // stage: a quad of lanes per patch, every row load in flight at once
+        {
+            using SS = StageShape<DK>;
+            const int pl = tid >> 2, q4 = tid & 3;
+            const long long pi = p0 + pl;
+            const bool pv = pi < npatch;
+            const long long o = patch_origin(g, pv ? pi : 0);
+            uint32_t rows[SS::RL][SS::LW];
+            fetch_rows(g, o, pv, q4, 4, rows);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(X + (size_t)pl * XS);
+            // raw bytes (the packed weights carry the input scaling); integers
+            // 0..255 are exact in BF16
+            for_row_words(g, o, rows, q4, 4, rw, tmask, [&](int r, int k, uint32_t v) {
+                const floatx4 f = {(float)(v & 255u), (float)((v >> 8) & 255u), (float)((v >> 16) & 255u),
+                                   (float)(v >> 24)};
+                const uint2 h = __builtin_bit_cast(uint2, __builtin_convertvector(f, bf16x4));
+                dst[2 * (r * rw + k)] = h.x;
+                dst[2 * (r * rw + k) + 1] = h.y;
+            });
+            // features past the padded rows, up to DK, are zero
+            for (int k = used / 2 + q4; k < DK / 2; k += 4) dst[k] = 0u;
         }
         __syncthreads();
         // layer 1
