@@ -55,10 +55,12 @@ struct BuildScratch {
     int *eqoff;                       // n
     uint32_t *box, *box2;             // 2 * nodes_max * dpad * 2 each (this / next level)
     int *nax;                         // nodes_max
-    uint32_t *prefix;                 // nodes_max
-    int *rank;                        // nodes_max
-    uint32_t *hist;                   // nodes_max * 256
-    int2 *cnt;                        // total chunks of the widest level
+    uint32_t *prefix;                 // 3 * nodes_max (two radix prefixes, the split key)
+    int *rank;                        // 2 * nodes_max
+    uint32_t *hist;                   // 3 * hist_words (the three radix passes)
+    size_t hist_words;                // max over the levels of level_hist_words(nodes, cpn)
+    size_t nodes_max;
+    int2 *cnt, *cnt_off;              // total chunks of the widest level each
     int2 *tot;                        // nodes_max
     int *bnd;                         // all levels' bounds
     int *err;                         // 1 int, non-zero if a kernel saw an impossible state
@@ -85,6 +87,8 @@ hipError_t iota(int *dst, int n, hipStream_t st);
 // Builds the tree; `t` receives the device pointers of the result (inside
 // scratch ping-pong buffers and the given node/leaf arrays).
 // host_maxc[l]: the largest node population of level l.
+// words of one radix-pass histogram of a level (copies included)
+size_t level_hist_words(int nodes, int cpn);
 hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
                       const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,

@@ -3,9 +3,10 @@
 // The semantics follow the reference's openmp core (annfmp/openmp/kdtree/
 // src/*.c); the structure is GPU-first:
 //   * the tree is built level-synchronously: per-node data box (min/max
-//     reduction), first longest axis, radix select (4 x 8-bit LDS histograms)
-//     of the rank-c/2 value, and an order-exact partition computed from
-//     block scans (the closed form of helper.c:63-113, see k_scatter);
+//     reduction), first longest axis, radix select (11/11/10-bit LDS
+//     histograms) of the rank-c/2 value, and an order-exact partition
+//     computed from block scans (the closed form of helper.c:63-113, see
+//     k_lvl_eqoff); the last levels run one subtree per block in LDS;
 //   * the natural-leaf scan of every query is one pass over queries bucketed
 //     by leaf, the leaf's points staged in LDS;
 //   * row 0's exact search and the per-row propagation run one query per
@@ -277,15 +278,101 @@ __device__ __forceinline__ void chunk_range(const LevelArgs &L, int &node, int &
     e = min(s + kChunk, hi);
 }
 
-// Resets the level's per-node state; box (this level's nodes) and nbox (the
-// next level's, filled by k_move) when given.
-__global__ void k_level_init(LevelArgs L, int dpad, uint32_t *box, uint32_t *nbox, uint32_t *prefix, int *rank,
-                             uint32_t *hist) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (box && i < L.nodes * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
-    if (nbox && i < 2 * L.nodes * dpad) { nbox[2 * i] = 0xFFFFFFFFu; nbox[2 * i + 1] = 0u; }
-    if (i < L.nodes) { prefix[i] = 0u; rank[i] = (L.bnd[i + 1] - L.bnd[i]) / 2; }
-    if (i < L.nodes * 256) hist[i] = 0u;
+// Radix select of the rank-c/2 key in three passes of 11, 11 and 10 bits.
+// A node's histogram has hist_rep(cpn) copies (its blocks add into copy
+// chunk % R, which spreads the atomics of the wide top levels); after each
+// pass one block per node sums the copies and picks the digit.
+constexpr int kRBins = 2048;
+constexpr int kHistRep = 8;
+__host__ __device__ __forceinline__ int hist_rep(int cpn) { return cpn < kHistRep ? cpn : kHistRep; }
+__host__ __device__ constexpr int pass_shift(int p) { return p == 0 ? 21 : (p == 1 ? 10 : 0); }
+__host__ __device__ constexpr int pass_bins(int p) { return p == 2 ? 1024 : 2048; }
+
+size_t level_hist_words(int nodes, int cpn) { return (size_t)nodes * hist_rep(cpn) * kRBins; }
+
+// Level set-up where the level kernels do not cover it: zero the level's
+// first-pass histograms (the first level; later ones are cleared by the
+// previous level) and reset the node boxes k_box accumulates into.
+__global__ void k_lvl_prep(LevelArgs L, int dpad, uint32_t *box, uint32_t *H0, size_t hwords) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (box && i < (size_t)L.nodes * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
+    if (H0 && i < hwords) H0[i] = 0u;
+}
+
+// The digit of an NB-bin pass holding rank r of a node, and the number of
+// keys in lower digits, from the R copies of its histogram at h (stride
+// kRBins); every thread of the block gets the result.
+template <int NB>
+__device__ __forceinline__ void block_pick(const uint32_t *__restrict__ h, int R, uint32_t r, int2 *wsum,
+                                           uint32_t *sres, uint32_t &digit, uint32_t &before) {
+    constexpr int PER = NB / kBuildTPB;
+    static_assert(PER % 4 == 0, "bins are read as uint4");
+    const int tid = threadIdx.x;
+    uint32_t c[PER], tot = 0;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) c[j] = 0;
+    // every copy's loads are issued before the first use
+#pragma unroll
+    for (int rr = 0; rr < kHistRep; ++rr) {
+        if (rr < R) {
+            const uint4 *p = reinterpret_cast<const uint4 *>(h + (size_t)rr * kRBins + tid * PER);
+#pragma unroll
+            for (int q = 0; q < PER / 4; ++q) {
+                const uint4 v = p[q];
+                c[4 * q + 0] += v.x;
+                c[4 * q + 1] += v.y;
+                c[4 * q + 2] += v.z;
+                c[4 * q + 3] += v.w;
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) tot += c[j];
+    const uint32_t exc = (uint32_t)block_scan_int2(make_int2((int)tot, 0), wsum).x;
+    if (r >= exc && r < exc + tot) {
+        uint32_t b = exc;
+        int dg = tid * PER + PER - 1;
+        bool found = false;
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            if (!found && r < b + c[j]) {
+                dg = tid * PER + j;
+                found = true;
+            }
+            if (!found) b += c[j];
+        }
+        sres[0] = (uint32_t)dg;
+        sres[1] = b;
+    }
+    __syncthreads();
+    digit = sres[0];
+    before = sres[1];
+}
+
+// Flushes a block's LDS histogram into copy `rep` of the node's histogram.
+__device__ __forceinline__ void flush_hist(const uint32_t *h, uint32_t *g) {
+    for (int i = threadIdx.x; i < kRBins; i += kBuildTPB) {
+        const uint32_t c = h[i];
+        if (c) atomicAdd(&g[i], c);
+    }
+}
+
+// The pick of pass PASS for every node, one block per node: prefix and
+// remaining rank (PASS 2: the split key vkey, into pre_out).
+template <int PASS>
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_pick(LevelArgs L, const uint32_t *H, int R, const uint32_t *pre_in,
+                                                        const int *rank_in, uint32_t *pre_out, int *rank_out) {
+    __shared__ int2 wsum[kBuildTPB / kWave];
+    __shared__ uint32_t sres[2];
+    const int node = blockIdx.x;
+    const uint32_t pre = PASS == 0 ? 0u : pre_in[node];
+    const uint32_t r0 = PASS == 0 ? (uint32_t)((L.bnd[node + 1] - L.bnd[node]) / 2) : (uint32_t)rank_in[node];
+    uint32_t digit, before;
+    block_pick<pass_bins(PASS)>(H + (size_t)node * R * kRBins, R, r0, wsum, sres, digit, before);
+    if (threadIdx.x == 0) {
+        pre_out[node] = pre | (digit << pass_shift(PASS));
+        if (rank_out) rank_out[node] = (int)(r0 - before);
+    }
 }
 
 // Data box of each node (kdtree.c:379-394), as min/max of ordered keys.
@@ -334,128 +421,108 @@ __global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts
 
 // Split axis (kdtree.c:369-418): cyclic = depth % d; longest = first axis of
 // strictly largest FP32 width max - min, starting from 0.0.
-__global__ void k_axis(LevelArgs L, int d, int dpad, const uint32_t *box, int splitting_type, int *nax, int *axis_out) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= L.nodes) return;
+__device__ __forceinline__ int node_axis(const LevelArgs &L, int node, int d, int dpad, const uint32_t *box,
+                                         int splitting_type) {
+    if (splitting_type == 0) return L.lvl % d;
     int ax = 0;
-    if (splitting_type == 0) {
-        ax = L.lvl % d;
-    } else {
-        float longest = 0.0f;
-        for (int k = 0; k < d; ++k) {
-            float mn = key2f(box[((size_t)j * dpad + k) * 2 + 0]);
-            float mx = key2f(box[((size_t)j * dpad + k) * 2 + 1]);
-            float w = mx - mn;
-            if (w > longest) { longest = w; ax = k; }
-        }
+    float longest = 0.0f;
+    for (int k = 0; k < d; ++k) {
+        const float mn = key2f(box[((size_t)node * dpad + k) * 2 + 0]);
+        const float mx = key2f(box[((size_t)node * dpad + k) * 2 + 1]);
+        const float w = mx - mn;
+        if (w > longest) { longest = w; ax = k; }
     }
-    nax[j] = ax;
-    axis_out[L.node0 + j] = ax;
+    return ax;
 }
 
-// One 8-bit digit histogram pass of the radix select of the rank-c/2 key.
+// First pass of a level: the node's split axis, the split-axis keys and the
+// histogram of their top 11 bits.  Also clears this level's second- and
+// third-pass histograms and resets the next level's boxes (nbox, filled by
+// k_move) when given.
 template <int DP>
-__global__ __launch_bounds__(kBuildTPB) void k_hist(const float *__restrict__ pts, LevelArgs L, const int *nax,
-                                                    const uint32_t *prefix, int pass, uint32_t *keys, uint32_t *hist) {
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict__ pts, LevelArgs L, int d, int dpad,
+                                                         const uint32_t *box, int splitting_type, int *nax,
+                                                         int *axis_out, uint32_t *keys, uint32_t *H0, uint32_t *H1,
+                                                         uint32_t *H2, size_t hwords, uint32_t *nbox) {
+    __shared__ uint32_t h[kRBins];
+    __shared__ int sax;
+    {
+        const size_t gt = (size_t)blockIdx.x * kBuildTPB + threadIdx.x, gn = (size_t)gridDim.x * kBuildTPB;
+        for (size_t i = gt; i < hwords; i += gn) {
+            H1[i] = 0u;
+            H2[i] = 0u;
+        }
+        if (nbox)
+            for (size_t i = gt; i < (size_t)2 * L.nodes * dpad; i += gn) {
+                nbox[2 * i] = 0xFFFFFFFFu;
+                nbox[2 * i + 1] = 0u;
+            }
+    }
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
     if (s >= e) return;
-    __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0u;
-    __syncthreads();
-    const int ax = nax[node];
-    const uint32_t pre = prefix[node];
-    const int dsh = 24 - 8 * pass;
-    for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
-        uint32_t k;
-        if (pass == 0) {
-            k = f2key(pts[(size_t)i * DP + ax]);
-            keys[i] = k;
-        } else {
-            k = keys[i];
+    for (int i = threadIdx.x; i < kRBins; i += kBuildTPB) h[i] = 0u;
+    if (threadIdx.x == 0) {
+        const int ax = node_axis(L, node, d, dpad, box, splitting_type);
+        sax = ax;
+        if (s == lo) {
+            nax[node] = ax;
+            axis_out[L.node0 + node] = ax;
         }
-        bool m = (pass == 0) || (((k ^ pre) >> (dsh + 8)) == 0u);
-        if (m) atomicAdd(&h[(k >> dsh) & 255u], 1u);
     }
     __syncthreads();
-    uint32_t c = h[threadIdx.x];
-    if (c) atomicAdd(&hist[(size_t)node * 256 + threadIdx.x], c);
+    const int ax = sax;
+    for (int i = s + threadIdx.x; i < e; i += kBuildTPB) {
+        const uint32_t k = f2key(pts[(size_t)i * DP + ax]);
+        keys[i] = k;
+        atomicAdd(&h[k >> pass_shift(0)], 1u);
+    }
+    __syncthreads();
+    const int R = hist_rep(L.cpn), rep = (blockIdx.x - node * L.cpn) % R;
+    flush_hist(h, H0 + ((size_t)node * R + rep) * kRBins);
 }
 
-// Picks the digit holding the remaining rank; one wavefront per node.
-__global__ void k_pick(LevelArgs L, int pass, uint32_t *hist, uint32_t *prefix, int *rank) {
-    const int node = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
-    const int lane = threadIdx.x & (kWave - 1);
-    if (node >= L.nodes) return;
-    uint32_t *h = hist + (size_t)node * 256;
-    uint32_t c0 = h[lane * 4 + 0], c1 = h[lane * 4 + 1], c2 = h[lane * 4 + 2], c3 = h[lane * 4 + 3];
-    h[lane * 4 + 0] = 0u; h[lane * 4 + 1] = 0u; h[lane * 4 + 2] = 0u; h[lane * 4 + 3] = 0u;
-    const uint32_t tot = c0 + c1 + c2 + c3;
-    uint32_t inc = tot;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        uint32_t t = (uint32_t)__shfl_up((int)inc, o, kWave);
-        if (lane >= o) inc += t;
+// Pass PASS (1 or 2): the histogram of the next digit over the keys that
+// match the prefix picked so far.
+template <int PASS>
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_pass(LevelArgs L, const uint32_t *keys, uint32_t *Hc,
+                                                        const uint32_t *pre_in) {
+    __shared__ uint32_t h[kRBins];
+    int node, lo, hi, s, e;
+    chunk_range(L, node, lo, hi, s, e);
+    if (s >= e) return;
+    const int R = hist_rep(L.cpn), rep = (blockIdx.x - node * L.cpn) % R;
+    const uint32_t pre = pre_in[node];
+    for (int i = threadIdx.x; i < kRBins; i += kBuildTPB) h[i] = 0u;
+    __syncthreads();
+    for (int i = s + threadIdx.x; i < e; i += kBuildTPB) {
+        const uint32_t k = keys[i];
+        if (((k ^ pre) >> pass_shift(PASS - 1)) == 0u)
+            atomicAdd(&h[(k >> pass_shift(PASS)) & (uint32_t)(pass_bins(PASS) - 1)], 1u);
     }
-    const uint32_t exc = inc - tot;
-    const uint32_t r = (uint32_t)rank[node];
-    const bool mine = r >= exc && r < inc;
-    int digit = 0;
-    uint32_t before = exc;
-    if (mine) {
-        if (r < exc + c0) { digit = lane * 4 + 0; before = exc; }
-        else if (r < exc + c0 + c1) { digit = lane * 4 + 1; before = exc + c0; }
-        else if (r < exc + c0 + c1 + c2) { digit = lane * 4 + 2; before = exc + c0 + c1; }
-        else { digit = lane * 4 + 3; before = exc + c0 + c1 + c2; }
-    }
-    const unsigned long long b = __ballot(mine);
-    const int owner = __ffsll(b) - 1;
-    digit = readlane_i(digit, owner);
-    before = (uint32_t)readlane_i((int)before, owner);
-    if (lane == 0) {
-        prefix[node] |= (uint32_t)digit << (24 - 8 * pass);
-        rank[node] = (int)(r - before);
-    }
+    __syncthreads();
+    flush_hist(h, Hc + ((size_t)node * R + rep) * kRBins);
 }
 
 // Per-chunk counts of keys below / equal to the node's split key.
-__global__ __launch_bounds__(kBuildTPB) void k_count(LevelArgs L, const uint32_t *keys, const uint32_t *vkey, int2 *cnt) {
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_count(LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
+                                                         int2 *cnt) {
     __shared__ int2 wsum[kBuildTPB / kWave];
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
+    if (s >= e) {
+        if (threadIdx.x == 0) cnt[blockIdx.x] = make_int2(0, 0);
+        return;
+    }
     const uint32_t vk = vkey[node];
     int lt = 0, eq = 0;
-    for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
-        uint32_t k = keys[i];
+    for (int i = s + threadIdx.x; i < e; i += kBuildTPB) {
+        const uint32_t k = keys[i];
         lt += k < vk;
         eq += k == vk;
     }
-    int2 tot;
-    int2 ex = block_scan_int2(make_int2(lt, eq), wsum);
-    if (threadIdx.x == blockDim.x - 1) {
-        tot = make_int2(ex.x + lt, ex.y + eq);
-        cnt[blockIdx.x] = tot;
-    }
-}
-
-// Exclusive offsets of the chunk counts inside each node; one block per node.
-__global__ __launch_bounds__(kBuildTPB) void k_scan_nodes(LevelArgs L, int2 *cnt, int2 *tot) {
-    __shared__ int2 wsum[kBuildTPB / kWave];
-    const int node = blockIdx.x;
-    int2 *c = cnt + (size_t)node * L.cpn;
-    const int per = (L.cpn + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per;
-    int2 loc = make_int2(0, 0);
-    for (int k = 0; k < per && b + k < L.cpn; ++k) { loc.x += c[b + k].x; loc.y += c[b + k].y; }
-    int2 ex = block_scan_int2(loc, wsum);
-    int2 run = ex;
-    for (int k = 0; k < per && b + k < L.cpn; ++k) {
-        int2 v = c[b + k];
-        c[b + k] = run;
-        run.x += v.x;
-        run.y += v.y;
-    }
-    if (threadIdx.x == blockDim.x - 1) tot[node] = run;
+    const int2 ex = block_scan_int2(make_int2(lt, eq), wsum);
+    if (threadIdx.x == kBuildTPB - 1) cnt[blockIdx.x] = make_int2(ex.x + lt, ex.y + eq);
 }
 
 // Order-exact partition of one node segment (helper.c:63-113), in closed
@@ -465,76 +532,67 @@ __global__ __launch_bounds__(kBuildTPB) void k_scan_nodes(LevelArgs L, int2 *cnt
 //   x >  v : G-offset go = (g-1) - rank among '>=' elements; follow
 //            go -> eqoff[go] while go < E; slot = L + go
 // where eqoff[t] is the G-offset of the t-th equal element in G order.
-// WRITE = false records eqoff; WRITE = true moves the records and the split.
-template <int DP, bool WRITE>
-__global__ __launch_bounds__(kBuildTPB) void k_scatter(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
-                                                       LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
-                                                       const int2 *cnt_off, const int2 *tot, int *eqoff,
-                                                       float *__restrict__ pts_out, int *__restrict__ orig_out,
-                                                       const int *nax, float *split, int *err) {
+// This kernel sums the chunk counts into the block's offsets (cnt_off) and
+// the node totals (tot) and records eqoff; k_move moves the records.  It also
+// clears the next level's first-pass histograms (H0n, hwords_n words).
+template <int DP>
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_eqoff(LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
+                                                         const int2 *cnt, int2 *cnt_off, int2 *tot, int *eqoff,
+                                                         uint32_t *H0n, size_t hwords_n, int *err) {
     __shared__ int2 wsum[kBuildTPB / kWave];
+    {
+        const size_t gt = (size_t)blockIdx.x * kBuildTPB + threadIdx.x, gn = (size_t)gridDim.x * kBuildTPB;
+        for (size_t i = gt; i < hwords_n; i += gn) H0n[i] = 0u;
+    }
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
     if (s >= e) return;
+    const int ch = blockIdx.x - node * L.cpn;
+    int2 before = make_int2(0, 0), all = make_int2(0, 0);
+    for (int j = threadIdx.x; j < L.cpn; j += kBuildTPB) {
+        const int2 v = cnt[(size_t)node * L.cpn + j];
+        all.x += v.x;
+        all.y += v.y;
+        if (j < ch) { before.x += v.x; before.y += v.y; }
+    }
+    int2 sum_b, sum_a;
+    block_scan_int2_tot(before, wsum, sum_b);
+    block_scan_int2_tot(all, wsum, sum_a);
+    if (threadIdx.x == 0) {
+        cnt_off[blockIdx.x] = sum_b;
+        if (ch == 0) tot[node] = sum_a;
+    }
     const int c = hi - lo;
-    const int Lt = tot[node].x, E = tot[node].y, g = c - Lt;
+    const int Lt = sum_a.x, E = sum_a.y, g = c - Lt;
     const uint32_t vk = vkey[node];
     const int i0 = s + threadIdx.x * kBuildEPT;
     uint32_t k[kBuildEPT];
     int nlt = 0, neq = 0;
 #pragma unroll
     for (int u = 0; u < kBuildEPT; ++u) {
-        int i = i0 + u;
+        const int i = i0 + u;
         k[u] = (i < e) ? keys[i] : 0u;
         if (i < e) { nlt += k[u] < vk; neq += k[u] == vk; }
     }
-    int2 ex = block_scan_int2(make_int2(nlt, neq), wsum);
-    int r_lt = cnt_off[blockIdx.x].x + ex.x;
-    int r_eq = cnt_off[blockIdx.x].y + ex.y;
-    const int ax = WRITE ? nax[node] : 0;
+    const int2 ex = block_scan_int2(make_int2(nlt, neq), wsum);
+    int r_lt = sum_b.x + ex.x;
+    int r_eq = sum_b.y + ex.y;
 #pragma unroll
     for (int u = 0; u < kBuildEPT; ++u) {
-        int i = i0 + u;
+        const int i = i0 + u;
         if (i >= e) break;
-        const int off = i - lo;
-        int slot;
         if (k[u] < vk) {
-            slot = r_lt++;
-        } else {
-            const int go = (g - 1) - (off - r_lt);
-            if (k[u] == vk) {
-                const int t = (E - 1) - r_eq++;
-                if (!WRITE) {
-                    if (t < 0 || t >= c) { atomicOr(err, 2); continue; }
-                    eqoff[lo + t] = go;
-                }
-                slot = Lt + t;
-            } else {
-                int p = go;
-                if (WRITE) {
-                    // the chain strictly increases, so it ends within E steps
-                    for (int it = 0; p < E && it <= E; ++it) p = eqoff[lo + p];
-                }
-                slot = Lt + p;
-            }
-        }
-        if (WRITE) {
-            if (slot < 0 || slot >= c) {   // cannot happen for a correct partition
-                atomicOr(err, 1);
-                continue;
-            }
-            const int dst = lo + slot;
-            const float4 *src4 = reinterpret_cast<const float4 *>(pts_in + (size_t)i * DP);
-            float4 *dst4 = reinterpret_cast<float4 *>(pts_out + (size_t)dst * DP);
-#pragma unroll
-            for (int q = 0; q < DP / 4; ++q) dst4[q] = src4[q];
-            orig_out[dst] = orig_in[i];
-            if (slot == c / 2) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
+            ++r_lt;
+        } else if (k[u] == vk) {
+            const int go = (g - 1) - (i - lo - r_lt);
+            const int t = (E - 1) - r_eq++;
+            if (t < 0 || t >= c) { atomicOr(err, 2); continue; }
+            eqoff[lo + t] = go;
         }
     }
 }
 
-// The moving pass of the partition (the slots of k_scatter), coalesced: the
+// The moving pass of the partition (the slots of k_lvl_eqoff), coalesced: the
 // block first computes every point's slot (points s + u*256 + tid, so the
 // segment-order ranks come from eight block scans), then copies the records
 // one float4 per lane.  With nbox, the same copy accumulates the two
@@ -976,25 +1034,34 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
         uint32_t *nbox = (lvl & 1) ? s.box : s.box2;
         const bool need_kbox = splitting_type != 0 && (!fused_box || lvl == 0);
         if (!(fused_box && lvl + 1 < L0)) nbox = nullptr;
-        hipLaunchKernelGGL(k_level_init, dim3(blocks_for((long long)L.nodes * 256 * 2, 256)), dim3(256), 0, st, L,
-                           dpad, need_kbox ? box : (uint32_t *)nullptr, nbox, s.prefix, s.rank, s.hist);
+        const size_t hw = level_hist_words(L.nodes, L.cpn);
+        const size_t hw_next = lvl + 1 < L0 ? level_hist_words(2 * L.nodes, host_cpn[lvl + 1]) : 0;
+        uint32_t *H0 = s.hist, *H1 = s.hist + s.hist_words, *H2 = s.hist + 2 * s.hist_words;
+        uint32_t *P1 = s.prefix, *P2 = s.prefix + s.nodes_max, *vkey = s.prefix + 2 * s.nodes_max;
+        int *R1 = s.rank, *R2 = s.rank + s.nodes_max;
+        if (lvl == 0 || need_kbox) {
+            const size_t m = std::max((size_t)L.nodes * dpad, lvl == 0 ? hw : 0);
+            hipLaunchKernelGGL(k_lvl_prep, dim3(blocks_for((long long)m, 256)), dim3(256), 0, st, L, dpad,
+                               need_kbox ? box : (uint32_t *)nullptr, lvl == 0 ? H0 : (uint32_t *)nullptr, hw);
+        }
+        const int R = hist_rep(L.cpn);
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
             if (need_kbox)
                 hipLaunchKernelGGL(k_box<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, d, box);
-            hipLaunchKernelGGL(k_axis, dim3(blocks_for(L.nodes, 256)), dim3(256), 0, st, L, d, dpad, box,
-                               splitting_type, s.nax, axis);
-            for (int pass = 0; pass < 4; ++pass) {
-                hipLaunchKernelGGL(k_hist<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, s.nax, s.prefix, pass,
-                                   s.keys, s.hist);
-                hipLaunchKernelGGL(k_pick, dim3(blocks_for((long long)L.nodes * kWave, 256)), dim3(256), 0, st, L,
-                                   pass, s.hist, s.prefix, s.rank);
-            }
-            hipLaunchKernelGGL(k_count, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, s.prefix, s.cnt);
-            hipLaunchKernelGGL(k_scan_nodes, dim3(L.nodes), dim3(kBuildTPB), 0, st, L, s.cnt, s.tot);
-            hipLaunchKernelGGL((k_scatter<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys,
-                               s.prefix, s.cnt, s.tot, s.eqoff, pout, oout, s.nax, split, s.err);
-            hipLaunchKernelGGL(k_move<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, s.prefix, s.cnt,
+            const dim3 g(nblk), b(kBuildTPB), gn((unsigned)L.nodes);
+            hipLaunchKernelGGL(k_lvl_hist0<DP>, g, b, 0, st, pin, L, d, dpad, box, splitting_type, s.nax, axis,
+                               s.keys, H0, H1, H2, hw, nbox);
+            hipLaunchKernelGGL(k_lvl_pick<0>, gn, b, 0, st, L, H0, R, (const uint32_t *)nullptr,
+                               (const int *)nullptr, P1, R1);
+            hipLaunchKernelGGL(k_lvl_pass<1>, g, b, 0, st, L, s.keys, H1, P1);
+            hipLaunchKernelGGL(k_lvl_pick<1>, gn, b, 0, st, L, H1, R, P1, R1, P2, R2);
+            hipLaunchKernelGGL(k_lvl_pass<2>, g, b, 0, st, L, s.keys, H2, P2);
+            hipLaunchKernelGGL(k_lvl_pick<2>, gn, b, 0, st, L, H2, R, P2, R2, vkey, (int *)nullptr);
+            hipLaunchKernelGGL(k_lvl_count, g, b, 0, st, L, s.keys, vkey, s.cnt);
+            hipLaunchKernelGGL(k_lvl_eqoff<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, vkey, s.cnt,
+                               s.cnt_off, s.tot, s.eqoff, H0, hw_next, s.err);
+            hipLaunchKernelGGL(k_move<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey, s.cnt_off,
                                s.tot, s.eqoff, pout, oout, s.nax, split, s.err, nbox, d);
             return hipGetLastError();
         });
