@@ -725,14 +725,14 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     const size_t sort_bytes = sort_scratch_bytes((int)n_sort, nleaves);
     RCHK(c->sort_tmp.ensure(sort_bytes));
     int *skeys = c->sort_keys.as<int>();
-    RCHK(c->mm_blkleaf.ensure(leafmm_blocks((int)n_sort, nleaves) * sizeof(int)));
-    int *blk_leaf = c->mm_blkleaf.as<int>();
+    RCHK(c->mm_blkleaf.ensure(leafmm_scratch_bytes((int)n_sort, nleaves)));
+    void *blk_info = c->mm_blkleaf.p;
     HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, nullptr, c->stream));
     HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafmm_chunk(), skeys, skeys + n_sort,
                          c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(), c->bchunk.as<int>(),
                          c->sort_tmp.p, sort_bytes, c->stream));
     c->mark(4);
-    HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_leaf, n_a,
+    HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, n_a,
                    nprobe, shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), c->stream));
     c->mark(5);
     uint64_t *keys2 = nullptr;
@@ -747,7 +747,7 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                              skeys + n_sort, c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(),
                              c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
         HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
-        HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_leaf,
+        HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info,
                        (int)n_pairs, 1, shortlist, K, fp8, prop_k, keys2, c->stream));
     }
     HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,

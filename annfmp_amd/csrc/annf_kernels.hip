@@ -753,7 +753,10 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
                                                      float *__restrict__ split_out, int *err) {
     constexpr int SM = sub_max<DP>();
     constexpr int EPT = (SM + kSubTPB - 1) / kSubTPB;   // points per thread in the partition scan
-    __shared__ float4 data[SM * DP / 4];                // the node's points, level-L0 order
+    // the node's points, level-L0 order; dimension j of point p is stored at
+    // p * DP + (j + p) % DP, so the lanes reading one dimension of different
+    // points spread over the LDS banks
+    __shared__ float dataf[SM * DP];
     __shared__ uint16_t perm[2][SM];
     __shared__ uint32_t keys[SM];
     __shared__ int hist_eq[kSubMaxAll];                 // radix histograms, then eqoff
@@ -764,14 +767,21 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
     __shared__ int srank[kSubNodes];
     __shared__ int PL[kSubNodes + 1], PE[kSubNodes + 1];
     __shared__ int wlt[kSubTPB / kWave], weq[kSubTPB / kWave];
-    const float *dataf = reinterpret_cast<const float *>(data);
+    auto at = [](int p, int j) { return p * DP + (j + p) % DP; };
 
     const int tid = threadIdx.x, lane = tid & (kWave - 1), wid = tid / kWave;
     const int node = blockIdx.x;
     const int lo = bnd0[node], c0 = bnd0[node + 1] - lo;
     {
         const float4 *src = reinterpret_cast<const float4 *>(pts_in + (size_t)lo * DP);
-        for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) data[e] = src[e];
+        for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) {
+            const int i = e / (DP / 4), j = 4 * (e - i * (DP / 4));
+            const float4 v = src[e];
+            dataf[at(i, j + 0)] = v.x;
+            dataf[at(i, j + 1)] = v.y;
+            dataf[at(i, j + 2)] = v.z;
+            dataf[at(i, j + 3)] = v.w;
+        }
     }
     for (int i = tid; i < c0; i += kSubTPB) perm[0][i] = (uint16_t)i;
     if (tid == 0) {
@@ -802,7 +812,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
                 s2 = pair / d;
                 j = pair - s2 * d;
                 for (int i = sb[s2] + lip; i < sb[s2 + 1]; i += T) {
-                    const uint32_t k = f2key(dataf[(size_t)perm[cur][i] * DP + j]);
+                    const uint32_t k = f2key(dataf[at(perm[cur][i], j)]);
                     mn = min(mn, k);
                     mx = max(mx, k);
                 }
@@ -837,7 +847,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         __syncthreads();
         // (C) split-axis keys
         for (int i = tid; i < c0; i += kSubTPB)
-            keys[i] = f2key(dataf[(size_t)perm[cur][i] * DP + sax[sub_of(sb, nsub, i)]]);
+            keys[i] = f2key(dataf[at(perm[cur][i], sax[sub_of(sb, nsub, i)])]);
         // (D) radix select of the rank-c/2 key of every sub-node
         for (int pass = 0; pass < 4; ++pass) {
             for (int i = tid; i < nsub * 256; i += kSubTPB) hist_eq[i] = 0;
@@ -963,7 +973,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
             if (slot < 0 || slot >= c) { atomicOr(err, 1); continue; }
             const int pidx = perm[cur][i];
             perm[cur ^ 1][sb[s2] + slot] = (uint16_t)pidx;
-            if (slot == c / 2) split_out[heap0 + s2] = dataf[(size_t)pidx * DP + sax[s2]];
+            if (slot == c / 2) split_out[heap0 + s2] = dataf[at(pidx, sax[s2])];
         }
         __syncthreads();
         // children bounds for the next level: the left child gets c/2 points
@@ -986,8 +996,9 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
     }
     float4 *dst = reinterpret_cast<float4 *>(pts_out + (size_t)lo * DP);
     for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) {
-        const int i = e / (DP / 4), c = e - i * (DP / 4);
-        dst[e] = data[(size_t)perm[cur][i] * (DP / 4) + c];
+        const int i = e / (DP / 4), j = 4 * (e - i * (DP / 4));
+        const int pi = perm[cur][i];
+        dst[e] = make_float4(dataf[at(pi, j + 0)], dataf[at(pi, j + 1)], dataf[at(pi, j + 2)], dataf[at(pi, j + 3)]);
     }
     for (int i = tid; i < c0; i += kSubTPB) orig_out[lo + i] = orig_in[lo + perm[cur][i]];
 }

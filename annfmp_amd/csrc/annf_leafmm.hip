@@ -230,9 +230,8 @@ __device__ __forceinline__ float sqrt_f(uint32_t d2) {
 template <int DK, int LM, bool FP8>
 __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(TreeView t, PatchGeom ga, PatchGeom gb,
                                                          const int *__restrict__ order,
-                                                         const int *__restrict__ offset,
                                                          const int *__restrict__ chunk_off,
-                                                         const int *__restrict__ blk_leaf, int nprobe, int K,
+                                                         const int4 *__restrict__ blk_info, int nprobe, int K,
                                                          int slot_div, uint64_t *__restrict__ keys_out) {
     constexpr int RB = DK + 4;                              // LDS patch stride (bytes, odd dword count)
     constexpr int kBD = (kNP * RB > kMQ * kDS * 4) ? kNP * RB : kMQ * kDS * 4;
@@ -258,10 +257,10 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
     const int D = ga.rows * ga.rowlen;
     constexpr int dkw = DK / 4;
     const int b = blockIdx.x;
-    const int leaf = blk_leaf[b];   // blk_leaf holds gridDim.x entries, the used ones set
+    // this block's chunk (blk_info holds gridDim.x pairs, the used ones set)
+    const int4 bi0 = blk_info[2 * b], bi1 = blk_info[2 * b + 1];
     if (b >= chunk_off[nleaves]) return;
-    const int qb = offset[leaf] + (b - chunk_off[leaf]) * kMQ;
-    const int nqb = min(kMQ, offset[leaf + 1] - qb);
+    const int leaf = bi0.x, qb = bi0.y, nqb = bi0.z;
     if (nqb <= 0) return;
 
     using SS = StageShape<DK>;
@@ -272,7 +271,7 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
     // tile's indices go out before the query rows and its points' rows right
     // after them, so a block waits on global memory about three times.
     int pr = 0, t0 = 0;
-    int from = t.leaves[2 * leaf], cnt = t.leaves[2 * leaf + 1] - from;
+    int from = bi0.w, cnt = bi1.x;
     auto seek = [&]() -> bool {   // the next non-empty tile at or after (pr, t0)
         while (t0 >= cnt) {
             if (++pr >= nprobe) return false;
@@ -524,11 +523,18 @@ __global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K,
     if (count && leaf >= 0) wave_count1(count, leaf);
 }
 
-// blk_leaf[c] = l for the chunks c of leaf l (chunk_off[l] <= c < chunk_off[l + 1])
-__global__ void k_block_leaf(const int *__restrict__ chunk_off, int nleaves, int *__restrict__ blk_leaf) {
+// The chunks c of leaf l (chunk_off[l] <= c < chunk_off[l + 1]): blk_info[2c]
+// = (leaf, first query slot in `order`, queries, first point of the leaf),
+// blk_info[2c + 1].x = points of the leaf; one read gives a block its work.
+__global__ void k_block_info(const int *__restrict__ chunk_off, const int *__restrict__ offset,
+                             const int *__restrict__ leaves, int nleaves, int4 *__restrict__ blk_info) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nleaves) return;
-    for (int c = chunk_off[l], e = chunk_off[l + 1]; c < e; ++c) blk_leaf[c] = l;
+    const int q0 = offset[l], q1 = offset[l + 1], from = leaves[2 * l], cnt = leaves[2 * l + 1] - from;
+    for (int c = chunk_off[l], e = chunk_off[l + 1], qb = q0; c < e; ++c, qb += kMQ) {
+        blk_info[2 * c] = make_int4(l, qb, min(kMQ, q1 - qb), from);
+        blk_info[2 * c + 1] = make_int4(cnt, 0, 0, 0);
+    }
 }
 
 __global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
@@ -624,10 +630,12 @@ hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, i
     return hipGetLastError();
 }
 
-size_t leafmm_blocks(int nq, int nleaves) { return (size_t)nleaves + ((size_t)nq + kMQ - 1) / kMQ; }
+static size_t leafmm_blocks(int nq, int nleaves) { return (size_t)nleaves + ((size_t)nq + kMQ - 1) / kMQ; }
+
+size_t leafmm_scratch_bytes(int nq, int nleaves) { return leafmm_blocks(nq, nleaves) * 2 * sizeof(int4); }
 
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
-                   const int *chunk_off, int *blk_leaf, int nq, int nprobe, int shortlist, int K, int fp8,
+                   const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K, int fp8,
                    int slot_div, uint64_t *keys_out, hipStream_t st) {
     if (nq == 0) return hipSuccess;
     const int p = ga.rows;
@@ -639,8 +647,9 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
     const int L = fp8 ? shortlist : K;
     if (L < K || L > 32) return hipErrorInvalidValue;
     const unsigned nblk = (unsigned)leafmm_blocks(nq, nleaves);
-    hipLaunchKernelGGL(k_block_leaf, dim3((unsigned)((nleaves + 255) / 256)), dim3(256), 0, st, chunk_off, nleaves,
-                       blk_leaf);
+    int4 *blk_info = static_cast<int4 *>(scratch);
+    hipLaunchKernelGGL(k_block_info, dim3((unsigned)((nleaves + 255) / 256)), dim3(256), 0, st, chunk_off, offset,
+                       t.leaves, nleaves, blk_info);
     return with_dk_lm(leafmm_dk(p), L, [&](auto dk, auto lm) -> hipError_t {
         constexpr int DK = decltype(dk)::value;
         constexpr int LM = decltype(lm)::value;
@@ -654,7 +663,7 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb, order,
-                               offset, chunk_off, blk_leaf, nprobe, K, slot_div, keys_out);
+                               chunk_off, blk_info, nprobe, K, slot_div, keys_out);
             return hipGetLastError();
         };
         return fp8 ? launch(std::true_type{}) : launch(std::false_type{});
