@@ -73,6 +73,9 @@ constexpr int kStages = 7;
 struct annf_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // the query image's projection runs here, beside the tree build on `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // init_extern parameters (global.h:18-27)
     bool inited = false;
     int n_neighbors = 10, tree_depth = 0, max_leaves = 2147483640, num_threads = 1;
@@ -346,13 +349,20 @@ int annf_create(annf_ctx **out, int device) {
     HIPCHK(hipSetDevice(device));
     annf_ctx *c = new annf_ctx();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    // both streams at the default priority
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess) {
+        annf_destroy(c);
         return fail(ANNF_EHIP, "hipStreamCreateWithFlags failed");
+    }
+    if (hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+        annf_destroy(c);
+        return fail(ANNF_EHIP, "hipEventCreate failed");
     }
     for (int i = 0; i <= kStages; ++i) {
         if (hipEventCreate(&c->ev[i]) != hipSuccess) {
-            delete c;
+            annf_destroy(c);
             return fail(ANNF_EHIP, "hipEventCreate failed");
         }
     }
@@ -363,10 +373,14 @@ int annf_create(annf_ctx **out, int device) {
 int annf_destroy(annf_ctx *c) {
     if (!c) return ANNF_OK;
     (void)hipSetDevice(c->device);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     c->release_all();
     for (int i = 0; i <= kStages; ++i)
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+    if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+    if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return ANNF_OK;
@@ -623,31 +637,48 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                            c->mean64.as<double>());
         HIPCHK(hipGetLastError());
     }
-    auto project = [&](const uint8_t *img, int h, int w, const PatchGeom &g, long long n, float *out) -> int {
+    auto project = [&](const uint8_t *img, int h, int w, const PatchGeom &g, long long n, float *out,
+                       hipStream_t st, int per_cu) -> int {
         if (enc) {
             HIPCHK(encode_patches(g, n, c->enc_w1.as<__bf16>(), c->enc_b1.as<float>(), c->enc_w2.as<__bf16>(),
                                   c->enc_b2.as<float>(), c->enc_w3.as<__bf16>(), c->enc_b3.as<float>(), d, dpad, out,
-                                  c->stream));
+                                  st, per_cu));
         } else {
             HIPCHK(project_patches(img, h, w, p, c->basis32.as<float>(), c->mean32.as<float>(),
-                                   c->basis64.as<double>(), c->mean64.as<double>(), d, dpad, projection, out,
-                                   c->stream));
+                                   c->basis64.as<double>(), c->mean64.as<double>(), d, dpad, projection, out, st));
         }
         return ANNF_OK;
     };
     RCHK(c->qbuf.ensure((size_t)n_a * dpad * sizeof(float)));
-    RCHK(project(img_a, ha, wa, ga, n_a, c->qbuf.as<float>()));
-    c->mark(1);
     c->has_tree = false;
     RCHK(c->pts_a.ensure((size_t)n_b * dpad * sizeof(float)));
-    RCHK(project(img_b, hb, wb, gb, n_b, c->pts_a.as<float>()));
+    // B is projected on `stream`; then A (the queries) is projected on the
+    // side stream while `stream` builds B's tree, and `stream` waits for A
+    // before anything reads it
+    RCHK(project(img_b, hb, wb, gb, n_b, c->pts_a.as<float>(), c->stream, 2));
     if (red_b_out) {
         hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n_b * d + 255) / 256)), dim3(256), 0, c->stream,
                            c->pts_a.as<float>(), n_b, d, dpad, red_b_out);
         HIPCHK(hipGetLastError());
     }
+    c->mark(1);
+    HIPCHK(hipEventRecord(c->fork_ev, c->stream));
+    HIPCHK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
+    // one encoder block per CU: the build's short kernels keep room beside it
+    int rc = project(img_a, ha, wa, ga, n_a, c->qbuf.as<float>(), c->side, 1);
+    if (rc == ANNF_OK && hipEventRecord(c->join_ev, c->side) != hipSuccess) rc = fail(ANNF_EHIP, "hipEventRecord failed");
+    if (rc != ANNF_OK) {
+        (void)hipStreamSynchronize(c->side);
+        return rc;
+    }
     c->mark(2);
-    RCHK(build_from_padded(c, (int)n_b, d, dpad, tree_depth));
+    rc = build_from_padded(c, (int)n_b, d, dpad, tree_depth);
+    if (hipStreamWaitEvent(c->stream, c->join_ev, 0) != hipSuccess && rc == ANNF_OK)
+        rc = fail(ANNF_EHIP, "hipStreamWaitEvent failed");
+    if (rc != ANNF_OK) {
+        (void)hipStreamSynchronize(c->side);
+        return rc;
+    }
     if (red_a_out) {
         hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n_a * d + 255) / 256)), dim3(256), 0, c->stream,
                            c->qbuf.as<float>(), n_a, d, dpad, red_a_out);

@@ -167,9 +167,11 @@ int encoder_dk(int p);
 bool encoder_supported(int p, int h1, int h2, int d);
 hipError_t pack_encoder(const float *params, int p, int h1, int h2, int d, __bf16 *W1p, float *b1, __bf16 *W2,
                         float *b2, __bf16 *W3p, float *b3p, hipStream_t st);
+// per_cu: persistent blocks per CU (2 fill the GPU; 1 leaves room for
+// kernels of another stream)
 hipError_t encode_patches(const PatchGeom &g, long long npatch, const __bf16 *W1p, const float *b1,
                           const __bf16 *W2, const float *b2, const __bf16 *W3p, const float *b3p, int d, int dpad,
-                          float *out, hipStream_t st);
+                          float *out, hipStream_t st, int per_cu = 2);
 
 // exact field (annf_eval.hip)
 hipError_t patch_matrix(const uint8_t *img, int h, int w, int p, uint8_t *out, int dpad_b, hipStream_t st);

@@ -443,13 +443,14 @@ hipError_t pack_encoder(const float *params, int p, int h1, int h2, int d, __bf1
 
 hipError_t encode_patches(const PatchGeom &g, long long npatch, const __bf16 *W1p, const float *b1,
                           const __bf16 *W2, const float *b2, const __bf16 *W3p, const float *b3p, int d, int dpad,
-                          float *out, hipStream_t st) {
+                          float *out, hipStream_t st, int per_cu) {
     if (npatch == 0) return hipSuccess;
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const long long ntiles = (npatch + kEncTile - 1) / kEncTile;
-    const unsigned grid = (unsigned)(ntiles < 2LL * ncu ? ntiles : 2LL * ncu);
+    const long long cap = (long long)(per_cu > 0 ? per_cu : 2) * ncu;
+    const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
     auto launch = [&](auto dk) -> hipError_t {
         constexpr int DK = decltype(dk)::value;
         const size_t lds = ((size_t)kEncTile * (DK + 8) + (size_t)kEncTile * kHS) * sizeof(__bf16);

@@ -38,9 +38,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, 8 TB/s
 BF16_PEAK_TFS = 2500.0         # MI355X dense BF16 MFMA, 2.5 PFLOP/s
-PLAIN_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_scan", "row0_exact", "row_sweep",
-                "best_of_k"]
-MM_STAGES = ["project_a", "project_b", "tree_build", "natural_leaf_bucket", "leaf_knn_mfma", "propagated_pass"]
+# stage boundaries on the main stream: B is projected first; A is projected on
+# a second stream during the tree build, whose stage ends once A is done too
+PLAIN_STAGES = ["project_b", "launch_project_a", "tree_build_with_project_a", "natural_leaf_scan", "row0_exact",
+                "row_sweep", "best_of_k"]
+MM_STAGES = ["project_b", "launch_project_a", "tree_build_with_project_a", "natural_leaf_bucket", "leaf_knn_mfma",
+             "propagated_pass"]
 DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz")
 DEFAULT_ENCODER_4K = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p9_d16.npz")
 FP8_PEAK_TFS = 5000.0          # MI355X dense FP8 MFMA, 5.0 PFLOP/s
