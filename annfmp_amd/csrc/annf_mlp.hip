@@ -36,6 +36,14 @@ __device__ __forceinline__ float gelu_f(float x) {
     return x * sigmoid_f(t);
 }
 
+// The same GELU for the fused inference kernel, with fused multiply-adds
+// and the exp2 scale folded into the polynomial.
+__device__ __forceinline__ float gelu_fast(float x) {
+    constexpr float kL = 1.4426950408889634f;
+    const float t = x * __builtin_fmaf(-kG1 * kL, x * x, -kG0 * kL);
+    return x * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(t));
+}
+
 __device__ __forceinline__ float gelu_grad(float x) {
     const float t = x * (kG0 + kG1 * (x * x));
     const float s = sigmoid_f(t);
@@ -202,13 +210,14 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 constexpr int kEncTile = 64;    // patches per tile
 constexpr int kHS = 128 + 8;    // hidden tile row stride (bf16)
 
-// Persistent blocks: each wavefront keeps its 32 columns of W1 and W2 (and
-// waves 0-1 all of W3) in registers as MFMA B fragments; per tile of 64
+// Persistent blocks: each wavefront keeps its 32 rows of W1 and W2 (and
+// waves 0-1 all of W3) in registers as MFMA A fragments; per tile of 64
 // patches: stage the patches as BF16 in LDS (rows padded to whole dwords:
 // W1 is stored in the same padded feature order, zero at the pads), layer 1
-// (64 x 128, wave w owns hidden columns 32w..32w+31) + GELU into LDS,
-// layer 2 likewise, layer 3 (64 x d) by waves 0-1, written as FP32 rows of
-// dpad floats (zero past d).
+// as W1 . X^T (wave w owns hidden units 32w..32w+31, so each lane holds four
+// consecutive units of one patch and writes them as one 8-byte LDS store) +
+// GELU, layer 2 likewise, layer 3 (d x 64) by waves 0-1, written as FP32 rows
+// of dpad floats (zero past d), four floats per store.
 template <int DK>
 __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch, const __bf16 *__restrict__ W1p,
                                                 const float *__restrict__ b1, const __bf16 *__restrict__ W2,
@@ -224,6 +233,7 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
     const int lc = lane & 31, lh = lane >> 5;
     const int hcol = 32 * wid + lc;
 
+    __shared__ __align__(16) float sbias[128 + 128 + 32];   // b1, b2, b3 (padded)
     bf16x8 w1f[KS1], w2f[8], w3f[8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ++ks)
@@ -233,7 +243,9 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
         w2f[ks] = *reinterpret_cast<const bf16x8 *>(W2 + (size_t)hcol * 128 + 16 * ks + 8 * lh);
         w3f[ks] = *reinterpret_cast<const bf16x8 *>(W3p + (size_t)lc * 128 + 16 * ks + 8 * lh);
     }
-    const float bias1 = b1[hcol], bias2 = b2[hcol], bias3 = b3p[lc];
+    for (int i = tid; i < 128 + 128 + 32; i += 256) sbias[i] = i < 128 ? b1[i] : (i < 256 ? b2[i - 128] : b3p[i - 256]);
+    // rows of this lane's accumulator group g: units 32 * wid + 8 * g + 4 * lh + 0..3
+    const int u0 = 32 * wid + 4 * lh;
 
     const int rw = (g.rowlen + 3) >> 2;
     const int tail = g.rowlen & 3;
@@ -266,19 +278,23 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
             for (int k = used / 2 + q4; k < DK / 2; k += 4) dst[k] = 0u;
         }
         __syncthreads();
-        // layer 1
+        // layer 1: acc[r] = unit 32 * wid + 8 * (r / 4) + 4 * lh + r % 4 of patch 32 * pt + lc
 #pragma unroll
         for (int pt = 0; pt < 2; ++pt) {
             floatx16 acc = {};
-            const __bf16 *xa = X + (size_t)(32 * pt + lc) * XS + 8 * lh;
+            const __bf16 *xb = X + (size_t)(32 * pt + lc) * XS + 8 * lh;
 #pragma unroll
             for (int ks = 0; ks < KS1; ++ks)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8 *>(xa + 16 * ks),
-                                                              w1f[ks], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1f[ks], *reinterpret_cast<const bf16x8 *>(xb + 16 * ks),
+                                                              acc, 0, 0, 0);
+            __bf16 *hrow = H1 + (size_t)(32 * pt + lc) * kHS;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = 32 * pt + (r >> 2) * 8 + lh * 4 + (r & 3);
-                H1[row * kHS + hcol] = (__bf16)gelu_f(acc[r] + bias1);
+            for (int gq = 0; gq < 4; ++gq) {
+                const int u = u0 + 8 * gq;
+                const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + u);
+                const floatx4 v = {gelu_fast(acc[4 * gq + 0] + bb[0]), gelu_fast(acc[4 * gq + 1] + bb[1]),
+                                   gelu_fast(acc[4 * gq + 2] + bb[2]), gelu_fast(acc[4 * gq + 3] + bb[3])};
+                *reinterpret_cast<bf16x4 *>(hrow + u) = __builtin_convertvector(v, bf16x4);
             }
         }
         __syncthreads();
@@ -286,33 +302,45 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
         floatx16 acc2[2] = {};
 #pragma unroll
         for (int pt = 0; pt < 2; ++pt) {
-            const __bf16 *ha = H1 + (32 * pt + lc) * kHS + 8 * lh;
+            const __bf16 *hb = H1 + (32 * pt + lc) * kHS + 8 * lh;
 #pragma unroll
             for (int ks = 0; ks < 8; ++ks)
-                acc2[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8 *>(ha + 16 * ks),
-                                                                   w2f[ks], acc2[pt], 0, 0, 0);
+                acc2[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w2f[ks], *reinterpret_cast<const bf16x8 *>(hb + 16 * ks),
+                                                                   acc2[pt], 0, 0, 0);
         }
 #pragma unroll
-        for (int pt = 0; pt < 2; ++pt)
+        for (int pt = 0; pt < 2; ++pt) {
+            __bf16 *hrow = X + (size_t)(32 * pt + lc) * kHS;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = 32 * pt + (r >> 2) * 8 + lh * 4 + (r & 3);
-                X[row * kHS + hcol] = (__bf16)gelu_f(acc2[pt][r] + bias2);
+            for (int gq = 0; gq < 4; ++gq) {
+                const int u = u0 + 8 * gq;
+                const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + 128 + u);
+                const floatx4 v = {gelu_fast(acc2[pt][4 * gq + 0] + bb[0]), gelu_fast(acc2[pt][4 * gq + 1] + bb[1]),
+                                   gelu_fast(acc2[pt][4 * gq + 2] + bb[2]), gelu_fast(acc2[pt][4 * gq + 3] + bb[3])};
+                *reinterpret_cast<bf16x4 *>(hrow + u) = __builtin_convertvector(v, bf16x4);
             }
+        }
         __syncthreads();
-        // layer 3
+        // layer 3: acc[r] = output 8 * (r / 4) + 4 * lh + r % 4 of patch 32 * wid + lc
         if (wid < 2) {
             floatx16 acc = {};
-            const __bf16 *ha = X + (32 * wid + lc) * kHS + 8 * lh;
+            const __bf16 *hb = X + (32 * wid + lc) * kHS + 8 * lh;
 #pragma unroll
             for (int ks = 0; ks < 8; ++ks)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(*reinterpret_cast<const bf16x8 *>(ha + 16 * ks),
-                                                              w3f[ks], acc, 0, 0, 0);
-            // acc[r]: patch row (r/4)*8 + lh*4 + r%4 of this wave's 32, output column lc
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w3f[ks], *reinterpret_cast<const bf16x8 *>(hb + 16 * ks),
+                                                              acc, 0, 0, 0);
+            const long long pi = p0 + 32 * wid + lc;
+            if (pi < npatch) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const long long pi = p0 + 32 * wid + (r >> 2) * 8 + lh * 4 + (r & 3);
-                if (pi < npatch && lc < dpad) out[pi * dpad + lc] = lc < d ? acc[r] + bias3 : 0.0f;
+                for (int gq = 0; gq < 4; ++gq) {
+                    const int o = 8 * gq + 4 * lh;
+                    if (o < dpad) {
+                        floatx4 v;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) v[i] = o + i < d ? acc[4 * gq + i] + sbias[256 + o + i] : 0.0f;
+                        *reinterpret_cast<floatx4 *>(out + pi * dpad + o) = v;
+                    }
+                }
             }
         }
         __syncthreads();
