@@ -84,17 +84,18 @@ __device__ __forceinline__ void klist_init(uint64_t (&l)[LM], int L) {
     for (int s = 0; s < LM; ++s) l[s] = (s < LM - L) ? 0ull : kNoKey;
 }
 
+// Branch-free: the list is sorted, so l[s] <= key holds for a prefix of the
+// slots; slot s keeps its key inside that prefix, takes `key` right after it
+// and its left neighbour's key further on (a key above every entry changes
+// nothing).
 template <int LM>
 __device__ __forceinline__ void klist_insert(uint64_t (&l)[LM], uint64_t key) {
-    int r = 0;
+    bool le[LM];
 #pragma unroll
-    for (int s = 0; s < LM - 1; ++s) r += (l[s] <= key) ? 1 : 0;
+    for (int s = 0; s < LM; ++s) le[s] = l[s] <= key;
 #pragma unroll
-    for (int s = LM - 1; s >= 1; --s)
-        if (s > r) l[s] = l[s - 1];
-#pragma unroll
-    for (int s = 0; s < LM; ++s)
-        if (s == r) l[s] = key;
+    for (int s = LM - 1; s >= 1; --s) l[s] = le[s] ? l[s] : (le[s - 1] ? key : l[s - 1]);
+    l[0] = le[0] ? l[0] : key;
 }
 
 constexpr int kMaxRW = 10;   // dwords per padded patch row (p <= 12)
