@@ -97,7 +97,7 @@ struct annf_ctx {
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
     DevBuf pca_scratch;
-    DevBuf sort_keys, sort_tmp, mm_blkleaf;
+    DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // timing
@@ -135,7 +135,7 @@ struct annf_ctx {
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
                           &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch,
-                          &sort_keys, &sort_tmp, &mm_blkleaf};
+                          &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder};
         for (DevBuf *b : list) b->release();
         has_tree = false;
         has_encoder = false;
@@ -758,12 +758,18 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     int *skeys = c->sort_keys.as<int>();
     RCHK(c->mm_blkleaf.ensure(leafmm_scratch_bytes((int)n_sort, nleaves)));
     void *blk_info = c->mm_blkleaf.p;
+    const int *pts_order = t.orig;
+    if (!fp8) {
+        RCHK(c->mm_ptorder.ensure((size_t)fs.n_b * sizeof(int)));
+        HIPCHK(leaf_point_order(t, c->mm_ptorder.as<int>(), c->stream));
+        pts_order = c->mm_ptorder.as<int>();
+    }
     HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, nullptr, c->stream));
     HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafmm_chunk(), skeys, skeys + n_sort,
                          c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(), c->bchunk.as<int>(),
                          c->sort_tmp.p, sort_bytes, c->stream));
     c->mark(4);
-    HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, n_a,
+    HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, n_a,
                    nprobe, shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), c->stream));
     c->mark(5);
     uint64_t *keys2 = nullptr;
@@ -778,7 +784,7 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                              skeys + n_sort, c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(),
                              c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
         HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
-        HIPCHK(leaf_mm(t, fs.ga, fs.gb, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info,
+        HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info,
                        (int)n_pairs, 1, shortlist, K, fp8, prop_k, keys2, c->stream));
     }
     HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,

@@ -98,6 +98,17 @@ __device__ __forceinline__ void klist_insert(uint64_t (&l)[LM], uint64_t key) {
     l[0] = le[0] ? l[0] : key;
 }
 
+// The same sorted list of 32-bit keys.
+template <int LM>
+__device__ __forceinline__ void klist_insert32(uint32_t (&l)[LM], uint32_t key) {
+    bool le[LM];
+#pragma unroll
+    for (int s = 0; s < LM; ++s) le[s] = l[s] <= key;
+#pragma unroll
+    for (int s = LM - 1; s >= 1; --s) l[s] = le[s] ? l[s] : (le[s - 1] ? key : l[s - 1]);
+    l[0] = le[0] ? l[0] : key;
+}
+
 constexpr int kMaxRW = 10;   // dwords per padded patch row (p <= 12)
 
 // Row r of a patch as rw (<= kMaxRW) dwords, tail masked, into w[]; every
@@ -230,6 +241,7 @@ __device__ __forceinline__ float sqrt_f(uint32_t d2) {
 
 template <int DK, int LM, bool FP8>
 __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(TreeView t, PatchGeom ga, PatchGeom gb,
+                                                         const int *__restrict__ pts_order,
                                                          const int *__restrict__ order,
                                                          const int *__restrict__ chunk_off,
                                                          const int4 *__restrict__ blk_info, int nprobe, int K,
@@ -291,7 +303,7 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
 #pragma unroll
         for (int u = 0; u < kBP; ++u) {
             const int i = (tid >> 2) + u * (kThreads / 4);
-            bj[u] = i < np ? t.orig[from + t0 + i] : 0;
+            bj[u] = i < np ? pts_order[from + t0 + i] : 0;
         }
     };
     auto tile_fetch = [&]() {
@@ -317,6 +329,7 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
     // this lane's LM smallest keys (every slot live)
     uint64_t lst[LM];
     klist_init(lst, LM);
+    bool first_tile = true;
 
     while (more) {
         __syncthreads();   // the previous tile's distances are consumed
@@ -412,9 +425,12 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                             const float dd = ((float)na[qq] + (float)nbp) - 2.0f * (acc / (sa[qq] * sbp));
                             d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
                         } else {
-                            // a.b as the sum of two exact integer halves
+                            // a.b as the sum of two exact integer halves; the key
+                            // (d2 << 7) | tile position orders like (d2, B index)
+                            // (positions follow ascending B index, d2 < 2^25)
                             const float accx = h ? acc1x[r] : acc0x[r];
-                            d = na[qq] + nbp - 2u * ((uint32_t)(int)acc + (uint32_t)(int)accx);
+                            d = ((na[qq] + nbp - 2u * ((uint32_t)(int)acc + (uint32_t)(int)accx)) << 7) |
+                                (uint32_t)pcol;
                         }
                         Dm[qq * kDS + pcol] = pv ? d : 0xFFFFFFFFu;
                     }
@@ -423,11 +439,34 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
             __syncthreads();
             if (q < nqb) {
                 const uint32_t *dr = Dm + q * kDS;
-                for (int p = q4; p < tnp; p += 4) {
-                    const uint64_t key = ((uint64_t)dr[p] << 32) | (uint32_t)borig[p];
-                    if (key < lst[LM - 1]) klist_insert(lst, key);
+                if constexpr (FP8) {
+                    for (int p = q4; p < tnp; p += 4) {
+                        const uint64_t key = ((uint64_t)dr[p] << 32) | (uint32_t)borig[p];
+                        if (key < lst[LM - 1]) klist_insert(lst, key);
+                    }
+                } else {
+                    // this tile's best with 32-bit keys, then as (d2, B index) keys
+                    uint32_t l32[LM];
+#pragma unroll
+                    for (int s2 = 0; s2 < LM; ++s2) l32[s2] = 0xFFFFFFFFu;
+                    for (int p = q4; p < tnp; p += 4) {
+                        const uint32_t key = dr[p];
+                        if (key < l32[LM - 1]) klist_insert32(l32, key);
+                    }
+#pragma unroll
+                    for (int s2 = 0; s2 < LM; ++s2) {
+                        const uint32_t k = l32[s2];
+                        const uint64_t key =
+                            k == 0xFFFFFFFFu ? kNoKey : (((uint64_t)(k >> 7) << 32) | (uint32_t)borig[k & 127u]);
+                        if (first_tile) {
+                            lst[s2] = key;   // ascending already
+                        } else if (key < lst[LM - 1]) {
+                            klist_insert(lst, key);
+                        }
+                    }
                 }
             }
+            first_tile = false;
         }
         // the next tile's loads, in flight across the barrier
         t0 += kNP;
@@ -522,6 +561,33 @@ __global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K,
     }
     pairleaf[t] = leaf;
     if (count && leaf >= 0) wave_count1(count, leaf);
+}
+
+// Each leaf's points in ascending B index (the BF16 kernel's tile position
+// order), by rank: out[from + #(smaller in the leaf)] = index.
+constexpr int kLeafSortLds = 2048;
+__global__ __launch_bounds__(256) void k_leaf_order(const int *__restrict__ leaves, const int *__restrict__ orig,
+                                                    int *__restrict__ out) {
+    __shared__ int v[kLeafSortLds];
+    const int l = blockIdx.x, from = leaves[2 * l], c = leaves[2 * l + 1] - from;
+    const int *src = orig + from;
+    if (c <= kLeafSortLds) {
+        for (int i = threadIdx.x; i < c; i += 256) v[i] = src[i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < c; i += 256) {
+            const int x = v[i];
+            int r = 0;
+            for (int j = 0; j < c; ++j) r += v[j] < x;
+            out[from + r] = x;
+        }
+    } else {
+        for (int i = threadIdx.x; i < c; i += 256) {
+            const int x = src[i];
+            int r = 0;
+            for (int j = 0; j < c; ++j) r += src[j] < x;
+            out[from + r] = x;
+        }
+    }
 }
 
 // The chunks c of leaf l (chunk_off[l] <= c < chunk_off[l + 1]): blk_info[2c]
@@ -635,9 +701,14 @@ static size_t leafmm_blocks(int nq, int nleaves) { return (size_t)nleaves + ((si
 
 size_t leafmm_scratch_bytes(int nq, int nleaves) { return leafmm_blocks(nq, nleaves) * 2 * sizeof(int4); }
 
-hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *order, const int *offset,
-                   const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K, int fp8,
-                   int slot_div, uint64_t *keys_out, hipStream_t st) {
+hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_leaf_order, dim3(1u << t.depth), dim3(256), 0, st, t.leaves, t.orig, out);
+    return hipGetLastError();
+}
+
+hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *pts_order, const int *order,
+                   const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K,
+                   int fp8, int slot_div, uint64_t *keys_out, hipStream_t st) {
     if (nq == 0) return hipSuccess;
     const int p = ga.rows;
     if (!leafmm_supported(p) || gb.rows != p || ga.rowlen != 3 * p || gb.rowlen != 3 * p) return hipErrorInvalidValue;
@@ -663,7 +734,8 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
             hipError_t e = hipFuncSetAttribute((const void *)k_leaf_mm<DK, LM, F8>,
                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb, order,
+            hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb,
+                               F8 ? t.orig : pts_order, order,
                                chunk_off, blk_info, nprobe, K, slot_div, keys_out);
             return hipGetLastError();
         };
