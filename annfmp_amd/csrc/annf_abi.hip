@@ -97,7 +97,7 @@ struct annf_ctx {
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
     DevBuf pca_scratch;
-    DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder;
+    DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder, proj_pack;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // timing
@@ -135,7 +135,7 @@ struct annf_ctx {
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
                           &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch,
-                          &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder};
+                          &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack};
         for (DevBuf *b : list) b->release();
         has_tree = false;
         has_encoder = false;
@@ -636,6 +636,11 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                            dpad, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
                            c->mean64.as<double>());
         HIPCHK(hipGetLastError());
+        if (projection == ANNF_PROJ_PCA32) {
+            RCHK(c->proj_pack.ensure(projection_pack_floats(p, dpad) * sizeof(float)));
+            HIPCHK(pack_projection(c->basis32.as<float>(), c->mean32.as<float>(), p, dpad, c->proj_pack.as<float>(),
+                                   c->stream));
+        }
     }
     auto project = [&](const uint8_t *img, int h, int w, const PatchGeom &g, long long n, float *out,
                        hipStream_t st, int per_cu) -> int {
@@ -645,7 +650,8 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                                   st, per_cu));
         } else {
             HIPCHK(project_patches(img, h, w, p, c->basis32.as<float>(), c->mean32.as<float>(),
-                                   c->basis64.as<double>(), c->mean64.as<double>(), d, dpad, projection, out, st));
+                                   c->basis64.as<double>(), c->mean64.as<double>(), d, dpad,
+                                   projection == ANNF_PROJ_PCA64, out, c->proj_pack.as<float>(), per_cu, st));
         }
         return ANNF_OK;
     };

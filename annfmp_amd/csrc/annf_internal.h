@@ -129,9 +129,14 @@ hipError_t best_of_k(const PatchGeom &ga, const PatchGeom &gb, const int *fl_pos
                      float *dist_out, int *knn_out, hipStream_t st);
 hipError_t convert_knn(const int *pos, const float *d2, const int *orig, long long count,
                        int *idx_out, float *dist_out, hipStream_t st);
+// FP64: CUDA-core FMA over basis_t64 / mean64.  FP32: MFMA GEMM over `pack`,
+// the padded component table pack_projection writes (projection_pack_floats
+// floats); per_cu persistent blocks per CU.
 hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float *basis_t,
                            const float *mean_f, const double *basis_t64, const double *mean64,
-                           int d, int dpad, int fp64, float *out, hipStream_t st);
+                           int d, int dpad, int fp64, float *out, const float *pack, int per_cu, hipStream_t st);
+size_t projection_pack_floats(int p, int dpad);
+hipError_t pack_projection(const float *basis_t, const float *mean_f, int p, int dpad, float *pack, hipStream_t st);
 
 // full-patch leaf k-NN on MFMA (annf_leafmm.hip): queries bucketed with
 // chunks of leafmm_chunk(); BF16 (exact distances) or FP8 (shortlist of

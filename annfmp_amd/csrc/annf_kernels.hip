@@ -20,6 +20,7 @@
 // rounding per operation (the library is compiled with -ffp-contract=off),
 // which is the reference's arithmetic (helper.c:164-179).
 #include "annf_internal.h"
+#include "annf_patch.h"
 
 #include <type_traits>
 #include <utility>
@@ -2295,9 +2296,113 @@ __global__ __launch_bounds__(256) void k_project(const uint8_t *__restrict__ img
     }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 projection as an MFMA GEMM (patches x components, FP32 accumulate)
+// ---------------------------------------------------------------------------
+// Persistent blocks of four wavefronts hold the components (Bt[c][f], the
+// patch's padded feature order, zero at the pads and for c >= d) and the mean
+// in LDS; per tile of 64 patches the windows are staged as bytes (a quad of
+// lanes per patch) and every wavefront multiplies its 16 patches by the
+// components with v_mfma_f32_16x16x4_f32.  Lane l of a wavefront takes patch
+// row l % 16 and, within each group of 16 features, the four consecutive
+// features 4 (l / 16) .. +3 (one dword of bytes, one float4 of mean and of
+// each component), so the sum runs over the same features as the CUDA-core
+// projection, in another order.
+typedef float floatx4v __attribute__((ext_vector_type(4)));
+constexpr int kProjTile = 64;
+
+template <int DK, int CT>
+__global__ __launch_bounds__(256) void k_project_mfma(PatchGeom g, long long npatch, const float *__restrict__ Bt,
+                                                      const float *__restrict__ mu, int dpad, float *__restrict__ out) {
+    constexpr int RB = DK + 4;   // staged row stride (bytes; an odd dword count)
+    extern __shared__ __align__(16) uint8_t psm[];
+    float *sBt = reinterpret_cast<float *>(psm);          // CT * 16 x DK
+    float *smu = sBt + CT * 16 * DK;                       // DK
+    uint8_t *X = reinterpret_cast<uint8_t *>(smu + DK);   // kProjTile x RB
+    const int tid = threadIdx.x, wid = tid >> 6, lane = tid & 63;
+    for (int i = tid; i < CT * 16 * DK; i += 256) sBt[i] = Bt[i];
+    for (int i = tid; i < DK; i += 256) smu[i] = mu[i];
+    const int rw = (g.rowlen + 3) >> 2;
+    const int tail = g.rowlen & 3;
+    const uint32_t tmask = tail ? ((1u << (8 * tail)) - 1u) : ~0u;
+    constexpr int dkw = DK / 4;
+    using SS = StageShape<DK>;
+    const long long ntiles = (npatch + kProjTile - 1) / kProjTile;
+    const int prow = lane & 15, kq = lane >> 4;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long long p0 = tile * kProjTile;
+        {
+            const int pl = tid >> 2, q4 = tid & 3;
+            const long long pi = p0 + pl;
+            const bool pv = pi < npatch;
+            const long long o = patch_origin(g, pv ? pi : 0);
+            uint32_t rows[SS::RL][SS::LW];
+            fetch_rows(g, o, pv, q4, 4, rows);
+            uint32_t *dst = reinterpret_cast<uint32_t *>(X + (size_t)pl * RB);
+            for_row_words(g, o, rows, q4, 4, rw, tmask, [&](int r, int k, uint32_t v) { dst[r * rw + k] = v; });
+            for (int k = g.rows * rw + q4; k < dkw; k += 4) dst[k] = 0u;
+        }
+        __syncthreads();
+        floatx4v acc[CT];
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) acc[ct] = floatx4v{0.0f, 0.0f, 0.0f, 0.0f};
+        const uint8_t *xr = X + (size_t)(16 * wid + prow) * RB + 4 * kq;
+#pragma unroll 5
+        for (int f0 = 0; f0 < DK; f0 += 16) {
+            const uint32_t x4 = *reinterpret_cast<const uint32_t *>(xr + f0);
+            const floatx4v m4 = *reinterpret_cast<const floatx4v *>(smu + f0 + 4 * kq);
+            float a[4];
+            a[0] = (float)(x4 & 255u) - m4[0];
+            a[1] = (float)((x4 >> 8) & 255u) - m4[1];
+            a[2] = (float)((x4 >> 16) & 255u) - m4[2];
+            a[3] = (float)(x4 >> 24) - m4[3];
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct) {
+                const floatx4v b4 = *reinterpret_cast<const floatx4v *>(sBt + (size_t)(16 * ct + prow) * DK + f0 + 4 * kq);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[j], b4[j], acc[ct], 0, 0, 0);
+            }
+        }
+        // acc[ct][r]: patch 16 * wid + 4 * kq + r, component 16 * ct + prow
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) {
+            const int comp = 16 * ct + prow;
+            if (comp < dpad) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const long long pi = p0 + 16 * wid + 4 * kq + r;
+                    if (pi < npatch) out[pi * dpad + comp] = acc[ct][r];
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Components and mean in the MFMA kernel's padded feature order:
+// Bt[c * DK + fp] and mu[fp] for padded feature fp = r * 4 rw + u, u < 3p.
+__global__ void k_pack_projection(const float *__restrict__ bt32, const float *__restrict__ m32, int p, int dpad,
+                                  int DK, int ctw, float *__restrict__ Bt, float *__restrict__ mu) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ctw * DK) return;
+    const int c = i / DK, fp = i - c * DK;
+    const int rl = 3 * p, rw4 = 4 * ((rl + 3) / 4);
+    const int r = fp / rw4, u = fp - r * rw4;
+    const bool real = r < p && u < rl;
+    const int f = r * rl + u;
+    Bt[i] = (real && c < dpad) ? bt32[(size_t)f * dpad + c] : 0.0f;
+    if (c == 0) mu[fp] = real ? m32[f] : 0.0f;
+}
+
+int projection_dk(int p) {
+    const int rw = (3 * p + 3) / 4;
+    const int need = p * rw * 4;
+    return need <= 176 ? 176 : need <= 256 ? 256 : need <= 400 ? 400 : need <= 512 ? 512 : 0;
+}
+
 hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float *basis_t, const float *mean_f,
                            const double *basis_t64, const double *mean64, int d, int dpad, int fp64, float *out,
-                           hipStream_t st) {
+                           const float *pack, int per_cu, hipStream_t st) {
     const int D = p * p * 3;
     const long long n = (long long)(h - p + 1) * (w - p + 1);
     if (n <= 0) return hipSuccess;
@@ -2311,7 +2416,8 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_project<DP, double>), dim3(blocks_for(n, 256)), dim3(256), lds, st, img, h, w, p,
                                basis_t64, mean64, d, out);
-        } else {
+        } else if (projection_dk(p) == 0) {
+            // p > 12: the CUDA-core kernel
             const size_t lds = (size_t)D * (DP + 1) * sizeof(float);
             if (lds > 160 * 1024) return hipErrorInvalidValue;
             hipError_t e = hipFuncSetAttribute((const void *)k_project<DP, float>,
@@ -2319,9 +2425,56 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_project<DP, float>), dim3(blocks_for(n, 256)), dim3(256), lds, st, img, h, w, p,
                                basis_t, mean_f, d, out);
+        } else {
+            // FP32 on MFMA over the table of pack_projection
+            const int DKp = projection_dk(p);
+            if (!pack) return hipErrorInvalidValue;
+            const int ct = DP <= 16 ? 1 : 2;
+            const float *Bt = pack, *mu = pack + (size_t)16 * ct * DKp;
+            const PatchGeom g{img, 1, w - p + 1, w, p, 3 * p, 3LL * w, 3LL * w * h};
+            int dev = 0, ncu = 256;
+            (void)hipGetDevice(&dev);
+            (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+            const long long ntiles = (n + kProjTile - 1) / kProjTile;
+            const long long cap = (long long)(per_cu > 0 ? per_cu : 2) * ncu;
+            const unsigned grid = (unsigned)(ntiles < cap ? ntiles : cap);
+            auto launch = [&](auto dk, auto ctv) -> hipError_t {
+                constexpr int DK = decltype(dk)::value, CT = decltype(ctv)::value;
+                const size_t lds = ((size_t)CT * 16 * DK + DK) * sizeof(float) + (size_t)kProjTile * (DK + 4);
+                hipError_t e = hipFuncSetAttribute((const void *)k_project_mfma<DK, CT>,
+                                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                if (e != hipSuccess) return e;
+                hipLaunchKernelGGL((k_project_mfma<DK, CT>), dim3(grid), dim3(256), lds, st, g, n, Bt, mu, DP, out);
+                return hipGetLastError();
+            };
+            auto with_ct = [&](auto dk) -> hipError_t {
+                return ct == 1 ? launch(dk, std::integral_constant<int, 1>{})
+                               : launch(dk, std::integral_constant<int, 2>{});
+            };
+            switch (DKp) {
+            case 176: return with_ct(std::integral_constant<int, 176>{});
+            case 256: return with_ct(std::integral_constant<int, 256>{});
+            case 400: return with_ct(std::integral_constant<int, 400>{});
+            default: return with_ct(std::integral_constant<int, 512>{});
+            }
         }
         return hipGetLastError();
     });
+}
+
+size_t projection_pack_floats(int p, int dpad) {
+    const int DK = projection_dk(p);
+    return DK == 0 ? 1 : (size_t)(dpad <= 16 ? 16 : 32) * DK + DK;
+}
+
+hipError_t pack_projection(const float *basis_t, const float *mean_f, int p, int dpad, float *pack, hipStream_t st) {
+    const int DK = projection_dk(p);
+    if (DK == 0) return hipSuccess;   // no table: the CUDA-core kernel projects
+    if (dpad > 32) return hipErrorInvalidValue;
+    const int ctw = dpad <= 16 ? 16 : 32;
+    hipLaunchKernelGGL(k_pack_projection, dim3(blocks_for((long long)ctw * DK, 256)), dim3(256), 0, st, basis_t,
+                       mean_f, p, dpad, DK, ctw, pack, pack + (size_t)ctw * DK);
+    return hipGetLastError();
 }
 
 }  // namespace annf
