@@ -292,17 +292,18 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         RCHK(c->bcursor.ensure((size_t)nleaves * sizeof(int)));
         RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
         RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
-        HIPCHK(hipMemsetAsync(c->bcount.p, 0, (size_t)nleaves * sizeof(int), c->stream));
+        RCHK(c->sort_keys.ensure(2 * (size_t)n_a * sizeof(int)));
+        const size_t sort_bytes = sort_scratch_bytes(n_a, nleaves);
+        RCHK(c->sort_tmp.ensure(sort_bytes));
         int *natdesc = nullptr;
         if (propagate) {
             RCHK(c->natdesc.ensure((size_t)n_a * sizeof(int)));
             natdesc = c->natdesc.as<int>();
         }
-        HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), c->bcount.as<int>(), natdesc,
-                              c->stream));
-        HIPCHK(bucket_queries(c->natleaf.as<int>(), n_a, nleaves, c->bcount.as<int>(), c->boffset.as<int>(),
-                              c->bcursor.as<int>(), c->order.as<int>(), c->bchunk.as<int>(), leafscan_chunk(),
-                              c->stream));
+        HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, natdesc, c->stream));
+        HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafscan_chunk(), c->sort_keys.as<int>(),
+                             c->sort_keys.as<int>() + n_a, c->order.as<int>(), c->boffset.as<int>(),
+                             c->bcount.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
         if (propagate) {
             RCHK(c->p1d.ensure(nk * sizeof(float)));
             RCHK(c->p1p.ensure(nk * sizeof(int)));

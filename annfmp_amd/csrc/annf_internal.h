@@ -98,11 +98,6 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
 // packed leaf ranges (desc: (from << 1) | (size - (n >> depth))).
 hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, int *desc,
                           hipStream_t st);
-// Buckets queries by natural leaf; `count` must hold the per-leaf counts and
-// is overwritten with the number of `chunk`-query chunks per leaf; chunk_off
-// receives nleaves+1 offsets.
-hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset,
-                          int *cursor, int *order, int *chunk_off, int chunk, hipStream_t st);
 int leafscan_chunk();
 // chunk_off[0..nleaves] of ceil(size / chunk) chunks per leaf (chunk_cnt: scratch)
 hipError_t chunk_offsets(const int *offset, int nleaves, int chunk, int *chunk_cnt, int *chunk_off, hipStream_t st);

@@ -124,32 +124,6 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 
-// atomicAdd(&ctr[key], 1) for every active lane; lanes of the wavefront that
-// share a key are combined into one atomic (two rounds, then per-lane), so a
-// run of identical keys (flat image regions) does not serialise on one
-// address.  Returns the value this lane's increment saw.
-__device__ __forceinline__ int wave_add1(int *ctr, int key) {
-    int res = 0;
-    bool done = false;
-    for (int round = 0; round < 2; ++round) {
-        if (!done) {
-            const int lead = __builtin_amdgcn_readfirstlane(key);
-            const bool mine = key == lead;
-            const unsigned long long m = __ballot(mine);
-            if (mine) {
-                const int r = lane_rank(m);
-                int base = 0;
-                if (r == 0) base = atomicAdd(&ctr[lead], __popcll(m));
-                base = __builtin_amdgcn_readlane(base, __ffsll(m) - 1);
-                res = base + r;
-                done = true;
-            }
-        }
-    }
-    if (!done) res = atomicAdd(&ctr[key], 1);
-    return res;
-}
-
 template <typename Fn>
 static hipError_t with_dp(int dpad, Fn &&fn) {
     switch (dpad) {
@@ -1203,16 +1177,6 @@ __global__ __launch_bounds__(1024) void k_scan_single(const int *count, int n, i
     if (threadIdx.x == blockDim.x - 1) offset[n] = run;
 }
 
-// order[cursor[leaf[i]]++] = i; entries with leaf < 0 are left out
-__global__ void k_bucket(const int *leaf, int nq, int *cursor, int *order) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
-    const int l = leaf[i];
-    if (l < 0) return;
-    const int p = wave_add1(cursor, l);
-    order[p] = i;
-}
-
 // Blocks of the leaf scan: a leaf's bucket is cut into chunks of kScanChunk
 // queries, so one crowded leaf (a flat image region) spreads over many blocks.
 constexpr int kScanChunk = 256;
@@ -1229,14 +1193,6 @@ hipError_t chunk_offsets(const int *offset, int nleaves, int chunk, int *chunk_c
                        chunk_cnt);
     hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, chunk_cnt, nleaves, chunk_off, (int *)nullptr);
     return hipGetLastError();
-}
-
-hipError_t bucket_queries(const int *leaf, int nq, int nleaves, int *count, int *offset, int *cursor, int *order,
-                          int *chunk_off, int chunk, hipStream_t st) {
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, count, nleaves, offset, cursor);
-    if (nq > 0) hipLaunchKernelGGL(k_bucket, dim3(blocks_for(nq, 256)), dim3(256), 0, st, leaf, nq, cursor, order);
-    // chunks per leaf (into count, no longer needed) and their offsets
-    return chunk_offsets(offset, nleaves, chunk, count, chunk_off, st);
 }
 
 // Sorted k-list of runtime length K <= KM held in registers.  The K live
