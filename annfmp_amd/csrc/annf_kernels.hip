@@ -254,9 +254,10 @@ __device__ __forceinline__ void chunk_range(const LevelArgs &L, int &node, int &
 }
 
 // Radix select of the rank-c/2 key in three passes of 11, 11 and 10 bits.
-// A node's histogram has hist_rep(cpn) copies (its blocks add into copy
-// chunk % R, which spreads the atomics of the wide top levels); after each
-// pass one block per node sums the copies and picks the digit.
+// The first pass's node histogram has hist_rep(cpn) copies (its blocks add
+// into copy chunk % R, which spreads the atomics of the wide top levels) and
+// one block per node sums them and picks the digit; the later passes count
+// only keys under the prefix picked so far, into one copy.
 constexpr int kRBins = 2048;
 constexpr int kHistRep = 8;
 __host__ __device__ __forceinline__ int hist_rep(int cpn) { return cpn < kHistRep ? cpn : kHistRep; }
@@ -423,7 +424,7 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict
     __shared__ int sax;
     {
         const size_t gt = (size_t)blockIdx.x * kBuildTPB + threadIdx.x, gn = (size_t)gridDim.x * kBuildTPB;
-        for (size_t i = gt; i < hwords; i += gn) {
+        for (size_t i = gt; i < (size_t)L.nodes * kRBins; i += gn) {
             H1[i] = 0u;
             H2[i] = 0u;
         }
@@ -458,17 +459,32 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict
 }
 
 // Pass PASS (1 or 2): the histogram of the next digit over the keys that
-// match the prefix picked so far.
+// match the prefix picked so far, into one copy per node (few keys match, so
+// the atomics do not contend).  Pass 2 first makes pass 1's pick itself, every
+// block from the complete histogram (the same result in every block; the
+// node's first block stores it).
 template <int PASS>
-__global__ __launch_bounds__(kBuildTPB) void k_lvl_pass(LevelArgs L, const uint32_t *keys, uint32_t *Hc,
-                                                        const uint32_t *pre_in) {
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_pass(LevelArgs L, const uint32_t *keys, const uint32_t *Hp,
+                                                        uint32_t *Hc, const uint32_t *pre_in, const int *rank_in,
+                                                        uint32_t *pre_out, int *rank_out) {
     __shared__ uint32_t h[kRBins];
+    __shared__ int2 wsum[kBuildTPB / kWave];
+    __shared__ uint32_t sres[2];
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
     if (s >= e) return;
-    const int R = hist_rep(L.cpn), rep = (blockIdx.x - node * L.cpn) % R;
-    const uint32_t pre = pre_in[node];
     for (int i = threadIdx.x; i < kRBins; i += kBuildTPB) h[i] = 0u;
+    uint32_t pre = pre_in[node];
+    if (PASS == 2) {
+        const uint32_t r0 = (uint32_t)rank_in[node];
+        uint32_t digit, before;
+        block_pick<pass_bins(1)>(Hp + (size_t)node * kRBins, 1, r0, wsum, sres, digit, before);
+        pre |= digit << pass_shift(1);
+        if (s == lo && threadIdx.x == 0) {
+            pre_out[node] = pre;
+            rank_out[node] = (int)(r0 - before);
+        }
+    }
     __syncthreads();
     for (int i = s + threadIdx.x; i < e; i += kBuildTPB) {
         const uint32_t k = keys[i];
@@ -476,20 +492,27 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_pass(LevelArgs L, const uint3
             atomicAdd(&h[(k >> pass_shift(PASS)) & (uint32_t)(pass_bins(PASS) - 1)], 1u);
     }
     __syncthreads();
-    flush_hist(h, Hc + ((size_t)node * R + rep) * kRBins);
+    flush_hist(h, Hc + (size_t)node * kRBins);
 }
 
-// Per-chunk counts of keys below / equal to the node's split key.
-__global__ __launch_bounds__(kBuildTPB) void k_lvl_count(LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
+// The last pick (the node's split key, made by every block from the complete
+// pass-2 histogram; the node's first block stores it in vkey) and per-chunk
+// counts of keys below / equal to it.
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_count(LevelArgs L, const uint32_t *keys, const uint32_t *H2,
+                                                         const uint32_t *pre_in, const int *rank_in, uint32_t *vkey,
                                                          int2 *cnt) {
     __shared__ int2 wsum[kBuildTPB / kWave];
+    __shared__ uint32_t sres[2];
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
     if (s >= e) {
         if (threadIdx.x == 0) cnt[blockIdx.x] = make_int2(0, 0);
         return;
     }
-    const uint32_t vk = vkey[node];
+    uint32_t digit, before;
+    block_pick<pass_bins(2)>(H2 + (size_t)node * kRBins, 1, (uint32_t)rank_in[node], wsum, sres, digit, before);
+    const uint32_t vk = pre_in[node] | digit;
+    if (s == lo && threadIdx.x == 0) vkey[node] = vk;
     int lt = 0, eq = 0;
     for (int i = s + threadIdx.x; i < e; i += kBuildTPB) {
         const uint32_t k = keys[i];
@@ -1040,11 +1063,10 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                                s.keys, H0, H1, H2, hw, nbox);
             hipLaunchKernelGGL(k_lvl_pick<0>, gn, b, 0, st, L, H0, R, (const uint32_t *)nullptr,
                                (const int *)nullptr, P1, R1);
-            hipLaunchKernelGGL(k_lvl_pass<1>, g, b, 0, st, L, s.keys, H1, P1);
-            hipLaunchKernelGGL(k_lvl_pick<1>, gn, b, 0, st, L, H1, R, P1, R1, P2, R2);
-            hipLaunchKernelGGL(k_lvl_pass<2>, g, b, 0, st, L, s.keys, H2, P2);
-            hipLaunchKernelGGL(k_lvl_pick<2>, gn, b, 0, st, L, H2, R, P2, R2, vkey, (int *)nullptr);
-            hipLaunchKernelGGL(k_lvl_count, g, b, 0, st, L, s.keys, vkey, s.cnt);
+            hipLaunchKernelGGL(k_lvl_pass<1>, g, b, 0, st, L, s.keys, (const uint32_t *)nullptr, H1, P1,
+                               (const int *)nullptr, (uint32_t *)nullptr, (int *)nullptr);
+            hipLaunchKernelGGL(k_lvl_pass<2>, g, b, 0, st, L, s.keys, H1, H2, P1, R1, P2, R2);
+            hipLaunchKernelGGL(k_lvl_count, g, b, 0, st, L, s.keys, H2, P2, R2, vkey, s.cnt);
             hipLaunchKernelGGL(k_lvl_eqoff<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, vkey, s.cnt,
                                s.cnt_off, s.tot, s.eqoff, H0, hw_next, s.err);
             hipLaunchKernelGGL(k_move<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey, s.cnt_off,
