@@ -566,22 +566,23 @@ __global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K,
 // Each leaf's points in ascending B index (the BF16 kernel's tile position
 // order), by rank: out[from + #(smaller in the leaf)] = index.
 constexpr int kLeafSortLds = 2048;
-__global__ __launch_bounds__(256) void k_leaf_order(const int *__restrict__ leaves, const int *__restrict__ orig,
+constexpr int kLeafSortTPB = 128;   // a leaf of the usual size: one point per thread
+__global__ __launch_bounds__(kLeafSortTPB) void k_leaf_order(const int *__restrict__ leaves, const int *__restrict__ orig,
                                                     int *__restrict__ out) {
     __shared__ int v[kLeafSortLds];
     const int l = blockIdx.x, from = leaves[2 * l], c = leaves[2 * l + 1] - from;
     const int *src = orig + from;
     if (c <= kLeafSortLds) {
-        for (int i = threadIdx.x; i < c; i += 256) v[i] = src[i];
+        for (int i = threadIdx.x; i < c; i += kLeafSortTPB) v[i] = src[i];
         __syncthreads();
-        for (int i = threadIdx.x; i < c; i += 256) {
+        for (int i = threadIdx.x; i < c; i += kLeafSortTPB) {
             const int x = v[i];
             int r = 0;
             for (int j = 0; j < c; ++j) r += v[j] < x;
             out[from + r] = x;
         }
     } else {
-        for (int i = threadIdx.x; i < c; i += 256) {
+        for (int i = threadIdx.x; i < c; i += kLeafSortTPB) {
             const int x = src[i];
             int r = 0;
             for (int j = 0; j < c; ++j) r += src[j] < x;
@@ -590,18 +591,23 @@ __global__ __launch_bounds__(256) void k_leaf_order(const int *__restrict__ leav
     }
 }
 
-// The chunks c of leaf l (chunk_off[l] <= c < chunk_off[l + 1]): blk_info[2c]
-// = (leaf, first query slot in `order`, queries, first point of the leaf),
-// blk_info[2c + 1].x = points of the leaf; one read gives a block its work.
+// Block c's work record: the leaf l with chunk_off[l] <= c < chunk_off[l + 1]
+// (binary search, one thread per block, so a leaf with many chunks is no
+// serial tail); blk_info[2c] = (leaf, first query slot in `order`, queries,
+// first point of the leaf), blk_info[2c + 1].x = points of the leaf.
 __global__ void k_block_info(const int *__restrict__ chunk_off, const int *__restrict__ offset,
-                             const int *__restrict__ leaves, int nleaves, int4 *__restrict__ blk_info) {
-    const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nleaves) return;
-    const int q0 = offset[l], q1 = offset[l + 1], from = leaves[2 * l], cnt = leaves[2 * l + 1] - from;
-    for (int c = chunk_off[l], e = chunk_off[l + 1], qb = q0; c < e; ++c, qb += kMQ) {
-        blk_info[2 * c] = make_int4(l, qb, min(kMQ, q1 - qb), from);
-        blk_info[2 * c + 1] = make_int4(cnt, 0, 0, 0);
+                             const int *__restrict__ leaves, int nleaves, int nblk, int4 *__restrict__ blk_info) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nblk || c >= chunk_off[nleaves]) return;
+    int lo = 0, hi = nleaves - 1;
+    while (lo < hi) {
+        const int m = (lo + hi + 1) >> 1;
+        if (chunk_off[m] <= c) lo = m; else hi = m - 1;
     }
+    const int l = lo;
+    const int qb = offset[l] + (c - chunk_off[l]) * kMQ, from = leaves[2 * l];
+    blk_info[2 * c] = make_int4(l, qb, min(kMQ, offset[l + 1] - qb), from);
+    blk_info[2 * c + 1] = make_int4(leaves[2 * l + 1] - from, 0, 0, 0);
 }
 
 __global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
@@ -702,7 +708,7 @@ static size_t leafmm_blocks(int nq, int nleaves) { return (size_t)nleaves + ((si
 size_t leafmm_scratch_bytes(int nq, int nleaves) { return leafmm_blocks(nq, nleaves) * 2 * sizeof(int4); }
 
 hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_leaf_order, dim3(1u << t.depth), dim3(256), 0, st, t.leaves, t.orig, out);
+    hipLaunchKernelGGL(k_leaf_order, dim3(1u << t.depth), dim3(kLeafSortTPB), 0, st, t.leaves, t.orig, out);
     return hipGetLastError();
 }
 
@@ -720,8 +726,8 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
     if (L < K || L > 32) return hipErrorInvalidValue;
     const unsigned nblk = (unsigned)leafmm_blocks(nq, nleaves);
     int4 *blk_info = static_cast<int4 *>(scratch);
-    hipLaunchKernelGGL(k_block_info, dim3((unsigned)((nleaves + 255) / 256)), dim3(256), 0, st, chunk_off, offset,
-                       t.leaves, nleaves, blk_info);
+    hipLaunchKernelGGL(k_block_info, dim3((nblk + 255) / 256), dim3(256), 0, st, chunk_off, offset, t.leaves,
+                       nleaves, (int)nblk, blk_info);
     return with_dk_lm(leafmm_dk(p), L, [&](auto dk, auto lm) -> hipError_t {
         constexpr int DK = decltype(dk)::value;
         constexpr int LM = decltype(lm)::value;
