@@ -158,10 +158,13 @@ def test_field_256_matches_oracle(K, propagate, select, gpu_available):
     _field_vs_oracle(a, b, 7, 10, 32, K, propagate, select)
 
 
-def test_field_fp32_projection(gpu_available):
+@pytest.mark.parametrize("p,d", [(5, 12), (9, 24), (13, 8)])
+def test_field_fp32_projection(p, d, gpu_available):
+    """FP32 projection: the MFMA GEMM with one (d <= 16) and two (d > 16)
+    component tiles, and the CUDA-core kernel for p > 12."""
     from annfmp_amd import synth
     a, b = synth.make_pair(synth.small_spec(96, 128), seed=4)
-    _field_vs_oracle(a, b, 5, 12, 16, 3, projection="fp32")
+    _field_vs_oracle(a, b, p, d, 16, 3, projection="fp32")
 
 
 def test_field_1080p_matches_oracle(gpu_available):
