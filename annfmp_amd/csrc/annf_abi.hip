@@ -777,7 +777,8 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                          c->sort_tmp.p, sort_bytes, c->stream));
     c->mark(4);
     HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, n_a,
-                   nprobe, shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), c->stream));
+                   nprobe, shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), prop ? nullptr : knn_out,
+                   prop ? nullptr : nn_out, prop ? nullptr : dist_out, c->stream));
     c->mark(5);
     uint64_t *keys2 = nullptr;
     if (prop) {
@@ -792,9 +793,10 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                              c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
         HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
         HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info,
-                       (int)n_pairs, 1, shortlist, K, fp8, prop_k, keys2, c->stream));
+                       (int)n_pairs, 1, shortlist, K, fp8, prop_k, keys2, nullptr, nullptr, nullptr, c->stream));
     }
-    HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,
+    if (prop)
+        HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,
                         c->stream));
     c->mark(6);
     RCHK(sync_stream(c));

@@ -140,7 +140,8 @@ int leafmm_chunk();
 int leafmm_dk(int p);
 bool leafmm_supported(int p);
 // order holds output slots; slot s is A patch s / slot_div and receives its
-// K exact keys ((d2 << 32) | B index, ascending) in keys_out[s * K ...].
+// K exact keys ((d2 << 32) | B index, ascending) in keys_out[s * K ...], or,
+// with nn given, the field itself (knn, nn, dist as finish_field writes them).
 // scratch: leafmm_scratch_bytes(nq, nleaves) bytes (each block's work item).
 // pts_order: every leaf's points in ascending B index (leaf_point_order);
 // the BF16 form needs it, the FP8 form uses the tree order.
@@ -148,7 +149,7 @@ size_t leafmm_scratch_bytes(int nq, int nleaves);
 hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st);
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *pts_order, const int *order,
                    const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K,
-                   int fp8, int slot_div, uint64_t *keys_out, hipStream_t st);
+                   int fp8, int slot_div, uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st);
 // propagated (query, leaf) pairs: pairleaf[q * prop_k + k] (-1: none), per-leaf counts
 hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, const int *natleaf, const int *pli,
                       int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st);

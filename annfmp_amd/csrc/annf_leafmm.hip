@@ -32,6 +32,13 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
+// Where k_leaf_mm writes a single pass's result directly: knn (K per
+// query, optional), nn and dist (optional); nn == nullptr: keys_out instead.
+struct FieldOut {
+    int *knn, *nn;
+    float *dist;
+};
+
 constexpr int kMQ = 64;                // queries per block
 constexpr int kNP = 128;               // points per tile
 constexpr int kThreads = 256;
@@ -245,7 +252,8 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                                                          const int *__restrict__ order,
                                                          const int *__restrict__ chunk_off,
                                                          const int4 *__restrict__ blk_info, int nprobe, int K,
-                                                         int slot_div, uint64_t *__restrict__ keys_out) {
+                                                         int slot_div, uint64_t *__restrict__ keys_out,
+                                                         FieldOut fo) {
     constexpr int RB = DK + 4;                              // LDS patch stride (bytes, odd dword count)
     constexpr int kBD = (kNP * RB > kMQ * kDS * 4) ? kNP * RB : kMQ * kDS * 4;
     extern __shared__ __align__(16) uint8_t smem[];
@@ -530,7 +538,20 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
     __syncthreads();
     if (q < nqb) {
         const size_t o = (size_t)qid[q] * K;
-        for (int s = q4; s < K; s += 4) keys_out[o + s] = cand[q * LM + s];
+        if (fo.nn) {
+            // the final field (what k_finish writes from a single pass)
+            for (int s = q4; s < K; s += 4) {
+                const uint64_t key = cand[q * LM + s];
+                const int j = key == kNoKey ? -1 : (int)(uint32_t)key;
+                if (fo.knn) fo.knn[o + s] = j;
+                if (s == 0) {
+                    fo.nn[qid[q]] = j;
+                    if (fo.dist) fo.dist[qid[q]] = key == kNoKey ? kFltMax : sqrt_f((uint32_t)(key >> 32));
+                }
+            }
+        } else {
+            for (int s = q4; s < K; s += 4) keys_out[o + s] = cand[q * LM + s];
+        }
     }
 }
 
@@ -714,7 +735,7 @@ hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st) {
 
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *pts_order, const int *order,
                    const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K,
-                   int fp8, int slot_div, uint64_t *keys_out, hipStream_t st) {
+                   int fp8, int slot_div, uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st) {
     if (nq == 0) return hipSuccess;
     const int p = ga.rows;
     if (!leafmm_supported(p) || gb.rows != p || ga.rowlen != 3 * p || gb.rowlen != 3 * p) return hipErrorInvalidValue;
@@ -742,7 +763,7 @@ hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, 
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_leaf_mm<DK, LM, F8>), dim3(nblk), dim3(kThreads), lds, st, t, ga, gb,
                                F8 ? t.orig : pts_order, order,
-                               chunk_off, blk_info, nprobe, K, slot_div, keys_out);
+                               chunk_off, blk_info, nprobe, K, slot_div, keys_out, FieldOut{knn, nn, dist});
             return hipGetLastError();
         };
         return fp8 ? launch(std::true_type{}) : launch(std::false_type{});

@@ -281,7 +281,13 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
         // layer 1: acc[r] = unit 32 * wid + 8 * (r / 4) + 4 * lh + r % 4 of patch 32 * pt + lc
 #pragma unroll
         for (int pt = 0; pt < 2; ++pt) {
-            floatx16 acc = {};
+            floatx16 acc;   // starts at the bias
+#pragma unroll
+            for (int gq = 0; gq < 4; ++gq) {
+                const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + u0 + 8 * gq);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc[4 * gq + i] = bb[i];
+            }
             const __bf16 *xb = X + (size_t)(32 * pt + lc) * XS + 8 * lh;
 #pragma unroll
             for (int ks = 0; ks < KS1; ++ks)
@@ -290,16 +296,20 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
             __bf16 *hrow = H1 + (size_t)(32 * pt + lc) * kHS;
 #pragma unroll
             for (int gq = 0; gq < 4; ++gq) {
-                const int u = u0 + 8 * gq;
-                const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + u);
-                const floatx4 v = {gelu_fast(acc[4 * gq + 0] + bb[0]), gelu_fast(acc[4 * gq + 1] + bb[1]),
-                                   gelu_fast(acc[4 * gq + 2] + bb[2]), gelu_fast(acc[4 * gq + 3] + bb[3])};
-                *reinterpret_cast<bf16x4 *>(hrow + u) = __builtin_convertvector(v, bf16x4);
+                const floatx4 v = {gelu_fast(acc[4 * gq + 0]), gelu_fast(acc[4 * gq + 1]), gelu_fast(acc[4 * gq + 2]),
+                                   gelu_fast(acc[4 * gq + 3])};
+                *reinterpret_cast<bf16x4 *>(hrow + u0 + 8 * gq) = __builtin_convertvector(v, bf16x4);
             }
         }
         __syncthreads();
         // layer 2 (H2 over the X tile)
-        floatx16 acc2[2] = {};
+        floatx16 acc2[2];   // start at the bias
+#pragma unroll
+        for (int gq = 0; gq < 4; ++gq) {
+            const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + 128 + u0 + 8 * gq);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc2[0][4 * gq + i] = acc2[1][4 * gq + i] = bb[i];
+        }
 #pragma unroll
         for (int pt = 0; pt < 2; ++pt) {
             const __bf16 *hb = H1 + (32 * pt + lc) * kHS + 8 * lh;
@@ -313,11 +323,9 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
             __bf16 *hrow = X + (size_t)(32 * pt + lc) * kHS;
 #pragma unroll
             for (int gq = 0; gq < 4; ++gq) {
-                const int u = u0 + 8 * gq;
-                const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + 128 + u);
-                const floatx4 v = {gelu_fast(acc2[pt][4 * gq + 0] + bb[0]), gelu_fast(acc2[pt][4 * gq + 1] + bb[1]),
-                                   gelu_fast(acc2[pt][4 * gq + 2] + bb[2]), gelu_fast(acc2[pt][4 * gq + 3] + bb[3])};
-                *reinterpret_cast<bf16x4 *>(hrow + u) = __builtin_convertvector(v, bf16x4);
+                const floatx4 v = {gelu_fast(acc2[pt][4 * gq + 0]), gelu_fast(acc2[pt][4 * gq + 1]),
+                                   gelu_fast(acc2[pt][4 * gq + 2]), gelu_fast(acc2[pt][4 * gq + 3])};
+                *reinterpret_cast<bf16x4 *>(hrow + u0 + 8 * gq) = __builtin_convertvector(v, bf16x4);
             }
         }
         __syncthreads();
