@@ -61,6 +61,7 @@ struct BuildScratch {
     size_t hist_words;                // max over the levels of level_hist_words(nodes, cpn)
     size_t nodes_max;
     int2 *cnt, *cnt_off;              // total chunks of the widest level each
+    int *src_idx;                     // n: record position of each slot (level before the subtree)
     int2 *tot;                        // nodes_max
     int *bnd;                         // all levels' bounds
     int *err;                         // 1 int, non-zero if a kernel saw an impossible state

@@ -590,17 +590,21 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_eqoff(LevelArgs L, const uint
     }
 }
 
+// RECORDS = false (the level before k_subtree): only orig moves, and
+// src_out[slot] records the position of the slot's record, which the subtree
+// blocks gather directly.
 // The moving pass of the partition (the slots of k_lvl_eqoff), coalesced: the
 // block first computes every point's slot (points s + u*256 + tid, so the
 // segment-order ranks come from eight block scans), then copies the records
 // one float4 per lane.  With nbox, the same copy accumulates the two
 // children's data boxes (the next level's k_box), for DP/4 a power of two.
-template <int DP>
+template <int DP, bool RECORDS>
 __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
                                                     LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
                                                     const int2 *cnt_off, const int2 *tot, const int *eqoff,
                                                     float *__restrict__ pts_out, int *__restrict__ orig_out,
-                                                    const int *nax, float *split, int *err, uint32_t *nbox, int d) {
+                                                    const int *nax, float *split, int *err, uint32_t *nbox, int d,
+                                                    int *__restrict__ src_out) {
     constexpr int P4 = DP / 4;
     constexpr bool kBoxOk = (P4 & (P4 - 1)) == 0;
     __shared__ int2 wsum[kBuildTPB / kWave];
@@ -648,11 +652,13 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
                 slot = -1;
             } else {
                 orig_out[lo + slot] = orig_in[i];
+                if (!RECORDS) src_out[lo + slot] = i;
                 if (slot == half) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
             }
         }
         slots[u * kBuildTPB + threadIdx.x] = slot;
     }
+    if (!RECORDS) return;   // the records stay; src_out says where each slot's record is
     __syncthreads();
     const float4 *src4 = reinterpret_cast<const float4 *>(pts_in) + (size_t)s * P4;
     float4 *dst4 = reinterpret_cast<float4 *>(pts_out) + (size_t)lo * P4;
@@ -745,8 +751,8 @@ __device__ __forceinline__ int sub_of(const int *sb, int nsub, int i) {
 
 template <int DP>
 __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
-                                                     const int *__restrict__ bnd0, int L0, int R, int d,
-                                                     int splitting_type, float *__restrict__ pts_out,
+                                                     const int *__restrict__ src_idx, const int *__restrict__ bnd0,
+                                                     int L0, int R, int d, int splitting_type, float *__restrict__ pts_out,
                                                      int *__restrict__ orig_out, int *__restrict__ axis_out,
                                                      float *__restrict__ split_out, int *err) {
     constexpr int SM = sub_max<DP>();
@@ -771,10 +777,11 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
     const int node = blockIdx.x;
     const int lo = bnd0[node], c0 = bnd0[node + 1] - lo;
     {
-        const float4 *src = reinterpret_cast<const float4 *>(pts_in + (size_t)lo * DP);
+        // point i of the node is record src_idx[lo + i] (or lo + i)
         for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) {
             const int i = e / (DP / 4), j = 4 * (e - i * (DP / 4));
-            const float4 v = src[e];
+            const size_t g = src_idx ? (size_t)src_idx[lo + i] : (size_t)(lo + i);
+            const float4 v = reinterpret_cast<const float4 *>(pts_in + g * DP)[j / 4];
             dataf[at(i, j + 0)] = v.x;
             dataf[at(i, j + 1)] = v.y;
             dataf[at(i, j + 2)] = v.z;
@@ -1031,6 +1038,7 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
     // the moving pass computes the next level's boxes when DP / 4 is a power
     // of two; boxes alternate between box and box2
     const bool fused_box = splitting_type != 0 && (dpad == 4 || dpad == 8 || dpad == 16 || dpad == 32);
+    const int *src_idx = nullptr;   // set when the last level moved indices only
     for (int lvl = 0; lvl < L0; ++lvl) {
         LevelArgs L;
         L.nodes = 1 << lvl;
@@ -1054,6 +1062,8 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                                need_kbox ? box : (uint32_t *)nullptr, lvl == 0 ? H0 : (uint32_t *)nullptr, hw);
         }
         const int R = hist_rep(L.cpn);
+        // the last level before the subtree blocks moves indices, not records
+        const bool index_only = lvl + 1 == L0 && L0 < depth;
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
             if (need_kbox)
@@ -1069,18 +1079,24 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
             hipLaunchKernelGGL(k_lvl_count, g, b, 0, st, L, s.keys, H2, P2, R2, vkey, s.cnt);
             hipLaunchKernelGGL(k_lvl_eqoff<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, vkey, s.cnt,
                                s.cnt_off, s.tot, s.eqoff, H0, hw_next, s.err);
-            hipLaunchKernelGGL(k_move<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey, s.cnt_off,
-                               s.tot, s.eqoff, pout, oout, s.nax, split, s.err, nbox, d);
+            if (index_only)
+                hipLaunchKernelGGL((k_move<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey,
+                                   s.cnt_off, s.tot, s.eqoff, pout, oout, s.nax, split, s.err, (uint32_t *)nullptr, d,
+                                   s.src_idx);
+            else
+                hipLaunchKernelGGL((k_move<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey,
+                                   s.cnt_off, s.tot, s.eqoff, pout, oout, s.nax, split, s.err, nbox, d, (int *)nullptr);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
-        std::swap(pin, pout);
+        if (!index_only) std::swap(pin, pout);
         std::swap(oin, oout);
+        if (index_only) src_idx = s.src_idx;
     }
     if (L0 < depth) {
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
-            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin, oin,
+            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin, oin, src_idx,
                                s.bnd + host_bnd_off[L0], L0, depth - L0, d, splitting_type, pout, oout, axis, split,
                                s.err);
             return hipGetLastError();
