@@ -104,6 +104,8 @@ class ANNFieldPropKDTreeHIP:
         self.knn_indices = res["knn"]
         st = res["stages_ms"]
         self._timers["extracting_patches"] = 0.0   # fused into the projection
+        # A is projected on a second stream during the tree build, so its
+        # projection time is inside kdtree_fit
         self._timers["pca_apply"] = (st[0] + st[1]) / 1000.0
         self._timers["kdtree_fit"] = st[2] / 1000.0
         self._timers["remaining_rows"] = sum(st[3:]) / 1000.0

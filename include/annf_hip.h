@@ -222,10 +222,14 @@ int annf_exact_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                          int *out_d2);
 
 /* Per-stage device times (ms) of the last annf_field_dev /
- * annf_process_rows call, measured with HIP events on the handle's stream.
- * Stage order: 0 project A, 1 project B, 2 tree build, 3 natural leaves +
- * leaf scan, 4 row 0 exact search, 5 row propagation sweep, 6 conversion +
- * best-of-K.  Returns the number of stages written (<= max_stages). */
+ * annf_process_rows / annf_field_mm_dev call, measured with HIP events on the
+ * handle's main stream.  The query image A is projected on the handle's second
+ * stream while the tree over B is built, so stage order is: 0 project B,
+ * 1 launch of A's projection, 2 tree build (ends once A is projected too),
+ * then for annf_field_dev 3 natural leaves + leaf scan, 4 row 0 exact search,
+ * 5 row propagation sweep, 6 conversion + best-of-K, and for
+ * annf_field_mm_dev 3 natural leaves + bucketing, 4 leaf k-NN, 5 propagated
+ * pass + merge.  Returns the number of stages written (<= max_stages). */
 int annf_stage_times(annf_ctx *ctx, float *ms, int max_stages);
 /* Enables (1) or disables (0) the per-stage events. */
 int annf_set_timing(annf_ctx *ctx, int on);
