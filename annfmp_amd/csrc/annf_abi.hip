@@ -294,7 +294,7 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         RCHK(c->bcursor.ensure((size_t)nleaves * sizeof(int)));
         RCHK(c->bchunk.ensure(((size_t)nleaves + 1) * sizeof(int)));
         RCHK(c->order.ensure((size_t)n_a * sizeof(int)));
-        RCHK(c->sort_keys.ensure(2 * (size_t)n_a * sizeof(int)));
+        RCHK(c->sort_keys.ensure((size_t)n_a * sizeof(int)));
         const size_t sort_bytes = sort_scratch_bytes(n_a, nleaves);
         RCHK(c->sort_tmp.ensure(sort_bytes));
         int *natdesc = nullptr;
@@ -304,8 +304,8 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         }
         HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, natdesc, c->stream));
         HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafscan_chunk(), c->sort_keys.as<int>(),
-                             c->sort_keys.as<int>() + n_a, c->order.as<int>(), c->boffset.as<int>(),
-                             c->bcount.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
+                             c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p,
+                             sort_bytes, c->stream));
         if (propagate) {
             RCHK(c->p1d.ensure(nk * sizeof(float)));
             RCHK(c->p1p.ensure(nk * sizeof(int)));
@@ -761,7 +761,7 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     RCHK(c->order.ensure(std::max((size_t)n_a, n_pairs) * sizeof(int)));
     RCHK(c->mm_keys1.ensure((size_t)n_a * K * sizeof(uint64_t)));
     const size_t n_sort = std::max((size_t)n_a, n_pairs);
-    RCHK(c->sort_keys.ensure(2 * n_sort * sizeof(int)));
+    RCHK(c->sort_keys.ensure(n_sort * sizeof(int)));
     const size_t sort_bytes = sort_scratch_bytes((int)n_sort, nleaves);
     RCHK(c->sort_tmp.ensure(sort_bytes));
     int *skeys = c->sort_keys.as<int>();
@@ -774,9 +774,8 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         pts_order = c->mm_ptorder.as<int>();
     }
     HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, nullptr, c->stream));
-    HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafmm_chunk(), skeys, skeys + n_sort,
-                         c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(), c->bchunk.as<int>(),
-                         c->sort_tmp.p, sort_bytes, c->stream));
+    HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafmm_chunk(), skeys, c->order.as<int>(),
+                         c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
     c->mark(4);
     HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, n_a,
                    nprobe, shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), prop ? nullptr : knn_out,
@@ -791,8 +790,8 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, c->natleaf.as<int>(), t.pli, fs.nc_a,
                           (int)fs.n_b, c->mm_pairleaf.as<int>(), nullptr, c->stream));
         HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, leafmm_chunk(), skeys,
-                             skeys + n_sort, c->order.as<int>(), c->boffset.as<int>(), c->bcount.as<int>(),
-                             c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
+                             c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p,
+                             sort_bytes, c->stream));
         HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
         HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info,
                        (int)n_pairs, 1, shortlist, K, fp8, prop_k, keys2, nullptr, nullptr, nullptr, c->stream));

@@ -100,16 +100,13 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
 hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, int *count, int *desc,
                           hipStream_t st);
 int leafscan_chunk();
-// chunk_off[0..nleaves] of ceil(size / chunk) chunks per leaf (chunk_cnt: scratch)
-hipError_t chunk_offsets(const int *offset, int nleaves, int chunk, int *chunk_cnt, int *chunk_off, hipStream_t st);
 // Buckets n entries by leaf (leaf < 0: left out, after every bucket) with a
 // stable radix sort: order = entry indices by (leaf, index), offset[0..nleaves],
-// chunk_off as chunk_offsets.  keys_a / keys_b: n ints each; scratch of
-// sort_scratch_bytes(n, nleaves).
+// chunk_off[0..nleaves] = exclusive sum of ceil(size / chunk) over the leaves.
+// keys: n ints of scratch; scratch of sort_scratch_bytes(n, nleaves).
 size_t sort_scratch_bytes(int n, int nleaves);
-hipError_t bucket_sorted(const int *leaf, int n, int nleaves, int chunk, int *keys_a, int *keys_b, int *order,
-                         int *offset, int *chunk_cnt, int *chunk_off, void *scratch, size_t scratch_bytes,
-                         hipStream_t st);
+hipError_t bucket_sorted(const int *leaf, int n, int nleaves, int chunk, int *keys, int *order, int *offset,
+                         int *chunk_off, void *scratch, size_t scratch_bytes, hipStream_t st);
 hipError_t leaf_scan(const TreeView &t, const float *q, int nq, const int *order, const int *offset,
                      const int *chunk_off, int K, const int *propleaf, float *out_d, int *out_p,
                      int *out_l, hipStream_t st);

@@ -1187,51 +1187,11 @@ hipError_t natural_leaves(const TreeView &t, const float *q, int nq, int *leaf, 
     });
 }
 
-// Exclusive scan of n counts into offset[0..n] with one 1024-thread block.
-__global__ __launch_bounds__(1024) void k_scan_single(const int *count, int n, int *offset, int *cursor) {
-    __shared__ int wsum[1024 / kWave];
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    const int per = (n + blockDim.x - 1) / blockDim.x;
-    const int b = threadIdx.x * per;
-    int loc = 0;
-    for (int k = 0; k < per && b + k < n; ++k) loc += count[b + k];
-    int inc = loc;
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        int t = __shfl_up(inc, o, kWave);
-        if (lane >= o) inc += t;
-    }
-    if (lane == kWave - 1) wsum[wid] = inc;
-    __syncthreads();
-    int off = 0;
-    for (int w = 0; w < wid; ++w) off += wsum[w];
-    int run = inc - loc + off;
-    for (int k = 0; k < per && b + k < n; ++k) {
-        const int c = count[b + k];
-        offset[b + k] = run;
-        if (cursor) cursor[b + k] = run;
-        run += c;
-    }
-    if (threadIdx.x == blockDim.x - 1) offset[n] = run;
-}
-
 // Blocks of the leaf scan: a leaf's bucket is cut into chunks of kScanChunk
 // queries, so one crowded leaf (a flat image region) spreads over many blocks.
 constexpr int kScanChunk = 256;
 
-__global__ void k_chunk_count(const int *offset, int nleaves, int chunk, int *chunks) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l < nleaves) chunks[l] = (offset[l + 1] - offset[l] + chunk - 1) / chunk;
-}
-
 int leafscan_chunk() { return kScanChunk; }
-
-hipError_t chunk_offsets(const int *offset, int nleaves, int chunk, int *chunk_cnt, int *chunk_off, hipStream_t st) {
-    hipLaunchKernelGGL(k_chunk_count, dim3(blocks_for(nleaves, 256)), dim3(256), 0, st, offset, nleaves, chunk,
-                       chunk_cnt);
-    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(1024), 0, st, chunk_cnt, nleaves, chunk_off, (int *)nullptr);
-    return hipGetLastError();
-}
 
 // Sorted k-list of runtime length K <= KM held in registers.  The K live
 // entries occupy the top slots KM-K..KM-1; the front slots hold -inf, which

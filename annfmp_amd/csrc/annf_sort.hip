@@ -1,25 +1,34 @@
 // Bucketing by leaf as a stable radix sort of (leaf, index) pairs (rocPRIM's
 // device radix sort): the order within a leaf is the index order, and the
 // per-leaf offsets are read off the sorted keys -- no counting atomics.
+#include <algorithm>
 #include <cstring>
 
 #include "annf_internal.h"
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/counting_iterator.hpp>
+#include <rocprim/iterator/transform_iterator.hpp>
 
 namespace annf {
 
 namespace {
 
-// keys for the sort: leaf, or nleaves for an entry without one (leaf < 0)
-__global__ void k_sort_keys(const int *__restrict__ leaf, int n, int nleaves, unsigned *__restrict__ key) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int l = leaf[i];
-        key[i] = (unsigned)(l < 0 ? nleaves : l);
+// the sort key of an entry: its leaf, or nleaves without one (leaf < 0)
+struct LeafKey {
+    unsigned nleaves;
+    __host__ __device__ unsigned operator()(int l) const { return l < 0 ? nleaves : (unsigned)l; }
+};
+
+// chunks of `chunk` entries in leaf l (0 for l = nleaves, the scan's total)
+struct LeafChunks {
+    const int *offset;
+    int nleaves, chunk;
+    __host__ __device__ int operator()(int l) const {
+        return l < nleaves ? (offset[l + 1] - offset[l] + chunk - 1) / chunk : 0;
     }
-}
+};
 
 // offset[l] = first sorted position with key >= l, for l = 0..nleaves
 __global__ void k_offsets_from_sorted(const unsigned *__restrict__ key, int n, int nleaves, int *__restrict__ offset) {
@@ -45,23 +54,26 @@ int key_bits(int nleaves) {
 }  // namespace
 
 size_t sort_scratch_bytes(int n, int nleaves) {
-    size_t bytes = 0;
-    rocprim::radix_sort_pairs(nullptr, bytes, (const unsigned *)nullptr, (unsigned *)nullptr, (const int *)nullptr,
-                              (int *)nullptr, (unsigned int)n, 0u, (unsigned int)key_bits(nleaves));
-    return bytes;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    rocprim::radix_sort_pairs(nullptr, sort_bytes,
+                              rocprim::make_transform_iterator((const int *)nullptr, LeafKey{(unsigned)nleaves}),
+                              (unsigned *)nullptr, rocprim::counting_iterator<int>(0), (int *)nullptr,
+                              (unsigned int)n, 0u, (unsigned int)key_bits(nleaves));
+    rocprim::exclusive_scan(nullptr, scan_bytes,
+                            rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0),
+                                                             LeafChunks{nullptr, nleaves, 1}),
+                            (int *)nullptr, 0, (size_t)nleaves + 1, rocprim::plus<int>());
+    return std::max(sort_bytes, scan_bytes);
 }
 
-hipError_t bucket_sorted(const int *leaf, int n, int nleaves, int chunk, int *keys_a_, int *keys_b_, int *order,
-                         int *offset, int *chunk_cnt, int *chunk_off, void *scratch, size_t scratch_bytes,
-                         hipStream_t st) {
-    unsigned *keys_a = reinterpret_cast<unsigned *>(keys_a_);
-    unsigned *keys_b = reinterpret_cast<unsigned *>(keys_b_);
+hipError_t bucket_sorted(const int *leaf, int n, int nleaves, int chunk, int *keys, int *order, int *offset,
+                         int *chunk_off, void *scratch, size_t scratch_bytes, hipStream_t st) {
+    unsigned *keys_b = reinterpret_cast<unsigned *>(keys);
     if (n > 0) {
         if (sort_scratch_bytes(n, nleaves) > scratch_bytes) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_sort_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, leaf, n, nleaves, keys_a);
-        hipError_t e = rocprim::radix_sort_pairs(scratch, scratch_bytes, keys_a, keys_b,
-                                                 rocprim::counting_iterator<int>(0), order, (unsigned int)n, 0u,
-                                                 (unsigned int)key_bits(nleaves), st);
+        hipError_t e = rocprim::radix_sort_pairs(
+            scratch, scratch_bytes, rocprim::make_transform_iterator(leaf, LeafKey{(unsigned)nleaves}), keys_b,
+            rocprim::counting_iterator<int>(0), order, (unsigned int)n, 0u, (unsigned int)key_bits(nleaves), st);
         if (e != hipSuccess) return e;
     }
     const int m = n > 0 ? n : nleaves + 1;
@@ -69,7 +81,12 @@ hipError_t bucket_sorted(const int *leaf, int n, int nleaves, int chunk, int *ke
                        offset);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return chunk_offsets(offset, nleaves, chunk, chunk_cnt, chunk_off, st);
+    // chunk offsets: exclusive sum of the chunks per leaf, nleaves + 1 entries
+    size_t bytes = scratch_bytes;
+    return rocprim::exclusive_scan(scratch, bytes,
+                                   rocprim::make_transform_iterator(rocprim::counting_iterator<int>(0),
+                                                                    LeafChunks{offset, nleaves, chunk}),
+                                   chunk_off, 0, (size_t)nleaves + 1, rocprim::plus<int>(), st);
 }
 
 }  // namespace annf
