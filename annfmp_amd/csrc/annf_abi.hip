@@ -213,8 +213,10 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     RCHK(c->pli.ensure((size_t)n * sizeof(int)));
     RCHK(c->keys.ensure((size_t)n * sizeof(uint32_t)));
     RCHK(c->eqoff.ensure((size_t)n * sizeof(int)));
-    RCHK(c->box.ensure(2 * nodes_max * dpad * 2 * sizeof(uint32_t)));
-    RCHK(c->box2.ensure(2 * nodes_max * dpad * 2 * sizeof(uint32_t)));
+    size_t box_words = (size_t)dpad * 2;
+    for (int l = 0; l < depth; ++l) box_words = std::max(box_words, level_box_words(1 << l, cpn[l], dpad));
+    RCHK(c->box.ensure(box_words * sizeof(uint32_t)));
+    RCHK(c->box2.ensure(box_words * sizeof(uint32_t)));
     RCHK(c->nax.ensure(nodes_max * sizeof(int)));
     size_t hist_words = 1;
     for (int l = 0; l < depth; ++l) hist_words = std::max(hist_words, level_hist_words(1 << l, cpn[l]));

@@ -53,7 +53,7 @@ struct BuildScratch {
     int *orig_a, *orig_b;             // n each
     uint32_t *keys;                   // n
     int *eqoff;                       // n
-    uint32_t *box, *box2;             // 2 * nodes_max * dpad * 2 each (this / next level)
+    uint32_t *box, *box2;             // the largest level_box_words each (this / next level)
     int *nax;                         // nodes_max
     uint32_t *prefix;                 // 3 * nodes_max (two radix prefixes, the split key)
     int *rank;                        // 2 * nodes_max
@@ -90,6 +90,8 @@ hipError_t iota(int *dst, int n, hipStream_t st);
 // host_maxc[l]: the largest node population of level l.
 // words of one radix-pass histogram of a level (copies included)
 size_t level_hist_words(int nodes, int cpn);
+// words of a level's node boxes (copies included)
+size_t level_box_words(int nodes, int cpn, int dpad);
 hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
                       const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,

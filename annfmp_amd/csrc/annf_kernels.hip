@@ -242,7 +242,14 @@ struct LevelArgs {
     const int *bnd;   // nodes + 1 segment bounds
     int node0;        // heap index of the level's first node
     int lvl;
+    int rbox;         // copies of each node's box at this level (block chunk % rbox adds into one)
+    int rbox_next;    // the same for the next level's boxes (filled by k_move)
 };
+
+// word of box copy `rep` of node `node`, dimension j, bound b (0 min, 1 max)
+__device__ __forceinline__ size_t box_at(int node, int rbox, int rep, int dp, int j, int b) {
+    return (((size_t)node * rbox + rep) * dp + j) * 2 + b;
+}
 
 __device__ __forceinline__ void chunk_range(const LevelArgs &L, int &node, int &lo, int &hi, int &s, int &e) {
     node = blockIdx.x / L.cpn;
@@ -266,12 +273,14 @@ __host__ __device__ constexpr int pass_bins(int p) { return p == 2 ? 1024 : 2048
 
 size_t level_hist_words(int nodes, int cpn) { return (size_t)nodes * hist_rep(cpn) * kRBins; }
 
+size_t level_box_words(int nodes, int cpn, int dpad) { return (size_t)nodes * hist_rep(cpn) * dpad * 2; }
+
 // Level set-up where the level kernels do not cover it: zero the level's
 // first-pass histograms (the first level; later ones are cleared by the
 // previous level) and reset the node boxes k_box accumulates into.
 __global__ void k_lvl_prep(LevelArgs L, int dpad, uint32_t *box, uint32_t *H0, size_t hwords) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (box && i < (size_t)L.nodes * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
+    if (box && i < (size_t)L.nodes * L.rbox * dpad) { box[2 * i] = 0xFFFFFFFFu; box[2 * i + 1] = 0u; }
     if (H0 && i < hwords) H0[i] = 0u;
 }
 
@@ -390,8 +399,9 @@ __global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts
     }
     __syncthreads();
     if ((int)threadIdx.x < d) {
-        atomicMin(&box[((size_t)node * DP + threadIdx.x) * 2 + 0], smn[threadIdx.x]);
-        atomicMax(&box[((size_t)node * DP + threadIdx.x) * 2 + 1], smx[threadIdx.x]);
+        const int rep = (blockIdx.x - node * L.cpn) % L.rbox;
+        atomicMin(&box[box_at(node, L.rbox, rep, DP, threadIdx.x, 0)], smn[threadIdx.x]);
+        atomicMax(&box[box_at(node, L.rbox, rep, DP, threadIdx.x, 1)], smx[threadIdx.x]);
     }
 }
 
@@ -399,16 +409,25 @@ __global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts
 // strictly largest FP32 width max - min, starting from 0.0.
 __device__ __forceinline__ int node_axis(const LevelArgs &L, int node, int d, int dpad, const uint32_t *box,
                                          int splitting_type) {
+    // called by a whole wavefront: lane l takes dimension l % 32 (d <= 32)
+    // and every other copy of the box, the halves combined by one shuffle
     if (splitting_type == 0) return L.lvl % d;
-    int ax = 0;
-    float longest = 0.0f;
-    for (int k = 0; k < d; ++k) {
-        const float mn = key2f(box[((size_t)node * dpad + k) * 2 + 0]);
-        const float mx = key2f(box[((size_t)node * dpad + k) * 2 + 1]);
-        const float w = mx - mn;
-        if (w > longest) { longest = w; ax = k; }
-    }
-    return ax;
+    const int lane = threadIdx.x & (kWave - 1), k = lane & 31;
+    uint32_t kmn = 0xFFFFFFFFu, kmx = 0u;
+    if (k < d)
+        for (int rep = lane >> 5; rep < L.rbox; rep += 2) {
+            kmn = min(kmn, box[box_at(node, L.rbox, rep, dpad, k, 0)]);
+            kmx = max(kmx, box[box_at(node, L.rbox, rep, dpad, k, 1)]);
+        }
+    kmn = min(kmn, (uint32_t)__shfl_xor((int)kmn, 32, kWave));
+    kmx = max(kmx, (uint32_t)__shfl_xor((int)kmx, 32, kWave));
+    // the first strictly largest FP32 width, starting from 0.0
+    const float w = k < d ? key2f(kmx) - key2f(kmn) : 0.0f;
+    float wmax = w;
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, m, kWave));
+    const unsigned long long hit = __ballot(k < d && lane < 32 && w == wmax && w > 0.0f);
+    return hit ? __ffsll(hit) - 1 : 0;
 }
 
 // First pass of a level: the node's split axis, the split-axis keys and the
@@ -429,7 +448,7 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict
             H2[i] = 0u;
         }
         if (nbox)
-            for (size_t i = gt; i < (size_t)2 * L.nodes * dpad; i += gn) {
+            for (size_t i = gt; i < (size_t)2 * L.nodes * L.rbox_next * dpad; i += gn) {
                 nbox[2 * i] = 0xFFFFFFFFu;
                 nbox[2 * i + 1] = 0u;
             }
@@ -438,12 +457,14 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict
     chunk_range(L, node, lo, hi, s, e);
     if (s >= e) return;
     for (int i = threadIdx.x; i < kRBins; i += kBuildTPB) h[i] = 0u;
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kWave) {
         const int ax = node_axis(L, node, d, dpad, box, splitting_type);
-        sax = ax;
-        if (s == lo) {
-            nax[node] = ax;
-            axis_out[L.node0 + node] = ax;
+        if (threadIdx.x == 0) {
+            sax = ax;
+            if (s == lo) {
+                nax[node] = ax;
+                axis_out[L.node0 + node] = ax;
+            }
         }
     }
     __syncthreads();
@@ -713,9 +734,9 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
     __syncthreads();
     if ((int)threadIdx.x < 2 * d) {
         const int ch = threadIdx.x / d, j = threadIdx.x - ch * d;
-        const size_t o = ((size_t)(2 * node + ch) * DP + j) * 2;
+        const size_t o = box_at(2 * node + ch, L.rbox_next, blockIdx.x % L.rbox_next, DP, j, 0);
         atomicMin(&nbox[o], sbox[ch][j][0]);
-        atomicMax(&nbox[o + 1], sbox[ch][j][1]);
+        atomicMax(&nbox[o + 1], sbox[ch][j][1]);   // copy blockIdx.x % rbox_next
     }
 }
 
@@ -1046,6 +1067,8 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
         L.bnd = s.bnd + host_bnd_off[lvl];
         L.node0 = L.nodes - 1;
         L.lvl = lvl;
+        L.rbox = hist_rep(L.cpn);
+        L.rbox_next = lvl + 1 < L0 ? hist_rep(host_cpn[lvl + 1]) : 1;
         const unsigned nblk = (unsigned)(L.nodes * L.cpn);
         uint32_t *box = (lvl & 1) ? s.box2 : s.box;
         uint32_t *nbox = (lvl & 1) ? s.box : s.box2;
@@ -1057,7 +1080,7 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
         uint32_t *P1 = s.prefix, *P2 = s.prefix + s.nodes_max, *vkey = s.prefix + 2 * s.nodes_max;
         int *R1 = s.rank, *R2 = s.rank + s.nodes_max;
         if (lvl == 0 || need_kbox) {
-            const size_t m = std::max((size_t)L.nodes * dpad, lvl == 0 ? hw : 0);
+            const size_t m = std::max((size_t)L.nodes * L.rbox * dpad, lvl == 0 ? hw : 0);
             hipLaunchKernelGGL(k_lvl_prep, dim3(blocks_for((long long)m, 256)), dim3(256), 0, st, L, dpad,
                                need_kbox ? box : (uint32_t *)nullptr, lvl == 0 ? H0 : (uint32_t *)nullptr, hw);
         }
