@@ -231,7 +231,6 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     RCHK(c->errflag.ensure(sizeof(int)));
     HIPCHK(hipMemsetAsync(c->errflag.p, 0, sizeof(int), c->stream));
     HIPCHK(hipMemcpyAsync(c->bnd.p, bnd.data(), bnd.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(iota(c->orig_a.as<int>(), n, c->stream));
     BuildScratch s;
     s.pts_a = c->pts_a.as<float>();
     s.pts_b = c->pts_b.as<float>();

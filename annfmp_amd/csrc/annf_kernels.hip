@@ -672,7 +672,7 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
                 atomicOr(err, 1);
                 slot = -1;
             } else {
-                orig_out[lo + slot] = orig_in[i];
+                orig_out[lo + slot] = orig_in ? orig_in[i] : i;
                 if (!RECORDS) src_out[lo + slot] = i;
                 if (slot == half) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
             }
@@ -775,7 +775,8 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
                                                      const int *__restrict__ src_idx, const int *__restrict__ bnd0,
                                                      int L0, int R, int d, int splitting_type, float *__restrict__ pts_out,
                                                      int *__restrict__ orig_out, int *__restrict__ axis_out,
-                                                     float *__restrict__ split_out, int *err) {
+                                                     float *__restrict__ split_out, int *err,
+                                                     int *__restrict__ leaves, int *__restrict__ pli) {
     constexpr int SM = sub_max<DP>();
     constexpr int EPT = (SM + kSubTPB - 1) / kSubTPB;   // points per thread in the partition scan
     // the node's points, level-L0 order; dimension j of point p is stored at
@@ -1026,7 +1027,27 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         const int pi = perm[cur][i];
         dst[e] = make_float4(dataf[at(pi, j + 0)], dataf[at(pi, j + 1)], dataf[at(pi, j + 2)], dataf[at(pi, j + 3)]);
     }
-    for (int i = tid; i < c0; i += kSubTPB) orig_out[lo + i] = orig_in[lo + perm[cur][i]];
+    // final order, and (when asked) the leaf bounds and point -> leaf map:
+    // sub-node s2 of the last level done splits at c/2 into leaves
+    // (node << R) + 2 s2 and + 1 (kdtree.c:86-100, 114-115)
+    const int nlast = 1 << (R - 1);
+    for (int i = tid; i < c0; i += kSubTPB) {
+        const int src = lo + perm[cur][i];
+        const int o = orig_in ? orig_in[src] : src;
+        orig_out[lo + i] = o;
+        if (pli) {
+            const int s2 = sub_of(sb, nlast, i);
+            const int half = (sb[s2 + 1] - sb[s2]) / 2;
+            pli[o] = (node << R) + 2 * s2 + (i - sb[s2] >= half ? 1 : 0);
+        }
+    }
+    if (leaves && tid < nlast) {
+        const int l0 = (node << R) + 2 * tid, mid = sb[tid] + (sb[tid + 1] - sb[tid]) / 2;
+        leaves[2 * l0] = lo + sb[tid];
+        leaves[2 * l0 + 1] = lo + mid;
+        leaves[2 * l0 + 2] = lo + mid;
+        leaves[2 * l0 + 3] = lo + sb[tid + 1];
+    }
 }
 
 // Leaf bounds and point -> leaf map (kdtree.c:86-100).
@@ -1060,6 +1081,7 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
     // of two; boxes alternate between box and box2
     const bool fused_box = splitting_type != 0 && (dpad == 4 || dpad == 8 || dpad == 16 || dpad == 32);
     const int *src_idx = nullptr;   // set when the last level moved indices only
+    bool orig_identity = true;      // orig is still 0..n-1 (read as the index itself)
     for (int lvl = 0; lvl < L0; ++lvl) {
         LevelArgs L;
         L.nodes = 1 << lvl;
@@ -1102,35 +1124,45 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
             hipLaunchKernelGGL(k_lvl_count, g, b, 0, st, L, s.keys, H2, P2, R2, vkey, s.cnt);
             hipLaunchKernelGGL(k_lvl_eqoff<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, vkey, s.cnt,
                                s.cnt_off, s.tot, s.eqoff, H0, hw_next, s.err);
+            const int *oin_k = orig_identity ? (const int *)nullptr : oin;
             if (index_only)
-                hipLaunchKernelGGL((k_move<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey,
+                hipLaunchKernelGGL((k_move<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin_k, L, s.keys, vkey,
                                    s.cnt_off, s.tot, s.eqoff, pout, oout, s.nax, split, s.err, (uint32_t *)nullptr, d,
                                    s.src_idx);
             else
-                hipLaunchKernelGGL((k_move<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin, L, s.keys, vkey,
+                hipLaunchKernelGGL((k_move<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin_k, L, s.keys, vkey,
                                    s.cnt_off, s.tot, s.eqoff, pout, oout, s.nax, split, s.err, nbox, d, (int *)nullptr);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
         if (!index_only) std::swap(pin, pout);
         std::swap(oin, oout);
+        orig_identity = false;
         if (index_only) src_idx = s.src_idx;
     }
     if (L0 < depth) {
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
-            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin, oin, src_idx,
-                               s.bnd + host_bnd_off[L0], L0, depth - L0, d, splitting_type, pout, oout, axis, split,
-                               s.err);
+            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin,
+                               orig_identity ? (const int *)nullptr : oin, src_idx, s.bnd + host_bnd_off[L0], L0,
+                               depth - L0, d, splitting_type, pout, oout, axis, split, s.err, leaves, pli);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
         std::swap(pin, pout);
         std::swap(oin, oout);
+        orig_identity = false;
+    } else {
+        // no subtree pass (or no level at all): leaf bounds and point -> leaf map here
+        if (orig_identity) {
+            err = iota(oin, n, st);
+            if (err != hipSuccess) return err;
+            orig_identity = false;
+        }
+        const int nleaves = 1 << depth;
+        hipLaunchKernelGGL(k_leaves, dim3(blocks_for(std::max(n, nleaves), 256)), dim3(256), 0, st,
+                           s.bnd + host_bnd_off[depth], nleaves, oin, n, leaves, pli);
     }
-    const int nleaves = 1 << depth;
-    hipLaunchKernelGGL(k_leaves, dim3(blocks_for(std::max(n, nleaves), 256)), dim3(256), 0, st,
-                       s.bnd + host_bnd_off[depth], nleaves, oin, n, leaves, pli);
     err = hipGetLastError();
     t.n = n;
     t.d = d;
