@@ -23,7 +23,7 @@ MAX_D = 32
 EXPORTS = (
     "annf_create", "annf_destroy", "annf_last_error", "annf_stream", "annf_init", "annf_fit", "annf_fit_dev",
     "annf_neighbors", "annf_process_rows", "annf_norm_array", "annf_free_resources", "annf_tree_info",
-    "annf_tree_export", "annf_field_dev", "annf_stage_times", "annf_set_timing", "annf_exact_field_dev",
+    "annf_tree_export", "annf_tree_import", "annf_field_dev", "annf_stage_times", "annf_set_timing", "annf_exact_field_dev",
     "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
     "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev",
 )
@@ -75,6 +75,7 @@ def _declare(L):
     L.annf_free_resources.argtypes = [_vp]
     L.annf_tree_info.argtypes = [_vp, _ip, _ip, _ip]
     L.annf_tree_export.argtypes = [_vp, _ip, _fp, _ip, _ip, _fp, _ip]
+    L.annf_tree_import.argtypes = [_vp, i, i, i, _ip, _fp, _ip, _ip, _fp, _ip]
     L.annf_field_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, _vp, i, i, i, i, i, i, _vp, _vp, _vp, _vp, _vp]
     L.annf_stage_times.argtypes = [_vp, _fp, i]
     L.annf_set_timing.argtypes = [_vp, i]

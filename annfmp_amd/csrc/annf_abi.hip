@@ -567,6 +567,70 @@ int annf_tree_export(annf_ctx *c, int *axis, float *split, int *leaves, int *xi_
     return sync_stream(c);
 }
 
+int annf_tree_import(annf_ctx *c, int n, int d, int depth, const int *axis, const float *split, const int *leaves,
+                     const int *xi_orig, const float *xi_pts, const int *pli) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!axis || !split || !leaves || !xi_orig || !xi_pts || !pli) return fail(ANNF_EINVAL, "null tree array");
+    RCHK(set_device(c));
+    RCHK(check_tree_params(c, n, d, depth));
+    if (depth != c->tree_depth)
+        return fail(ANNF_EINVAL, "tree depth %d differs from the handle's depth %d", depth, c->tree_depth);
+    c->has_tree = false;
+    // the leaf ranges are fixed by n and depth (the left child of c points gets c/2)
+    std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn, &maxc = c->h_maxc;
+    int max_leaf = 0;
+    level_bounds(n, depth, bnd, off, cpn, maxc, max_leaf);
+    const int nleaf = 1 << depth;
+    for (int l = 0; l < nleaf; ++l)
+        if (leaves[2 * l] != bnd[off[depth] + l] || leaves[2 * l + 1] != bnd[off[depth] + l + 1])
+            return fail(ANNF_EINVAL, "leaf %d range [%d, %d) is not the median split of %d points", l,
+                        leaves[2 * l], leaves[2 * l + 1], n);
+    const size_t nn = ((size_t)1 << depth) - 1;
+    for (size_t i = 0; i < nn; ++i)
+        if (axis[i] < 0 || axis[i] >= d) return fail(ANNF_EINVAL, "node %zu splits on axis %d", i, axis[i]);
+    std::vector<char> seen(n, 0);
+    for (int l = 0; l < nleaf; ++l)
+        for (int i = leaves[2 * l]; i < leaves[2 * l + 1]; ++i) {
+            const int o = xi_orig[i];
+            if (o < 0 || o >= n || seen[o]) return fail(ANNF_EINVAL, "xi_orig is not a permutation of [0, %d)", n);
+            seen[o] = 1;
+            if (pli[o] != l) return fail(ANNF_EINVAL, "pli[%d] = %d, the point lies in leaf %d", o, pli[o], l);
+        }
+    const int dpad = dpad_of(d);
+    RCHK(c->pts_a.ensure((size_t)n * dpad * sizeof(float)));
+    RCHK(c->orig_a.ensure((size_t)n * sizeof(int)));
+    RCHK(c->axis.ensure(std::max<size_t>(nn, 1) * sizeof(int)));
+    RCHK(c->split.ensure(std::max<size_t>(nn, 1) * sizeof(float)));
+    RCHK(c->leaves.ensure((size_t)2 * nleaf * sizeof(int)));
+    RCHK(c->pli.ensure((size_t)n * sizeof(int)));
+    HIPCHK(hipMemsetAsync(c->pts_a.p, 0, (size_t)n * dpad * sizeof(float), c->stream));
+    HIPCHK(hipMemcpy2DAsync(c->pts_a.p, (size_t)dpad * sizeof(float), xi_pts, (size_t)d * sizeof(float),
+                            (size_t)d * sizeof(float), (size_t)n, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->orig_a.p, xi_orig, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (nn) {
+        HIPCHK(hipMemcpyAsync(c->axis.p, axis, nn * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->split.p, split, nn * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(c->leaves.p, leaves, (size_t)2 * nleaf * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->pli.p, pli, (size_t)n * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    RCHK(sync_stream(c));
+    TreeView t;
+    t.n = n;
+    t.d = d;
+    t.dpad = dpad;
+    t.depth = depth;
+    t.max_leaf = max_leaf;
+    t.pts = c->pts_a.as<float>();
+    t.orig = c->orig_a.as<int>();
+    t.axis = c->axis.as<int>();
+    t.split = c->split.as<float>();
+    t.leaves = c->leaves.as<int>();
+    t.pli = c->pli.as<int>();
+    c->tree = t;
+    c->has_tree = true;
+    return ANNF_OK;
+}
+
 }  // extern "C"
 
 namespace {

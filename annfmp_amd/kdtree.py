@@ -165,6 +165,38 @@ class KDTree(object):
         out["depth"] = depth
         return out
 
+    def save(self, path):
+        """Writes the fitted tree and the constructor parameters to ``path``
+        (the ``annfmp_amd.kdtree`` version-1 ``.npz`` of ``formats``)."""
+        from . import formats
+        params = self.get_params()
+        params["device"] = self.device
+        formats.save_tree(path, self.tree_arrays(), params)
+
+    @classmethod
+    def load(cls, path, device=None):
+        """A KDTree restored from ``save``'s file, ready for ``kneighbors``
+        without refitting; ``X`` is rebuilt from the points in tree order."""
+        from . import formats
+        arr = formats.load_tree(path)
+        p = dict(arr["params"])
+        if device is not None:
+            p["device"] = device
+        known = ("n_neighbors", "leaf_size", "max_leaves", "tree_depth", "float_type", "splitting_type",
+                 "n_jobs", "verbose", "device")
+        t = cls(**{k: p[k] for k in known if k in p})
+        t._check_float_type()
+        h = t._init_native(arr["depth"])
+        check("annf_tree_import", lib().annf_tree_import(
+            h.h, arr["n"], arr["d"], arr["depth"], ptr(arr["axis"], _lib._ip), ptr(arr["split"], _lib._fp),
+            ptr(arr["leaves"], _lib._ip), ptr(arr["xi_orig"], _lib._ip), ptr(arr["xi_pts"], _lib._fp),
+            ptr(arr["pli"], _lib._ip)))
+        X = np.empty_like(arr["xi_pts"])
+        X[arr["xi_orig"]] = arr["xi_pts"]
+        t.X = X
+        t._final_tree_depth = arr["depth"]
+        return t
+
     def _compute_final_tree_depth(self, n, leaf_size, tree_depth):
         """kdtree/base.py:260-300 (same float log semantics)."""
         d = int(math.floor(math.log(n, 2)))

@@ -102,6 +102,17 @@ int annf_free_resources(annf_ctx *ctx);
 int annf_tree_info(annf_ctx *ctx, int *n, int *d, int *depth);
 int annf_tree_export(annf_ctx *ctx, int *axis, float *split, int *leaves,
                      int *xi_orig, float *xi_pts, int *pli);
+/* The inverse of annf_tree_export (no reference counterpart: the reference
+ * refits on every process start, kdtree/base.py:92-153).  Restores a tree
+ * for annf_neighbors on a handle set up by annf_init with the same depth.
+ * Host arrays, every one required, sizes as above.  Checked before anything
+ * is uploaded: the leaf ranges must be the median split of n points over
+ * depth levels, xi_orig a permutation of [0, n), pli the leaf of each
+ * original index, every axis in [0, d) (ANNF_EINVAL otherwise; a failed
+ * import leaves the handle without a tree). */
+int annf_tree_import(annf_ctx *ctx, int n, int d, int depth, const int *axis,
+                     const float *split, const int *leaves, const int *xi_orig,
+                     const float *xi_pts, const int *pli);
 
 #define ANNF_PROJ_PCA32 0
 #define ANNF_PROJ_PCA64 1

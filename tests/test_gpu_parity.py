@@ -167,6 +167,15 @@ def test_field_fp32_projection(p, d, gpu_available):
     _field_vs_oracle(a, b, p, d, 16, 3, projection="fp32")
 
 
+@pytest.mark.parametrize("d", [16, 32])
+def test_field_example_photographs_match_oracle(d, gpu_available):
+    """Real photographs: a 160 x 240 crop of the reference's example pair,
+    at its example's K = 8, p = 8, leaf 128 (examples/simple.py:16-28); its
+    dim_reduced = 40 exceeds this build's MAX_D, so d = 16 and 32."""
+    g = golden("example_crop.npz")
+    _field_vs_oracle(g["a"], g["b"], 8, d, 128, 8)
+
+
 def test_field_1080p_matches_oracle(gpu_available):
     """The 1920x1080 workload (p=11, d=16, leaf 64, K=5), full size."""
     from annfmp_amd import synth
