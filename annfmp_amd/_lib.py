@@ -23,7 +23,8 @@ MAX_D = 32
 EXPORTS = (
     "annf_create", "annf_destroy", "annf_last_error", "annf_stream", "annf_init", "annf_fit", "annf_fit_dev",
     "annf_neighbors", "annf_process_rows", "annf_norm_array", "annf_free_resources", "annf_tree_info",
-    "annf_tree_export", "annf_tree_import", "annf_field_dev", "annf_stage_times", "annf_set_timing", "annf_exact_field_dev",
+    "annf_tree_export", "annf_tree_import", "annf_field_dev", "annf_stage_times", "annf_set_timing",
+    "annf_exact_field_dev",
     "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
     "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev",
 )

@@ -210,14 +210,17 @@ __global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float
 constexpr int kEncTile = 64;    // patches per tile
 constexpr int kHS = 128 + 8;    // hidden tile row stride (bf16)
 
-// Persistent blocks: each wavefront keeps its 32 rows of W1 and W2 (and
-// waves 0-1 all of W3) in registers as MFMA A fragments; per tile of 64
-// patches: stage the patches as BF16 in LDS (rows padded to whole dwords:
-// W1 is stored in the same padded feature order, zero at the pads), layer 1
-// as W1 . X^T (wave w owns hidden units 32w..32w+31, so each lane holds four
-// consecutive units of one patch and writes them as one 8-byte LDS store) +
-// GELU, layer 2 likewise, layer 3 (d x 64) by waves 0-1, written as FP32 rows
-// of dpad floats (zero past d), four floats per store.
+// Persistent blocks: each wavefront keeps its 32 rows of W1 and W2 in
+// registers as MFMA A fragments (W3, used by waves 0-1, sits in LDS); per
+// tile of 64 patches: stage the patches as BF16 in LDS (rows padded to whole
+// dwords: W1 is stored in the same padded feature order, zero at the pads;
+// the rows were loaded during the previous tile), layer 1 as W1 . X^T (wave w
+// owns hidden units 32w..32w+31, so each lane holds four consecutive units of
+// one patch and writes them as one 8-byte LDS store) + GELU, the MFMAs of the
+// second 32 patches scheduled between the GELUs of the first; layer 2 the
+// same way over H1 (while the next tile's rows load); layer 3 (d x 64) by
+// waves 0-1, written as FP32 rows of dpad floats (zero past d), four floats
+// per store.
 template <int DK>
 __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch, const __bf16 *__restrict__ W1p,
                                                 const float *__restrict__ b1, const __bf16 *__restrict__ W2,
@@ -234,14 +237,16 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
     const int hcol = 32 * wid + lc;
 
     __shared__ __align__(16) float sbias[128 + 128 + 32];   // b1, b2, b3 (padded)
-    bf16x8 w1f[KS1], w2f[8], w3f[8];
+    __shared__ __align__(16) __bf16 sw3[32 * kHS];             // W3 (padded to 32 rows)
+    bf16x8 w1f[KS1], w2f[8];
 #pragma unroll
     for (int ks = 0; ks < KS1; ++ks)
         w1f[ks] = *reinterpret_cast<const bf16x8 *>(W1p + (size_t)hcol * DK + 16 * ks + 8 * lh);
 #pragma unroll
-    for (int ks = 0; ks < 8; ++ks) {
-        w2f[ks] = *reinterpret_cast<const bf16x8 *>(W2 + (size_t)hcol * 128 + 16 * ks + 8 * lh);
-        w3f[ks] = *reinterpret_cast<const bf16x8 *>(W3p + (size_t)lc * 128 + 16 * ks + 8 * lh);
+    for (int ks = 0; ks < 8; ++ks) w2f[ks] = *reinterpret_cast<const bf16x8 *>(W2 + (size_t)hcol * 128 + 16 * ks + 8 * lh);
+    for (int i = tid; i < 32 * 128 / 8; i += 256) {
+        const int r = i / 16, c = (i % 16) * 8;
+        *reinterpret_cast<bf16x8 *>(sw3 + r * kHS + c) = *reinterpret_cast<const bf16x8 *>(W3p + (size_t)r * 128 + c);
     }
     for (int i = tid; i < 128 + 128 + 32; i += 256) sbias[i] = i < 128 ? b1[i] : (i < 256 ? b2[i - 128] : b3p[i - 256]);
     // rows of this lane's accumulator group g: units 32 * wid + 8 * g + 4 * lh + 0..3
@@ -253,17 +258,24 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
     const int used = g.rows * rw * 4;                  // padded features staged per patch
     const long long ntiles = (npatch + kEncTile - 1) / kEncTile;
 
+    // a quad of lanes per patch; the rows of the next tile load while the
+    // current one runs layers 2 and 3
+    using SS = StageShape<DK>;
+    const int pl = tid >> 2, q4 = tid & 3;
+    uint32_t rows[SS::RL][SS::LW];
+    long long o = 0;
+    auto fetch = [&](long long tile) {
+        const long long pi = tile * kEncTile + pl;
+        const bool pv = pi < npatch;
+        o = patch_origin(g, pv ? pi : 0);
+        fetch_rows(g, o, pv, q4, 4, rows);
+    };
+    if (blockIdx.x < ntiles) fetch(blockIdx.x);
+
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long long p0 = tile * kEncTile;
-        // stage: a quad of lanes per patch, every row load in flight at once
+        // stage
         {
-            using SS = StageShape<DK>;
-            const int pl = tid >> 2, q4 = tid & 3;
-            const long long pi = p0 + pl;
-            const bool pv = pi < npatch;
-            const long long o = patch_origin(g, pv ? pi : 0);
-            uint32_t rows[SS::RL][SS::LW];
-            fetch_rows(g, o, pv, q4, 4, rows);
             uint32_t *dst = reinterpret_cast<uint32_t *>(X + (size_t)pl * XS);
             // raw bytes (the packed weights carry the input scaling); integers
             // 0..255 are exact in BF16
@@ -278,30 +290,52 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
             for (int k = used / 2 + q4; k < DK / 2; k += 4) dst[k] = 0u;
         }
         __syncthreads();
-        // layer 1: acc[r] = unit 32 * wid + 8 * (r / 4) + 4 * lh + r % 4 of patch 32 * pt + lc
-#pragma unroll
-        for (int pt = 0; pt < 2; ++pt) {
-            floatx16 acc;   // starts at the bias
+        // layer 1: acc[r] = unit 32 * wid + 8 * (r / 4) + 4 * lh + r % 4 of patch 32 * pt + lc;
+        // the second half's MFMAs are interleaved with the first half's GELU
+        {
+            floatx16 acc[2];   // start at the bias
 #pragma unroll
             for (int gq = 0; gq < 4; ++gq) {
                 const floatx4 bb = *reinterpret_cast<const floatx4 *>(sbias + u0 + 8 * gq);
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc[4 * gq + i] = bb[i];
+                for (int i = 0; i < 4; ++i) acc[0][4 * gq + i] = acc[1][4 * gq + i] = bb[i];
             }
-            const __bf16 *xb = X + (size_t)(32 * pt + lc) * XS + 8 * lh;
 #pragma unroll
-            for (int ks = 0; ks < KS1; ++ks)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w1f[ks], *reinterpret_cast<const bf16x8 *>(xb + 16 * ks),
-                                                              acc, 0, 0, 0);
-            __bf16 *hrow = H1 + (size_t)(32 * pt + lc) * kHS;
+            for (int pt = 0; pt < 2; ++pt) {
+                const __bf16 *xb = X + (size_t)(32 * pt + lc) * XS + 8 * lh;
 #pragma unroll
-            for (int gq = 0; gq < 4; ++gq) {
-                const floatx4 v = {gelu_fast(acc[4 * gq + 0]), gelu_fast(acc[4 * gq + 1]), gelu_fast(acc[4 * gq + 2]),
-                                   gelu_fast(acc[4 * gq + 3])};
-                *reinterpret_cast<bf16x4 *>(hrow + u0 + 8 * gq) = __builtin_convertvector(v, bf16x4);
+                for (int ks = 0; ks < KS1; ++ks)
+                    acc[pt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                        w1f[ks], *reinterpret_cast<const bf16x8 *>(xb + 16 * ks), acc[pt], 0, 0, 0);
+            }
+#pragma unroll
+            for (int pt = 0; pt < 2; ++pt) {
+                __bf16 *hrow = H1 + (size_t)(32 * pt + lc) * kHS;
+#pragma unroll
+                for (int gq = 0; gq < 4; ++gq) {
+                    const floatx4 v = {gelu_fast(acc[pt][4 * gq + 0]), gelu_fast(acc[pt][4 * gq + 1]),
+                                       gelu_fast(acc[pt][4 * gq + 2]), gelu_fast(acc[pt][4 * gq + 3])};
+                    *reinterpret_cast<bf16x4 *>(hrow + u0 + 8 * gq) = __builtin_convertvector(v, bf16x4);
+                }
+            }
+            // schedule: LDS reads eight ahead; the first half's MFMA chain,
+            // then each MFMA of the second half followed by a share of the
+            // first half's GELU
+            __builtin_amdgcn_sched_group_barrier(0x100, 8, 0);   // LDS reads
+#pragma unroll
+            for (int i = 0; i < KS1; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < KS1; ++i) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x402, 5, 0);   // VALU incl. transcendental
             }
         }
         __syncthreads();
+        if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);
         // layer 2 (H2 over the X tile)
         floatx16 acc2[2];   // start at the bias
 #pragma unroll
@@ -335,8 +369,9 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
             const __bf16 *hb = X + (32 * wid + lc) * kHS + 8 * lh;
 #pragma unroll
             for (int ks = 0; ks < 8; ++ks)
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w3f[ks], *reinterpret_cast<const bf16x8 *>(hb + 16 * ks),
-                                                              acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                    *reinterpret_cast<const bf16x8 *>(sw3 + lc * kHS + 16 * ks + 8 * lh),
+                    *reinterpret_cast<const bf16x8 *>(hb + 16 * ks), acc, 0, 0, 0);
             const long long pi = p0 + 32 * wid + lc;
             if (pi < npatch) {
 #pragma unroll
