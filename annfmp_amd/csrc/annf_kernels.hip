@@ -2142,8 +2142,10 @@ hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, 
 
 __device__ __forceinline__ long long patch_off(const PatchGeom &g, long long i) {
     if (g.image) {
-        long long y = i / g.ncols, x = i - y * g.ncols;
-        return (y * g.width + x) * 3;
+        // window indices are below 2^31: a 32-bit division
+        const unsigned ui = (unsigned)i, nc = (unsigned)g.ncols;
+        const unsigned y = ui / nc, x = ui - y * nc;
+        return ((long long)y * g.width + x) * 3;
     }
     return i * (long long)g.rowlen;
 }

@@ -13,10 +13,12 @@
 
 namespace annf {
 
-// byte offset of window i (image mode: ncols windows per image row)
+// byte offset of window i (image mode: ncols windows per image row); window
+// indices are below 2^31, so the row split is a 32-bit division
 __device__ __forceinline__ long long patch_origin(const PatchGeom &g, long long i) {
-    const long long y = i / g.ncols, x = i - y * g.ncols;
-    return (y * g.width + x) * 3;
+    const unsigned ui = (unsigned)i, nc = (unsigned)g.ncols;
+    const unsigned y = ui / nc, x = ui - y * nc;
+    return ((long long)y * g.width + x) * 3;
 }
 
 // Rows per lane (RL, with a quad of lanes per patch) and dwords per row
