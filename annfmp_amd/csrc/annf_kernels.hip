@@ -615,9 +615,9 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_eqoff(LevelArgs L, const uint
 // src_out[slot] records the position of the slot's record, which the subtree
 // blocks gather directly.
 // The moving pass of the partition (the slots of k_lvl_eqoff), coalesced: the
-// block first computes every point's slot (points s + u*256 + tid, so the
-// segment-order ranks come from eight block scans), then copies the records
-// one float4 per lane.  With nbox, the same copy accumulates the two
+// block first computes every point's slot (eight consecutive points per
+// thread, so the segment-order ranks come from one block scan), then copies
+// the records one float4 per lane.  With nbox, the same copy accumulates the two
 // children's data boxes (the next level's k_box), for DP/4 a power of two.
 template <int DP, bool RECORDS>
 __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
@@ -643,25 +643,33 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
         sbox[threadIdx.x / DP][threadIdx.x % DP][0] = 0xFFFFFFFFu;
         sbox[threadIdx.x / DP][threadIdx.x % DP][1] = 0u;
     }
-    int2 run = cnt_off[blockIdx.x];
+    // thread t takes the kBuildEPT consecutive points s + t * kBuildEPT, ...:
+    // one block scan of the per-thread counts gives every rank
+    const int i0 = s + (int)threadIdx.x * kBuildEPT;
+    uint32_t kk[kBuildEPT];
+    int nlt = 0, neq = 0;
 #pragma unroll
     for (int u = 0; u < kBuildEPT; ++u) {
-        const int i = s + u * kBuildTPB + threadIdx.x;
-        const bool valid = i < e;
-        const uint32_t k = valid ? keys[i] : 0u;
-        const int lt = valid && k < vk, eq = valid && k == vk;
-        int2 sum;
-        const int2 ex = block_scan_int2_tot(make_int2(lt, eq), wsum, sum);
-        const int r_lt = run.x + ex.x, r_eq = run.y + ex.y;
-        run.x += sum.x;
-        run.y += sum.y;
+        const int i = i0 + u;
+        kk[u] = i < e ? keys[i] : 0u;
+        nlt += i < e && kk[u] < vk;
+        neq += i < e && kk[u] == vk;
+    }
+    int2 sum;
+    const int2 ex = block_scan_int2_tot(make_int2(nlt, neq), wsum, sum);
+    const int2 run = cnt_off[blockIdx.x];
+    int r_lt = run.x + ex.x, r_eq = run.y + ex.y;   // points below / equal to the split key before point i
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) {
+        const int i = i0 + u;
+        const uint32_t k = kk[u];
         int slot = -1;
-        if (valid) {
+        if (i < e) {
             const int off = i - lo;
-            if (lt) {
-                slot = r_lt;
-            } else if (eq) {
-                slot = Lt + (E - 1) - r_eq;
+            if (k < vk) {
+                slot = r_lt++;
+            } else if (k == vk) {
+                slot = Lt + (E - 1) - r_eq++;
             } else {
                 int p = (g - 1) - (off - r_lt);
                 // the chain strictly increases, so it ends within E steps
@@ -677,7 +685,7 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
                 if (slot == half) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
             }
         }
-        slots[u * kBuildTPB + threadIdx.x] = slot;
+        slots[i - s] = slot;
     }
     if (!RECORDS) return;   // the records stay; src_out says where each slot's record is
     __syncthreads();
