@@ -125,8 +125,11 @@ struct annf_ctx {
         }
         have_stages = true;
     }
-    // host copies of the level bounds (kept alive for the async upload)
+    // host copies of the level bounds (kept alive for the async upload), and
+    // the (n, depth) they and the device copy in bnd were computed for
     std::vector<int> h_bnd, h_off, h_cpn, h_maxc;
+    long long bnd_key = -1;
+    int bnd_max_leaf = 0;
 
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box, &box2,
@@ -137,6 +140,7 @@ struct annf_ctx {
                           &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch,
                           &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack};
         for (DevBuf *b : list) b->release();
+        bnd_key = -1;
         has_tree = false;
         has_encoder = false;
     }
@@ -198,8 +202,14 @@ void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off
 // Builds the tree from n x dpad padded points already in c->pts_a.
 int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn, &maxc = c->h_maxc;
-    int max_leaf = 0;
-    level_bounds(n, depth, bnd, off, cpn, maxc, max_leaf);
+    // the level bounds depend on (n, depth) alone: computed and uploaded once
+    const long long bkey = ((long long)n << 6) | depth;
+    const bool same_bounds = c->bnd_key == bkey;
+    if (!same_bounds) {
+        c->bnd_key = -1;
+        level_bounds(n, depth, bnd, off, cpn, maxc, c->bnd_max_leaf);
+    }
+    const int max_leaf = c->bnd_max_leaf;
     const size_t nodes_max = depth > 0 ? ((size_t)1 << (depth - 1)) : 1;
     size_t chunks_max = 1;
     for (int l = 0; l < depth; ++l) chunks_max = std::max(chunks_max, ((size_t)1 << l) * cpn[l]);
@@ -227,10 +237,14 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     RCHK(c->cnt_off.ensure(chunks_max * sizeof(int2)));
     RCHK(c->src_idx.ensure((size_t)n * sizeof(int)));
     RCHK(c->tot.ensure(nodes_max * sizeof(int2)));
+    const void *bnd_was = c->bnd.p;
     RCHK(c->bnd.ensure(bnd.size() * sizeof(int)));
     RCHK(c->errflag.ensure(sizeof(int)));
     HIPCHK(hipMemsetAsync(c->errflag.p, 0, sizeof(int), c->stream));
-    HIPCHK(hipMemcpyAsync(c->bnd.p, bnd.data(), bnd.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (!same_bounds || c->bnd.p != bnd_was) {
+        HIPCHK(hipMemcpyAsync(c->bnd.p, bnd.data(), bnd.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        c->bnd_key = bkey;
+    }
     BuildScratch s;
     s.pts_a = c->pts_a.as<float>();
     s.pts_b = c->pts_b.as<float>();
@@ -576,8 +590,9 @@ int annf_tree_import(annf_ctx *c, int n, int d, int depth, const int *axis, cons
     if (depth != c->tree_depth)
         return fail(ANNF_EINVAL, "tree depth %d differs from the handle's depth %d", depth, c->tree_depth);
     c->has_tree = false;
-    // the leaf ranges are fixed by n and depth (the left child of c points gets c/2)
-    std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn, &maxc = c->h_maxc;
+    // the leaf ranges are fixed by n and depth (the left child of c points gets c/2);
+    // local bounds: the handle's cached build bounds stay as they are
+    std::vector<int> bnd, off, cpn, maxc;
     int max_leaf = 0;
     level_bounds(n, depth, bnd, off, cpn, maxc, max_leaf);
     const int nleaf = 1 << depth;

@@ -85,6 +85,33 @@ def test_tree_matches_oracle(n, d, kind, leaf, gpu_available):
     _assert_tree_equal(_fit(X, depth).tree_arrays(), ref)
 
 
+def test_refits_of_other_sizes_on_one_handle(gpu_available):
+    """One handle builds trees of different point counts and depths in turn
+    (the level bounds it keeps between builds must follow), with a tree
+    import of another size in between."""
+    from annfmp_amd import KDTree, _lib
+    from annfmp_amd._lib import check, lib, ptr
+    rng = np.random.default_rng(11)
+    sizes = [(30000, 6, 9), (70001, 6, 11), (30000, 6, 9), (30000, 6, 10)]
+    X0 = rng.standard_normal((5000, 6)).astype(np.float32)
+    donor = KDTree(n_neighbors=1, leaf_size=2, tree_depth=7, float_type="float", splitting_type="longest").fit(X0)
+    arr0 = donor.tree_arrays()
+    t = KDTree(n_neighbors=1, leaf_size=2, tree_depth=9, float_type="float", splitting_type="longest")
+    for it, (n, d, depth) in enumerate(sizes):
+        X = rng.standard_normal((n, d)).astype(np.float32)
+        t.tree_depth = depth
+        t.fit(X)
+        _assert_tree_equal(t.tree_arrays(), O.OracleTree(X, depth))
+        if it == 1:
+            # an import of a 5000-point, depth-7 tree on this handle
+            h = t._init_native(7)
+            check("annf_tree_import", lib().annf_tree_import(
+                h.h, 5000, 6, 7, ptr(arr0["axis"], _lib._ip), ptr(arr0["split"], _lib._fp),
+                ptr(arr0["leaves"], _lib._ip), ptr(arr0["xi_orig"], _lib._ip), ptr(arr0["xi_pts"], _lib._fp),
+                ptr(arr0["pli"], _lib._ip)))
+            assert np.array_equal(t.tree_arrays()["leaves"], arr0["leaves"])
+
+
 def test_cyclic_split_matches_oracle_rule(gpu_available):
     """splitting_type='cyclic' uses axis = depth % d (kdtree.c:369-371)."""
     rng = np.random.default_rng(5)
