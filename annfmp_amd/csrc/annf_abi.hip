@@ -130,6 +130,7 @@ struct annf_ctx {
     std::vector<int> h_bnd, h_off, h_cpn, h_maxc;
     long long bnd_key = -1;
     int bnd_max_leaf = 0;
+    int *h_flag = nullptr;   // pinned host copy of the build's error flag
 
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box, &box2,
@@ -153,11 +154,20 @@ int sync_stream(annf_ctx *c) {
     return ANNF_OK;
 }
 
-// Reads the device error flag of the last build (after the stream is idle).
-int check_build_flag(annf_ctx *c) {
-    if (!c->errflag.p) return ANNF_OK;
+// Waits for the stream and reads the device error flag of the last build;
+// the flag's copy to pinned memory is queued behind the work, so the wait
+// brings it back too.
+int sync_and_check_build(annf_ctx *c) {
     int flag = 0;
-    HIPCHK(hipMemcpy(&flag, c->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+    if (c->errflag.p && c->h_flag) {
+        *c->h_flag = 0;
+        HIPCHK(hipMemcpyAsync(c->h_flag, c->errflag.p, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        RCHK(sync_stream(c));
+        flag = *c->h_flag;
+    } else {
+        RCHK(sync_stream(c));
+        if (c->errflag.p) HIPCHK(hipMemcpy(&flag, c->errflag.p, sizeof(int), hipMemcpyDeviceToHost));
+    }
     if (flag) {
         c->has_tree = false;
         return fail(ANNF_EHIP, "tree build detected an inconsistent partition (flag %d)", flag);
@@ -384,6 +394,11 @@ int annf_create(annf_ctx **out, int device) {
             return fail(ANNF_EHIP, "hipEventCreate failed");
         }
     }
+    if (hipHostMalloc((void **)&c->h_flag, sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        c->h_flag = nullptr;
+        annf_destroy(c);
+        return fail(ANNF_ENOMEM, "hipHostMalloc failed");
+    }
     *out = c;
     return ANNF_OK;
 }
@@ -398,6 +413,7 @@ int annf_destroy(annf_ctx *c) {
         if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+    if (c->h_flag) (void)hipHostFree(c->h_flag);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -437,8 +453,7 @@ int annf_fit_dev(annf_ctx *c, const float *X, int n, int d) {
     RCHK(c->pts_a.ensure((size_t)n * dpad * sizeof(float)));
     HIPCHK(pad_rows(X, n, d, dpad, c->pts_a.as<float>(), c->stream));
     RCHK(build_from_padded(c, n, d, dpad, c->tree_depth));
-    RCHK(sync_stream(c));
-    return check_build_flag(c);
+    return sync_and_check_build(c);
 }
 
 int annf_fit(annf_ctx *c, const float *X, int n, int d) {
@@ -799,8 +814,7 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
                            n_neighbors, red_a_out, red_b_out, fs));
     RCHK(run_rows(c, (int)fs.n_a, fs.nr_a, fs.nc_a, n_neighbors, propagate, select_best_nn, fs.ga, fs.gb, nn_out,
                   dist_out, knn_out));
-    RCHK(sync_stream(c));
-    RCHK(check_build_flag(c));
+    RCHK(sync_and_check_build(c));
     c->collect_marks();
     return ANNF_OK;
 }
@@ -880,8 +894,7 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,
                         c->stream));
     c->mark(6);
-    RCHK(sync_stream(c));
-    RCHK(check_build_flag(c));
+    RCHK(sync_and_check_build(c));
     c->collect_marks();
     return ANNF_OK;
 }
