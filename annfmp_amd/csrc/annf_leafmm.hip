@@ -105,15 +105,14 @@ __device__ __forceinline__ void klist_insert(uint64_t (&l)[LM], uint64_t key) {
     l[0] = le[0] ? l[0] : key;
 }
 
-// The same sorted list of 32-bit keys.
+// The same sorted list of 32-bit keys as a min/max network: slot s of the
+// list with `key` added is the median of (old l[s-1], key, old l[s]) -- one
+// v_med3_u32 per slot, no branch; a key above every entry changes nothing.
 template <int LM>
 __device__ __forceinline__ void klist_insert32(uint32_t (&l)[LM], uint32_t key) {
-    bool le[LM];
 #pragma unroll
-    for (int s = 0; s < LM; ++s) le[s] = l[s] <= key;
-#pragma unroll
-    for (int s = LM - 1; s >= 1; --s) l[s] = le[s] ? l[s] : (le[s - 1] ? key : l[s - 1]);
-    l[0] = le[0] ? l[0] : key;
+    for (int s = LM - 1; s >= 1; --s) l[s] = min(max(l[s - 1], key), l[s]);
+    l[0] = min(l[0], key);
 }
 
 constexpr int kMaxRW = 10;   // dwords per padded patch row (p <= 12)
@@ -457,9 +456,15 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                     uint32_t l32[LM];
 #pragma unroll
                     for (int s2 = 0; s2 < LM; ++s2) l32[s2] = 0xFFFFFFFFu;
-                    for (int p = q4; p < tnp; p += 4) {
-                        const uint32_t key = dr[p];
-                        if (key < l32[LM - 1]) klist_insert32(l32, key);
+                    // all kNP columns of the row are written (0xFFFFFFFF past
+                    // the tile's points): eight reads in flight, then their inserts
+#pragma unroll
+                    for (int p0 = q4; p0 < kNP; p0 += 32) {
+                        uint32_t kk[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) kk[j] = dr[p0 + 4 * j];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) klist_insert32(l32, kk[j]);
                     }
 #pragma unroll
                     for (int s2 = 0; s2 < LM; ++s2) {
