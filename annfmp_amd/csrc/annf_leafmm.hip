@@ -105,14 +105,25 @@ __device__ __forceinline__ void klist_insert(uint64_t (&l)[LM], uint64_t key) {
     l[0] = le[0] ? l[0] : key;
 }
 
-// The same sorted list of 32-bit keys as a min/max network: slot s of the
-// list with `key` added is the median of (old l[s-1], key, old l[s]) -- one
-// v_med3_u32 per slot, no branch; a key above every entry changes nothing.
+// The same sorted list of 32-bit keys.  Short lists (LM <= 8) as a min/max
+// network, applied to every key: slot s of the list with `key` added is the
+// median of (old l[s-1], key, old l[s]), a key above every entry changes
+// nothing.  Longer lists only when the key beats the last entry (most keys
+// do not, and the network's cost grows with LM).
 template <int LM>
 __device__ __forceinline__ void klist_insert32(uint32_t (&l)[LM], uint32_t key) {
+    if constexpr (LM <= 8) {
 #pragma unroll
-    for (int s = LM - 1; s >= 1; --s) l[s] = min(max(l[s - 1], key), l[s]);
-    l[0] = min(l[0], key);
+        for (int s = LM - 1; s >= 1; --s) l[s] = min(max(l[s - 1], key), l[s]);
+        l[0] = min(l[0], key);
+    } else if (key < l[LM - 1]) {
+        bool le[LM];
+#pragma unroll
+        for (int s = 0; s < LM; ++s) le[s] = l[s] <= key;
+#pragma unroll
+        for (int s = LM - 1; s >= 1; --s) l[s] = le[s] ? l[s] : (le[s - 1] ? key : l[s - 1]);
+        l[0] = le[0] ? l[0] : key;
+    }
 }
 
 constexpr int kMaxRW = 10;   // dwords per padded patch row (p <= 12)
