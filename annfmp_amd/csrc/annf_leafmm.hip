@@ -431,26 +431,34 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
             __syncthreads();   // every wavefront is done with the staged points
             {
                 const bool pv = pcol < tnp;
+                // a column past the tile's points makes every key 0xFFFFFFFF
+                const uint32_t colbits = pv ? (uint32_t)pcol : 0xFFFFFFFFu;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int row = (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                for (int r4 = 0; r4 < 4; ++r4) {
 #pragma unroll
                     for (int h = 0; h < 2; ++h) {
-                        const int qq = row + 32 * h;
-                        const float acc = h ? acc1[r] : acc0[r];
-                        uint32_t d;
-                        if constexpr (FP8) {
-                            const float dd = ((float)na[qq] + (float)nbp) - 2.0f * (acc / (sa[qq] * sbp));
-                            d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
-                        } else {
-                            // a.b as the sum of two exact integer halves; the key
-                            // (d2 << 7) | tile position orders like (d2, B index)
-                            // (positions follow ascending B index, d2 < 2^25)
-                            const float accx = h ? acc1x[r] : acc0x[r];
-                            d = ((na[qq] + nbp - 2u * ((uint32_t)(int)acc + (uint32_t)(int)accx)) << 7) |
-                                (uint32_t)pcol;
+                        // the accumulator rows r4 * 4 + 0..3 are four consecutive queries
+                        const int q0 = r4 * 8 + (lane >> 5) * 4 + 32 * h;
+                        const uint4 na4 = *reinterpret_cast<const uint4 *>(na + q0);
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int r = 4 * r4 + i, qq = q0 + i;
+                            const uint32_t naq = i == 0 ? na4.x : (i == 1 ? na4.y : (i == 2 ? na4.z : na4.w));
+                            const float acc = h ? acc1[r] : acc0[r];
+                            uint32_t d;
+                            if constexpr (FP8) {
+                                const float dd = ((float)naq + (float)nbp) - 2.0f * (acc / (sa[qq] * sbp));
+                                d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
+                                d = pv ? d : 0xFFFFFFFFu;
+                            } else {
+                                // a.b as the sum of two exact integer halves; the key
+                                // (d2 << 7) | tile position orders like (d2, B index)
+                                // (positions follow ascending B index, d2 < 2^25)
+                                const float accx = h ? acc1x[r] : acc0x[r];
+                                d = ((naq + nbp - 2u * ((uint32_t)(int)acc + (uint32_t)(int)accx)) << 7) | colbits;
+                            }
+                            Dm[qq * kDS + pcol] = d;
                         }
-                        Dm[qq * kDS + pcol] = pv ? d : 0xFFFFFFFFu;
                     }
                 }
             }
