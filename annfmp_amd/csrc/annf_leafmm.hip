@@ -59,14 +59,17 @@ __device__ __forceinline__ uint32_t fp8_word(uint32_t w, float scale, uint32_t m
 
 typedef float floatx8 __attribute__((ext_vector_type(8)));
 
-// eight bytes (two dwords at p) as BF16 (exact: bytes are small integers;
-// the FP32 -> BF16 step is one packed conversion per pair)
-__device__ __forceinline__ bf16x8 frag_bf16(const uint8_t *p) {
-    const uint32_t w0 = *reinterpret_cast<const uint32_t *>(p);
-    const uint32_t w1 = *reinterpret_cast<const uint32_t *>(p + 4);
-    const floatx8 f = {(float)(w0 & 255u), (float)((w0 >> 8) & 255u), (float)((w0 >> 16) & 255u),
-                       (float)(w0 >> 24), (float)(w1 & 255u), (float)((w1 >> 8) & 255u),
-                       (float)((w1 >> 16) & 255u), (float)(w1 >> 24)};
+// eight bytes (two dwords at p)
+__device__ __forceinline__ uint2 raw8(const uint8_t *p) {
+    return make_uint2(*reinterpret_cast<const uint32_t *>(p), *reinterpret_cast<const uint32_t *>(p + 4));
+}
+
+// eight bytes as BF16 (exact: bytes are small integers; the FP32 -> BF16
+// step is one packed conversion per pair)
+__device__ __forceinline__ bf16x8 bf16_of(uint2 w) {
+    const floatx8 f = {(float)(w.x & 255u), (float)((w.x >> 8) & 255u), (float)((w.x >> 16) & 255u),
+                       (float)(w.x >> 24), (float)(w.y & 255u), (float)((w.y >> 8) & 255u),
+                       (float)((w.y >> 16) & 255u), (float)(w.y >> 24)};
     return __builtin_convertvector(f, bf16x8);
 }
 
@@ -410,18 +413,28 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(frag_fp8(ar1 + k0), fb, acc1, 0, 0, 0);
                     }
                 } else {
-#pragma unroll 3
-                    for (int k0 = 0; k0 < DK; k0 += 32) {
-                        const bf16x8 fb = frag_bf16(br + k0);
-                        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar0 + k0), fb, acc0, 0, 0, 0);
-                        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar1 + k0), fb, acc1, 0, 0, 0);
+                    // the next 16-deep step's bytes are read before this step's
+                    // are converted, so each LDS read has a step to land
+                    uint2 wa0 = raw8(ar0), wa1 = raw8(ar1), wb = raw8(br);
+#pragma unroll
+                    for (int k0 = 0; k0 < DK; k0 += 16) {
+                        uint2 na0 = wa0, na1 = wa1, nbw = wb;
                         if (k0 + 16 < DK) {
-                            const bf16x8 fc = frag_bf16(br + k0 + 16);
-                            acc0x = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar0 + k0 + 16), fc, acc0x, 0,
-                                                                            0, 0);
-                            acc1x = __builtin_amdgcn_mfma_f32_32x32x16_bf16(frag_bf16(ar1 + k0 + 16), fc, acc1x, 0,
-                                                                            0, 0);
+                            na0 = raw8(ar0 + k0 + 16);
+                            na1 = raw8(ar1 + k0 + 16);
+                            nbw = raw8(br + k0 + 16);
                         }
+                        const bf16x8 fb = bf16_of(wb);
+                        if ((k0 & 16) == 0) {
+                            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf16_of(wa0), fb, acc0, 0, 0, 0);
+                            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf16_of(wa1), fb, acc1, 0, 0, 0);
+                        } else {
+                            acc0x = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf16_of(wa0), fb, acc0x, 0, 0, 0);
+                            acc1x = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bf16_of(wa1), fb, acc1x, 0, 0, 0);
+                        }
+                        wa0 = na0;
+                        wa1 = na1;
+                        wb = nbw;
                     }
                 }
             }
