@@ -10,7 +10,7 @@
 //    The K smallest (distance, B index) keys are the answer; no re-rank can
 //    change them.
 //  * FP8-E4M3: both patches centred and scaled per patch by 448 / max|x-128|,
-//    dot products de-scaled in FP32.  The S smallest approximate keys form a
+//    dot products de-scaled in FP32 (times the two inverse scales).  The S smallest approximate keys form a
 //    shortlist that is re-ranked with exact integer distances into the K
 //    list.
 //
@@ -366,7 +366,7 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                 const float sc = mx ? 448.0f / (float)mx : 1.0f;
                 fp8_rows(row, ga.rows, rw, tmask, sc, q4);
                 if (q4 == 0) {
-                    sa[q] = sc;
+                    sa[q] = mx ? (float)mx / 448.0f : 1.0f;   // the inverse scale
                     na[q] = ss - 256u * sm + 16384u * (uint32_t)D;   // sum (a - 128)^2
                 }
             } else {
@@ -387,7 +387,7 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                 const float sc = mx ? 448.0f / (float)mx : 1.0f;
                 fp8_rows(row, gb.rows, rw, tmask, sc, q4);
                 if (q4 == 0) {
-                    sb[i] = sc;
+                    sb[i] = mx ? (float)mx / 448.0f : 1.0f;
                     nb[i] = ss - 256u * sm + 16384u * (uint32_t)D;
                 }
             } else {
@@ -453,6 +453,8 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                         // the accumulator rows r4 * 4 + 0..3 are four consecutive queries
                         const int q0 = r4 * 8 + (lane >> 5) * 4 + 32 * h;
                         const uint4 na4 = *reinterpret_cast<const uint4 *>(na + q0);
+                        float4 sa4 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                        if constexpr (FP8) sa4 = *reinterpret_cast<const float4 *>(sa + q0);
 #pragma unroll
                         for (int i = 0; i < 4; ++i) {
                             const int r = 4 * r4 + i, qq = q0 + i;
@@ -460,7 +462,8 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                             const float acc = h ? acc1[r] : acc0[r];
                             uint32_t d;
                             if constexpr (FP8) {
-                                const float dd = ((float)naq + (float)nbp) - 2.0f * (acc / (sa[qq] * sbp));
+                                const float saq = i == 0 ? sa4.x : (i == 1 ? sa4.y : (i == 2 ? sa4.z : sa4.w));
+                                const float dd = ((float)naq + (float)nbp) - 2.0f * (acc * (saq * sbp));
                                 d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
                                 d = pv ? d : 0xFFFFFFFFu;
                             } else {
