@@ -2325,7 +2325,7 @@ __global__ __launch_bounds__(256) void k_project(const uint8_t *__restrict__ img
 // components with v_mfma_f32_16x16x4_f32.  Lane l of a wavefront takes patch
 // row l % 16 and, within each group of 16 features, the four consecutive
 // features 4 (l / 16) .. +3 (one dword of bytes, one float4 of mean and of
-// each component), so the sum runs over the same features as the CUDA-core
+// each component), so the sum runs over the same features as the VALU
 // projection, in another order.
 typedef float floatx4v __attribute__((ext_vector_type(4)));
 constexpr int kProjTile = 64;
@@ -2436,7 +2436,7 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
             hipLaunchKernelGGL((k_project<DP, double>), dim3(blocks_for(n, 256)), dim3(256), lds, st, img, h, w, p,
                                basis_t64, mean64, d, out);
         } else if (projection_dk(p) == 0) {
-            // p > 12: the CUDA-core kernel
+            // p > 12: the VALU kernel (k_project)
             const size_t lds = (size_t)D * (DP + 1) * sizeof(float);
             if (lds > 160 * 1024) return hipErrorInvalidValue;
             hipError_t e = hipFuncSetAttribute((const void *)k_project<DP, float>,
@@ -2488,7 +2488,7 @@ size_t projection_pack_floats(int p, int dpad) {
 
 hipError_t pack_projection(const float *basis_t, const float *mean_f, int p, int dpad, float *pack, hipStream_t st) {
     const int DK = projection_dk(p);
-    if (DK == 0) return hipSuccess;   // no table: the CUDA-core kernel projects
+    if (DK == 0) return hipSuccess;   // no table: the VALU kernel projects
     if (dpad > 32) return hipErrorInvalidValue;
     const int ctw = dpad <= 16 ? 16 : 32;
     hipLaunchKernelGGL(k_pack_projection, dim3(blocks_for((long long)ctw * DK, 256)), dim3(256), 0, st, basis_t,
