@@ -501,11 +501,16 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
 #pragma unroll
                         for (int j = 0; j < 8; ++j) klist_insert32(l32, kk[j]);
                     }
+                    // the B indices of the tile's best, every read in flight first
+                    // (k & 127 is a tile position, also for the empty key)
+                    int bo[LM];
+#pragma unroll
+                    for (int s2 = 0; s2 < LM; ++s2) bo[s2] = borig[l32[s2] & 127u];
 #pragma unroll
                     for (int s2 = 0; s2 < LM; ++s2) {
                         const uint32_t k = l32[s2];
                         const uint64_t key =
-                            k == 0xFFFFFFFFu ? kNoKey : (((uint64_t)(k >> 7) << 32) | (uint32_t)borig[k & 127u]);
+                            k == 0xFFFFFFFFu ? kNoKey : (((uint64_t)(k >> 7) << 32) | (uint32_t)bo[s2]);
                         if (first_tile) {
                             lst[s2] = key;   // ascending already
                         } else if (key < lst[LM - 1]) {
