@@ -21,10 +21,15 @@ distances in the projected space, the default; "bf16" / "fp8": the
 full-patch leaf k-NN on MFMA of ``annf_field_mm_dev``, its list ranked by the
 full-patch distance, propagation as the parallel second pass), ``encoder``
 (a ``PatchEncoder`` or the path of its ``.npz``: the learned projection
-replaces PCA), ``shortlist`` (FP8 re-rank length) and ``nprobe`` (leaves per
-query for "bf16" / "fp8").  Extra outputs after ``fit``: ``nn_distances`` (full-patch
-L2 of each chosen match, float32) and ``knn_indices`` (the K candidates of
-every patch before best-of-K, original indices).
+replaces PCA), ``shortlist`` (FP8 re-rank length), ``nprobe`` (leaves per
+query for "bf16" / "fp8") and ``encoder_check`` (with an encoder: when > 0,
+the field is also computed with the PCA projection, recall@K of both fields is
+measured on that many sampled A patches against the exact field, and the
+better one is returned -- PCA is the fallback when the encoder's recall is
+lower).  Extra outputs after ``fit``: ``nn_distances`` (full-patch L2 of each
+chosen match, float32), ``knn_indices`` (the K candidates of every patch before
+best-of-K, original indices), ``projection_used`` ("pca" or "encoder") and,
+with ``encoder_check``, ``encoder_check_result`` (both recalls).
 """
 from __future__ import annotations
 
@@ -41,7 +46,8 @@ from .kdtree import tree_depth_for
 class ANNFieldPropKDTreeHIP:
     def __init__(self, n_neighbors=3, psize=8, dim_reduced=5, n_subset=1000, leaf_size=64, n_jobs=-1,
                  propagate=True, select_best_nn=True, float_type="float", verbose=1, seed=0, device=0,
-                 projection="fp32", leaf_distance="reduced", encoder=None, shortlist=20, nprobe=1):
+                 projection="fp32", leaf_distance="reduced", encoder=None, shortlist=20, nprobe=1,
+                 encoder_check=0):
         self.n_neighbors = n_neighbors
         self.psize = psize
         self.dim_reduced = dim_reduced
@@ -66,6 +72,7 @@ class ANNFieldPropKDTreeHIP:
         self.encoder = encoder
         self.shortlist = shortlist
         self.nprobe = nprobe
+        self.encoder_check = int(encoder_check)
         self._handle = None
 
     # ------------------------------------------------------------------
@@ -78,8 +85,10 @@ class ANNFieldPropKDTreeHIP:
         self._n_cols = self.image_a.shape[1] - self.psize + 1
         self._n_rows = self.image_a.shape[0] - self.psize + 1
 
+        check = self.encoder is not None and self.encoder_check > 0
         tstart = time.time()
-        self._pca_model = None if self.encoder is not None else self._fit_pca_model(self.image_a, self.image_b)
+        need_pca = self.encoder is None or check
+        self._pca_model = self._fit_pca_model(self.image_a, self.image_b) if need_pca else None
         self._timers["pca_fit"] = time.time() - tstart
 
         # since there is just one leaf visit, this must hold (openmp/base.py:113)
@@ -89,16 +98,27 @@ class ANNFieldPropKDTreeHIP:
         depth = tree_depth_for(n_b, self.leaf_size)
         comp = self._pca_model.components_ if self._pca_model is not None else None
         mean = self._pca_model.mean_ if self._pca_model is not None else None
-        proj = self.encoder if self.encoder is not None else self.projection
-        if self.leaf_distance == "reduced":
-            res = field_on_device(image_a, image_b, self.psize, comp, mean, depth, self.n_neighbors, self.propagate,
-                                  self.select_best_nn, projection=proj, device=self.device,
-                                  handle=self._get_handle())
-        else:
-            res = field_mm_on_device(image_a, image_b, self.psize, comp, mean, depth, self.n_neighbors,
-                                     nprobe=self.nprobe, precision=self.leaf_distance, shortlist=self.shortlist,
-                                     projection=proj, propagate=bool(self.propagate), device=self.device,
-                                     handle=self._get_handle())
+        res = self._field(image_a, image_b, comp, mean, depth,
+                          self.encoder if self.encoder is not None else self.projection)
+        self.projection_used = "encoder" if self.encoder is not None else "pca"
+        self.encoder_check_result = None
+        if check:
+            # the fallback: PCA's field on the same pair, and recall@K of both
+            # against the exact field on sampled A patches
+            from . import metrics
+            res_pca = self._field(image_a, image_b, comp, mean, depth, self.projection)
+            k = self.n_neighbors
+            q = metrics.sample_queries(res["nn"].shape[0], self.encoder_check, seed=self.seed)
+            exact = metrics.exact_knn(image_a, image_b, self.psize, q, k, device=self.device,
+                                      handle=self._get_handle())
+            r_enc = metrics.field_quality(image_a, image_b, self.psize, res["nn"], res["knn"], k=k,
+                                          n_sample=q.size, seed=self.seed, exact=exact)["recall@%d" % k]
+            r_pca = metrics.field_quality(image_a, image_b, self.psize, res_pca["nn"], res_pca["knn"], k=k,
+                                          n_sample=q.size, seed=self.seed, exact=exact)["recall@%d" % k]
+            self.encoder_check_result = {"encoder": r_enc, "pca": r_pca, "sample": int(q.size)}
+            if r_enc < r_pca:
+                res = res_pca
+                self.projection_used = "pca"
         nn_indices = res["nn"]
         self.nn_distances = res["dist"]
         self.knn_indices = res["knn"]
@@ -124,6 +144,16 @@ class ANNFieldPropKDTreeHIP:
                                                       "kdtree_fit", "remaining_rows"))))
             print("-----------------------------------------")
         return nn_indices
+
+    def _field(self, image_a, image_b, comp, mean, depth, proj):
+        if self.leaf_distance == "reduced":
+            return field_on_device(image_a, image_b, self.psize, comp, mean, depth, self.n_neighbors,
+                                   self.propagate, self.select_best_nn, projection=proj, device=self.device,
+                                   handle=self._get_handle())
+        return field_mm_on_device(image_a, image_b, self.psize, comp, mean, depth, self.n_neighbors,
+                                  nprobe=self.nprobe, precision=self.leaf_distance, shortlist=self.shortlist,
+                                  projection=proj, propagate=bool(self.propagate), device=self.device,
+                                  handle=self._get_handle())
 
     def _get_handle(self):
         if self._handle is None:
@@ -180,7 +210,9 @@ def _projection_args(projection, components, mean, psize, handle, device):
         return None, None, enc.d, _lib.PROJ_ENCODER
     if projection not in ("fp32", "fp64"):
         raise ValueError(f"projection {projection!r}: 'fp32', 'fp64' or a PatchEncoder")
-    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=numpy.float32)).to(cuda)
+    # the FP64 projection takes the basis as given (float64), the FP32 one rounded
+    ftype = numpy.float64 if projection == "fp64" else numpy.float32
+    comp = torch.from_numpy(numpy.ascontiguousarray(components, dtype=ftype)).to(cuda)
     mu = torch.from_numpy(numpy.ascontiguousarray(mean, dtype=numpy.float64)).to(cuda)
     return comp, mu, comp.shape[0], _lib.PROJ_PCA64 if projection == "fp64" else _lib.PROJ_PCA32
 

@@ -324,6 +324,10 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
         RCHK(c->sort_tmp.ensure(sort_bytes));
         int *natdesc = nullptr;
         if (propagate) {
+            // the sweep's leaf descriptor packs a leaf's first tree position
+            // as (from << 1) in an int
+            if (t.n >= (1 << 30))
+                return fail(ANNF_EUNSUPPORTED, "%d tree points: propagation supports fewer than 2^30", t.n);
             RCHK(c->natdesc.ensure((size_t)n_a * sizeof(int)));
             natdesc = c->natdesc.as<int>();
         }
@@ -665,14 +669,19 @@ int annf_tree_import(annf_ctx *c, int n, int d, int depth, const int *axis, cons
 
 namespace {
 
-__global__ void k_prep_basis(const float *basis, const double *mean, int d, int D, int dpad, float *bt32,
-                             float *m32, double *bt64, double *m64) {
+// basis: d x D, float32 (basis_f64 == 0) or float64; the FP32 table holds it
+// rounded once, the FP64 table as given
+__global__ void k_prep_basis(const void *basis, int basis_f64, const double *mean, int d, int D, int dpad,
+                             float *bt32, float *m32, double *bt64, double *m64) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < D * dpad) {
         int f = i / dpad, k = i - f * dpad;
-        float v = k < d ? basis[(size_t)k * D + f] : 0.0f;
-        bt32[i] = v;
-        bt64[i] = (double)v;
+        double v = 0.0;
+        if (k < d)
+            v = basis_f64 ? static_cast<const double *>(basis)[(size_t)k * D + f]
+                          : (double)static_cast<const float *>(basis)[(size_t)k * D + f];
+        bt32[i] = (float)v;
+        bt64[i] = v;
     }
     if (i < D) {
         m32[i] = (float)mean[i];
@@ -697,7 +706,7 @@ struct FieldShape {
 };
 
 int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
-                      int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
+                      int psize, const void *basis, const double *mean, int d, int projection, int tree_depth,
                       int K, float *red_a_out, float *red_b_out, FieldShape &fs) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
     if (!img_a || !img_b) return fail(ANNF_EINVAL, "null image pointer");
@@ -730,8 +739,8 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         RCHK(c->mean32.ensure((size_t)D * sizeof(float)));
         RCHK(c->basis64.ensure((size_t)D * dpad * sizeof(double)));
         RCHK(c->mean64.ensure((size_t)D * sizeof(double)));
-        hipLaunchKernelGGL(k_prep_basis, dim3((D * dpad + 255) / 256), dim3(256), 0, c->stream, basis, mean, d, D,
-                           dpad, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
+        hipLaunchKernelGGL(k_prep_basis, dim3((D * dpad + 255) / 256), dim3(256), 0, c->stream, basis,
+                           (int)(projection == ANNF_PROJ_PCA64), mean, d, D, dpad, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
                            c->mean64.as<double>());
         HIPCHK(hipGetLastError());
         if (projection == ANNF_PROJ_PCA32) {
@@ -803,7 +812,7 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
 extern "C" {
 
 int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
-                   int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
+                   int psize, const void *basis, const double *mean, int d, int projection, int tree_depth,
                    int n_neighbors, int propagate, int select_best_nn, int *nn_out, float *dist_out, int *knn_out,
                    float *red_a_out, float *red_b_out) {
     if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
@@ -820,7 +829,7 @@ int annf_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint
 }
 
 int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
-                      int psize, const float *basis, const double *mean, int d, int projection, int tree_depth,
+                      int psize, const void *basis, const double *mean, int d, int projection, int tree_depth,
                       int n_neighbors, int nprobe, int precision, int shortlist, int propagate, int *nn_out,
                       float *dist_out, int *knn_out, float *red_a_out, float *red_b_out) {
     if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
@@ -881,7 +890,7 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         RCHK(c->mm_pairleaf.ensure(n_pairs * sizeof(int)));
         RCHK(c->mm_keys2.ensure(n_pairs * K * sizeof(uint64_t)));
         keys2 = c->mm_keys2.as<uint64_t>();
-        HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, c->natleaf.as<int>(), t.pli, fs.nc_a,
+        HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, nprobe, c->natleaf.as<int>(), t.pli, fs.nc_a,
                           (int)fs.n_b, c->mm_pairleaf.as<int>(), nullptr, c->stream));
         HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, leafmm_chunk(), skeys,
                              c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p,

@@ -150,9 +150,10 @@ hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st);
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *pts_order, const int *order,
                    const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K,
                    int fp8, int slot_div, uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st);
-// propagated (query, leaf) pairs: pairleaf[q * prop_k + k] (-1: none), per-leaf counts
-hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, const int *natleaf, const int *pli,
-                      int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st);
+// propagated (query, leaf) pairs: pairleaf[q * prop_k + k] (-1: none, or a leaf the
+// first pass scanned: natleaf ^ r, r < nprobe), per-leaf counts
+hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, int nprobe, const int *natleaf,
+                      const int *pli, int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st);
 hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st);
 hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, int K, int prop_k, int *knn, int *nn,
                         float *dist, hipStream_t st);

@@ -10,9 +10,11 @@
 //    The K smallest (distance, B index) keys are the answer; no re-rank can
 //    change them.
 //  * FP8-E4M3: both patches centred and scaled per patch by 448 / max|x-128|,
-//    dot products de-scaled in FP32 (times the two inverse scales).  The S smallest approximate keys form a
-//    shortlist that is re-ranked with exact integer distances into the K
-//    list.
+//    dot products de-scaled in FP32 (times the two inverse scales), the two
+//    norms those of the same quantized values (so the approximate distance is
+//    |a' - b'|^2 of the quantized patches).  The S smallest approximate keys
+//    form a shortlist that is re-ranked with exact integer distances into the
+//    K list.
 //
 // Keys are (squared distance << 32) | B index, so ties go to the lower B
 // index, the order of the exact field.  The first list entry is the match.
@@ -216,10 +218,27 @@ __device__ __forceinline__ void store_patch(const PatchGeom &g, long long o, con
 }
 
 // In-place FP8 conversion of the rows this lane staged (after the quad knows
-// the patch scale).
-__device__ __forceinline__ void fp8_rows(uint32_t *row, int rows, int rw, uint32_t tmask, float scale, int q4) {
+// the patch scale); returns this lane's share of the sum of squares of the
+// converted values (in the scaled domain), so that the distance
+// |a|^2 + |b|^2 - 2 a.b is formed from one consistent set of quantized values
+// (a patch against itself then scores 0, whatever the rounding).
+__device__ __forceinline__ float fp8_rows(uint32_t *row, int rows, int rw, uint32_t tmask, float scale, int q4) {
+    float ss = 0.0f;
     for (int r = q4; r < rows; r += 4)
-        for (int k = 0; k < rw; ++k) row[r * rw + k] = fp8_word(row[r * rw + k], scale, k == rw - 1 ? tmask : ~0u);
+        for (int k = 0; k < rw; ++k) {
+            const uint32_t w = fp8_word(row[r * rw + k], scale, k == rw - 1 ? tmask : ~0u);
+            row[r * rw + k] = w;
+            const float f0 = __builtin_amdgcn_cvt_f32_fp8((int)w, 0), f1 = __builtin_amdgcn_cvt_f32_fp8((int)w, 1);
+            const float f2 = __builtin_amdgcn_cvt_f32_fp8((int)w, 2), f3 = __builtin_amdgcn_cvt_f32_fp8((int)w, 3);
+            ss += (f0 * f0 + f1 * f1) + (f2 * f2 + f3 * f3);
+        }
+    return ss;
+}
+
+__device__ __forceinline__ float quad_sumf(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    return v;
 }
 
 // max |x - 128| over the real bytes of the rows this lane staged
@@ -364,10 +383,12 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
             if constexpr (FP8) {
                 const uint32_t mx = quad_max(maxabs_rows(row, ga.rows, rw, tmask, q4));
                 const float sc = mx ? 448.0f / (float)mx : 1.0f;
-                fp8_rows(row, ga.rows, rw, tmask, sc, q4);
+                const float inv = mx ? (float)mx / 448.0f : 1.0f;
+                const float qq = quad_sumf(fp8_rows(row, ga.rows, rw, tmask, sc, q4));
                 if (q4 == 0) {
-                    sa[q] = mx ? (float)mx / 448.0f : 1.0f;   // the inverse scale
-                    na[q] = ss - 256u * sm + 16384u * (uint32_t)D;   // sum (a - 128)^2
+                    sa[q] = inv;   // the inverse scale
+                    // |a - 128|^2 of the quantized patch, de-scaled
+                    na[q] = __float_as_uint(qq * (inv * inv));
                 }
             } else {
                 if (q4 == 0) na[q] = ss;
@@ -385,10 +406,11 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
             if constexpr (FP8) {
                 const uint32_t mx = quad_max(maxabs_rows(row, gb.rows, rw, tmask, q4));
                 const float sc = mx ? 448.0f / (float)mx : 1.0f;
-                fp8_rows(row, gb.rows, rw, tmask, sc, q4);
+                const float inv = mx ? (float)mx / 448.0f : 1.0f;
+                const float qq = quad_sumf(fp8_rows(row, gb.rows, rw, tmask, sc, q4));
                 if (q4 == 0) {
-                    sb[i] = mx ? (float)mx / 448.0f : 1.0f;
-                    nb[i] = ss - 256u * sm + 16384u * (uint32_t)D;
+                    sb[i] = inv;
+                    nb[i] = __float_as_uint(qq * (inv * inv));
                 }
             } else {
                 if (q4 == 0) nb[i] = ss;
@@ -463,7 +485,8 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
                             uint32_t d;
                             if constexpr (FP8) {
                                 const float saq = i == 0 ? sa4.x : (i == 1 ? sa4.y : (i == 2 ? sa4.z : sa4.w));
-                                const float dd = ((float)naq + (float)nbp) - 2.0f * (acc * (saq * sbp));
+                                const float dd = (__uint_as_float(naq) + __uint_as_float(nbp)) -
+                                                 2.0f * (acc * (saq * sbp));
                                 d = dd <= 0.0f ? 0u : (dd >= 4.0e9f ? 0xFFFFFFFEu : (uint32_t)dd);
                                 d = pv ? d : 0xFFFFFFFFu;
                             } else {
@@ -602,10 +625,10 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
 
 // Propagated leaves (the parallel form of base.c:255-282): for A patch q
 // below the first row and k < prop_k, the B patch one row below the k-th
-// match of the patch above (first-pass keys), its leaf pli[.]; leaves equal
-// to q's natural leaf or to an earlier k are dropped (-1).  Counts the kept
-// (q, k) pairs per leaf.
-__global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K, int prop_k,
+// match of the patch above (first-pass keys), its leaf pli[.]; leaves the
+// first pass already scanned (natleaf[q] ^ r, r < nprobe) or equal to an
+// earlier k's are dropped (-1).  Counts the kept (q, k) pairs per leaf.
+__global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K, int prop_k, int nprobe,
                              const int *__restrict__ natleaf, const int *__restrict__ pli, int n_cols, int n_b,
                              int *__restrict__ pairleaf, int *__restrict__ count) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -621,7 +644,7 @@ __global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K,
             return pli[o];
         };
         leaf = leaf_of(k);
-        if (leaf == natleaf[q]) leaf = -1;
+        if (leaf >= 0 && (leaf ^ natleaf[q]) < nprobe) leaf = -1;
         for (int kk = 0; kk < k && leaf >= 0; ++kk)
             if (leaf_of(kk) == leaf) leaf = -1;
     }
@@ -682,7 +705,9 @@ __global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
 }
 
 // The K smallest of a query's first-pass keys and its propagated pairs'
-// keys (disjoint leaves, so no duplicates) -> knn, nn, dist.
+// keys (k_prop_pairs leaves out every leaf the first pass scanned and
+// repeats, so the leaves are disjoint and no B index comes twice) -> knn,
+// nn, dist.
 template <int KM>
 __global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__restrict__ keys2, int n_a, int K,
                          int prop_k, int *__restrict__ knn, int *__restrict__ nn, float *__restrict__ dist) {
@@ -743,12 +768,12 @@ int leafmm_dk(int p) {
 
 bool leafmm_supported(int p) { return p >= 1 && leafmm_dk(p) <= 512; }
 
-hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, const int *natleaf, const int *pli,
-                      int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st) {
+hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, int nprobe, const int *natleaf,
+                      const int *pli, int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st) {
     const long long n = (long long)n_a * prop_k;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(k_prop_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys1, n_a, K, prop_k,
-                       natleaf, pli, n_cols, n_b, pairleaf, count);
+                       nprobe, natleaf, pli, n_cols, n_b, pairleaf, count);
     return hipGetLastError();
 }
 

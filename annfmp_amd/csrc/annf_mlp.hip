@@ -381,7 +381,16 @@ __global__ __launch_bounds__(256, 2) void k_encode(PatchGeom g, long long npatch
                         floatx4 v;
 #pragma unroll
                         for (int i = 0; i < 4; ++i) v[i] = o + i < d ? acc[4 * gq + i] + sbias[256 + o + i] : 0.0f;
-                        *reinterpret_cast<floatx4 *>(out + pi * dpad + o) = v;
+                        float *dst = out + pi * dpad + o;
+                        if (o + 4 <= dpad && (dpad & 3) == 0) {
+                            *reinterpret_cast<floatx4 *>(dst) = v;
+                        } else {
+                            // a row stride that is not a whole number of
+                            // float4s: element stores, none past the row
+#pragma unroll
+                            for (int i = 0; i < 4; ++i)
+                                if (o + i < dpad) dst[i] = v[i];
+                        }
                     }
                 }
             }

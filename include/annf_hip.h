@@ -121,11 +121,14 @@ int annf_tree_import(annf_ctx *ctx, int n, int d, int depth, const int *axis,
 /* ---- image-level fused field (the openmp backend's fit,
  *      annfmp/openmp/base.py:53-150, from projection onwards) -----------
  * All pointers are device pointers.  img_a is ha x wa x 3, img_b is
- * hb x wb x 3 (uint8, row-major).  basis is d x (p*p*3) float32 (PCA
- * components, sklearn's components_ layout), mean is p*p*3 float64.
+ * hb x wb x 3 (uint8, row-major).  basis is d x (p*p*3) (PCA components,
+ * sklearn's components_ layout): float32 for ANNF_PROJ_PCA32, float64 for
+ * ANNF_PROJ_PCA64; mean is p*p*3 float64.
  * Patches are projected as sum_f (x_f - mean_f) * basis[k][f], accumulated
  * in FP32 (projection = ANNF_PROJ_PCA32) or FP64 (ANNF_PROJ_PCA64, then
- * rounded to FP32), or by the handle's learned encoder (ANNF_PROJ_ENCODER,
+ * rounded to FP32: the reference's FP64 transform, openmp/base.py:193-200,
+ * up to its summation order), or by the handle's learned encoder
+ * (ANNF_PROJ_ENCODER,
  * set with annf_set_encoder; basis and mean may then be NULL).
  * The tree over B's projected patches is built at depth tree_depth.
  * Outputs: nn_out (n_a int32, original B indices), optionally dist_out
@@ -134,7 +137,7 @@ int annf_tree_import(annf_ctx *ctx, int n, int d, int depth, const int *axis,
  * float32 projected patches).  Any optional output may be NULL. */
 int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                    const uint8_t *img_b, int hb, int wb, int psize,
-                   const float *basis, const double *mean, int d,
+                   const void *basis, const double *mean, int d,
                    int projection, int tree_depth, int n_neighbors,
                    int propagate, int select_best_nn, int *nn_out,
                    float *dist_out, int *knn_out, float *red_a_out,
@@ -161,17 +164,20 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
  * natural leaf and repeats) are scanned the same way, and the K smallest
  * keys of both passes are kept.  The candidates come from the
  * first pass of the row above, not from its propagated result (the
- * reference sweeps rows in order); this needs wa == wb.
+ * reference sweeps rows in order); this needs wa == wb.  A propagated leaf
+ * the first pass already scanned (natural leaf ^ r, r < nprobe) is left
+ * out, so no B index is listed twice.
  * Outputs as annf_field_dev: nn_out is the first list entry, dist_out its
  * exact L2 (correctly rounded), knn_out the K list (-1 past the candidates).
  * No best-of-K pass: the list is already ranked by the full patch distance.
- * Stage marks: 0 project A, 1 project B, 2 tree build, 3 natural leaves +
+ * Stage times (annf_stage_times): 0 project B, 1 launch of A's projection,
+ * 2 tree build (ends once A is projected too), 3 natural leaves +
  * bucketing, 4 MFMA leaf k-NN, 5 propagated pass + merge. */
 #define ANNF_PREC_BF16 0
 #define ANNF_PREC_FP8 1
 int annf_field_mm_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                       const uint8_t *img_b, int hb, int wb, int psize,
-                      const float *basis, const double *mean, int d,
+                      const void *basis, const double *mean, int d,
                       int projection, int tree_depth, int n_neighbors,
                       int nprobe, int precision, int shortlist, int propagate,
                       int *nn_out, float *dist_out, int *knn_out,

@@ -23,6 +23,9 @@ Plus numpy restatements of the Python-side stages:
                           value is an integer below 2**25)
 * ``leaf_knn_exact``   -- the same restricted to a query's probed leaves (the
                           answer of the BF16 full-patch leaf k-NN)
+* ``leaf_knn_propagated`` (and ``_rows`` for a sample of the queries) -- the
+                          same over the probed leaves plus the leaves the row
+                          above's matches propagate (base.c:255-282)
 """
 from __future__ import annotations
 
@@ -155,23 +158,25 @@ def leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k):
     return idx, dst
 
 
-def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k):
+def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=1, prop_k=None):
     """The full-patch leaf k-NN with the parallel propagated pass: pass one is
-    leaf_knn_exact over the natural leaf; a patch below the first row then
-    also scans the leaves pli[o'] of its row-above neighbour's pass-one
-    matches o, shifted one row down (o' = o + n_cols when that is a B patch,
-    base.c:265-276), leaving out its natural leaf and repeats, and keeps the
-    k smallest (squared distance, B index) over all scanned points.
+    leaf_knn_exact over the leaves qleaf ^ 0 .. qleaf ^ (nprobe - 1); a patch
+    below the first row then also scans the leaves pli[o'] of the first
+    prop_k (default k) pass-one matches o of its row-above neighbour, shifted
+    one row down (o' = o + n_cols when that is a B patch, base.c:265-276),
+    leaving out the leaves pass one scanned and repeats, and keeps the k
+    smallest (squared distance, B index) over all scanned points.
     Returns (idx [n_a, k], sqdist [n_a, k]) int64, -1 past the candidates."""
-    i1, d1 = leaf_knn_exact(pa, pb, qleaf, leaves, orig, 1, k)
+    prop_k = k if prop_k is None else min(prop_k, k)
+    i1, d1 = leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k)
     n_a, n_b = pa.shape[0], pb.shape[0]
     out_i, out_d = i1.copy(), d1.copy()
     A = pa.astype(np.int64)
     B = pb.astype(np.int64)
     orig = np.asarray(orig, np.int64)
     for q in range(n_cols, n_a):
-        ls = [int(qleaf[q])]
-        for j in range(k):
+        ls = [int(qleaf[q]) ^ r for r in range(nprobe)]
+        for j in range(prop_k):
             o = int(i1[q - n_cols, j])
             if o < 0:
                 continue
@@ -180,13 +185,54 @@ def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k):
             lf = int(pli[o])
             if lf not in ls:
                 ls.append(lf)
-        if len(ls) == 1:
+        if len(ls) == nprobe:
             continue
         cand = orig[np.concatenate([np.arange(leaves[2 * lf], leaves[2 * lf + 1]) for lf in ls])]
         d2 = ((A[q][None, :] - B[cand]) ** 2).sum(1)
         order = np.lexsort((cand, d2))[:k]
         out_i[q, :order.size] = cand[order]
         out_d[q, :order.size] = d2[order]
+    return out_i, out_d
+
+
+def leaf_knn_propagated_rows(pa, pb, qleaf, leaves, orig, pli, n_cols, k, rows, nprobe=1, prop_k=None):
+    """leaf_knn_propagated for the A patches listed in `rows` only (for
+    workloads too large to scan every query in Python): pass one is computed
+    for those rows and for the patches directly above them, then each row
+    scans its probed and propagated leaves as leaf_knn_propagated does.
+    qleaf: natural leaf of every A patch.  Returns (idx, sqdist) of shape
+    [len(rows), k], int64, -1 past the candidates."""
+    rows = np.asarray(rows, np.int64)
+    prop_k = k if prop_k is None else min(prop_k, k)
+    n_b = pb.shape[0]
+    above = np.where(rows >= n_cols, rows - n_cols, rows)
+    both = np.unique(np.concatenate([rows, above]))
+    i1, d1 = leaf_knn_exact(pa[both], pb, np.asarray(qleaf)[both], leaves, orig, nprobe, k)
+    at = {int(r): j for j, r in enumerate(both)}
+    out_i = np.full((rows.size, k), -1, np.int64)
+    out_d = np.full((rows.size, k), -1, np.int64)
+    orig = np.asarray(orig, np.int64)
+    for j, q in enumerate(rows.tolist()):
+        out_i[j], out_d[j] = i1[at[q]], d1[at[q]]
+        if q < n_cols:
+            continue
+        ls = [int(qleaf[q]) ^ r for r in range(nprobe)]
+        for m in range(prop_k):
+            o = int(i1[at[q - n_cols], m])
+            if o < 0:
+                continue
+            if o + n_cols < n_b:
+                o += n_cols
+            lf = int(pli[o])
+            if lf not in ls:
+                ls.append(lf)
+        if len(ls) == nprobe:
+            continue
+        cand = orig[np.concatenate([np.arange(leaves[2 * lf], leaves[2 * lf + 1]) for lf in ls])]
+        d2 = ((pa[q].astype(np.int64)[None, :] - pb[cand].astype(np.int64)) ** 2).sum(1)
+        order = np.lexsort((cand, d2))[:k]
+        out_i[j, :order.size] = cand[order]
+        out_d[j, :order.size] = d2[order]
     return out_i, out_d
 
 

@@ -135,6 +135,33 @@ def test_fused_encoder_matches_forward(p, gpu_available):
     assert np.abs(got - ref).max() <= 3e-3 * scale
 
 
+@pytest.mark.parametrize("d", [5, 3, 13])
+def test_encode_image_row_stride_not_a_multiple_of_four(d, gpu_available):
+    """annf_encode_image_dev writes n x d floats with row stride d: with
+    d % 4 != 0 every row must equal the oracle's and nothing past the end of
+    `out` may change (a guard region of sentinels follows it)."""
+    import torch
+    from annfmp_amd import _lib, synth
+    from annfmp_amd import device as dev
+    from annfmp_amd.encoder import PatchEncoder
+    p = 5
+    a, _ = synth.make_pair(synth.small_spec(41, 53), seed=d)
+    enc = PatchEncoder(p, d, seed=d)
+    h = _handle()
+    enc.install(h)
+    n = (a.shape[0] - p + 1) * (a.shape[1] - p + 1)
+    guard = 64
+    buf = torch.full((n * d + guard,), -7.25, dtype=torch.float32, device="cuda")
+    img = _cuda(a)
+    _lib.check("encode", _lib.lib().annf_encode_image_dev(h.h, dev.dptr(img), a.shape[0], a.shape[1], p,
+                                                          dev.dptr(buf)))
+    got = buf.cpu().numpy()
+    assert np.all(got[n * d:] == -7.25)
+    pa = O.extract_patches(a, p)
+    ref = M.forward(pa, enc.params, p * p * 3, H1, H2, d, bf16_ops=True)
+    assert np.abs(got[: n * d].reshape(n, d) - ref).max() <= 3e-3 * np.abs(ref).max()
+
+
 def test_field_on_encoder_embeddings(gpu_available):
     """Both field entry points with ANNF_PROJ_ENCODER: the plain path equals
     the oracle's rows on the same embeddings, the BF16 path the exact k-NN

@@ -158,6 +158,31 @@ def test_bf16_propagation_matches_oracle(p, K, seed, gpu_available):
     assert (wd[:, 0] <= d1[:, 0]).all() and (wd[:, 0] < d1[:, 0]).any()
 
 
+@pytest.mark.parametrize("nprobe,prop", [(2, True), (4, True), (2, 2)])
+def test_bf16_propagation_with_several_probed_leaves(nprobe, prop, gpu_available):
+    """nprobe > 1 with the propagated pass: a propagated leaf that the first
+    pass already scanned (natural leaf ^ r, r < nprobe) is left out, so every
+    list holds K distinct B indices and equals the oracle's."""
+    from annfmp_amd import synth, field_mm_on_device
+    a, b = synth.make_pair(synth.small_spec(96, 128), seed=60 + nprobe)
+    p, K = 7, 5
+    pa, pb = O.extract_patches(a, p), O.extract_patches(b, p)
+    comp, mu = _basis(pa, pb, 8)
+    depth = O.tree_depth(pb.shape[0], 32)
+    res = field_mm_on_device(a, b, p, comp, mu, depth, K, nprobe=nprobe, precision="bf16", propagate=prop,
+                             return_reduced=True)
+    tree = O.OracleTree(res["red_b"], depth)
+    qleaf = tree.natural_leaves(res["red_a"])
+    n_cols = a.shape[1] - p + 1
+    prop_k = None if prop is True else int(prop)
+    wi, wd = O.leaf_knn_propagated(pa, pb, qleaf, tree.leaves, tree.orig, tree.pli, n_cols, K, nprobe=nprobe,
+                                   prop_k=prop_k)
+    rows = np.arange(pa.shape[0])
+    _check_bf16(res, rows, wi, wd)
+    knn = res["knn"]
+    assert all(len(set(r)) == K for r in knn.tolist())
+
+
 def test_fp8_propagation_properties(gpu_available):
     from annfmp_amd import synth, field_mm_on_device
     a, b = synth.make_pair(synth.small_spec(96, 128), seed=44)

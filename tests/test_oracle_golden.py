@@ -219,10 +219,12 @@ def test_leaf_knn_exact_reduces_to_exact_field():
         assert [c for _, c in dd[:3]] == list(i3[r]) and [x for x, _ in dd[:3]] == list(d3[r])
 
 
-def test_leaf_knn_propagated_is_the_best_of_its_leaves():
-    """Every query's list is exactly ordered, and nothing among the points of
-    its natural leaf and its propagated leaves (collected here row by row, as
-    base.c:255-282 does) is closer than its k-th entry."""
+@pytest.mark.parametrize("nprobe,prop_k", [(1, None), (2, None), (1, 2), (2, 1)])
+def test_leaf_knn_propagated_is_the_best_of_its_leaves(nprobe, prop_k):
+    """Every query's list is exactly ordered, lists no B index twice, and
+    nothing among the points of its probed leaves and its propagated leaves
+    (collected here row by row, as base.c:255-282 does, from the first
+    prop_k matches above) is closer than its k-th entry."""
     rng = np.random.default_rng(5)
     n_rows, n_cols, D, k = 9, 11, 12, 3
     n = n_rows * n_cols
@@ -234,15 +236,68 @@ def test_leaf_knn_propagated_is_the_best_of_its_leaves():
     for lf in range(4):
         pli[orig[leaves[2 * lf]:leaves[2 * lf + 1]]] = lf
     qleaf = rng.integers(0, 4, n)
-    i1, _ = O.leaf_knn_exact(pa, pb, qleaf, leaves, orig, 1, k)
-    ip, dp = O.leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k)
+    i1, _ = O.leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k)
+    ip, dp = O.leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=nprobe, prop_k=prop_k)
+    pk = k if prop_k is None else prop_k
     for q in range(n):
-        allowed = {int(qleaf[q])}
+        allowed = {int(qleaf[q]) ^ r for r in range(nprobe)}
         if q >= n_cols:
-            for o in i1[q - n_cols]:
+            for o in i1[q - n_cols][:pk]:
                 o2 = o + n_cols if o + n_cols < n else o
                 allowed.add(int(pli[o2]))
         pts = np.concatenate([orig[leaves[2 * lf]:leaves[2 * lf + 1]] for lf in sorted(allowed)])
         d_all = sorted((int(((pa[q].astype(int) - pb[j].astype(int)) ** 2).sum()), int(j)) for j in pts)
         assert [j for _, j in d_all[:k]] == list(ip[q]) and [x for x, _ in d_all[:k]] == list(dp[q])
+        assert len(set(ip[q].tolist())) == k
     assert (ip[:n_cols] == i1[:n_cols]).all()
+
+
+@pytest.mark.parametrize("shape,p,d,n_fit", [((256, 256), 7, 10, 10000), ((300, 310), 11, 16, 5000),
+                                              ((64, 80), 5, 40, 2000)])
+def test_project_f64_is_sklearns_transform(shape, p, d, n_fit):
+    """oracle.project_f64 equals scikit-learn's PCA.transform followed by
+    astype(float32) (openmp/base.py:193-200) bit for bit, on a basis fitted
+    the reference's way (extract_patches_2d samples of both images,
+    openmp/base.py:168-191); the 300x310 case crosses the oracle's 65,536-row
+    block boundary.  The projection of the fitted mean patch is zero and the
+    map is linear, which a dropped mean term or a transposed basis breaks."""
+    from sklearn.decomposition import PCA
+    from sklearn.feature_extraction.image import extract_patches_2d
+    from annfmp_amd import synth
+    a, b = synth.make_pair(synth.small_spec(*shape), seed=p)
+    sa = extract_patches_2d(a, (p, p), max_patches=n_fit // 2, random_state=1).reshape(n_fit // 2, -1)
+    sb = extract_patches_2d(b, (p, p), max_patches=n_fit // 2, random_state=2).reshape(n_fit // 2, -1)
+    pca = PCA(n_components=d).fit(np.concatenate([sa, sb]))
+    pa = O.extract_patches(a, p)
+    assert pa.shape[0] > (1 << 16) or shape[0] < 300
+    got = O.project_f64(pa, pca.components_, pca.mean_)
+    want = pca.transform(pa).astype(np.float32)
+    assert got.dtype == np.float32 and got.shape == (pa.shape[0], d)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # closed forms: the mean maps to 0; linearity on integer combinations
+    z = O.project_f64(pca.mean_[None, :], pca.components_, pca.mean_)
+    assert np.abs(z).max() <= 1e-9 * np.abs(pca.components_).sum()
+    comp = pca.components_.astype(np.float64)
+    lin = (pa[:50].astype(np.float64) - pca.mean_) @ comp.T
+    assert np.allclose(got[:50], lin, rtol=1e-6, atol=1e-4)
+
+
+@pytest.mark.parametrize("nprobe,prop_k", [(1, None), (2, 2)])
+def test_leaf_knn_propagated_rows_equals_the_full_pass(nprobe, prop_k):
+    """The sampled form gives the full form's rows (first row included)."""
+    rng = np.random.default_rng(8)
+    n_rows, n_cols, D, k = 12, 13, 10, 4
+    n = n_rows * n_cols
+    pa = rng.integers(0, 256, (n, D)).astype(np.uint8)
+    pb = rng.integers(0, 256, (n, D)).astype(np.uint8)
+    leaves = np.array([0, 40, 40, 78, 78, 117, 117, 156])
+    orig = rng.permutation(n)
+    pli = np.empty(n, np.int64)
+    for lf in range(4):
+        pli[orig[leaves[2 * lf]:leaves[2 * lf + 1]]] = lf
+    qleaf = rng.integers(0, 4, n)
+    fi, fd = O.leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=nprobe, prop_k=prop_k)
+    rows = np.array([0, 5, 12, 13, 40, 77, 155])
+    si, sd = O.leaf_knn_propagated_rows(pa, pb, qleaf, leaves, orig, pli, n_cols, k, rows, nprobe=nprobe,
+                                        prop_k=prop_k)
+    assert np.array_equal(si, fi[rows]) and np.array_equal(sd, fd[rows])
