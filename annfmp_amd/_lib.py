@@ -26,7 +26,7 @@ EXPORTS = (
     "annf_tree_export", "annf_tree_import", "annf_field_dev", "annf_stage_times", "annf_set_timing",
     "annf_exact_field_dev",
     "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
-    "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev",
+    "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev", "annf_mlp_triplet_grad_noalloc_dev",
 )
 
 PROJ_PCA32 = 0
@@ -86,6 +86,7 @@ def _declare(L):
     f = C.c_float
     L.annf_mlp_forward_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp]
     L.annf_mlp_triplet_grad_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, f, _vp, _vp]
+    L.annf_mlp_triplet_grad_noalloc_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, f, _vp, _vp]
     L.annf_mlp_adam_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_longlong, f, f, f, f, i]
     L.annf_set_encoder.argtypes = [_vp, _vp, i, i, i, i]
     L.annf_encode_image_dev.argtypes = [_vp, _vp, i, i, i, _vp]

@@ -8,10 +8,19 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace annf;
+
+// Workspace of annf_mlp_triplet_grad_noalloc_dev, the entry point that
+// allocates nothing: one copy per GPU, held until the process ends.  Host
+// code never passes `scratch` itself to a kernel (on the host it names a
+// stand-in, not the device copy): it uses the device address
+// hipGetSymbolAddress returns for the current device (scratch_address).
+constexpr size_t kNoAllocFloats = (size_t)4 << 20;   // 16 MiB
+static __device__ __align__(256) float scratch[kNoAllocFloats];
 
 namespace {
 
@@ -100,6 +109,8 @@ struct annf_ctx {
     DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder, proj_pack;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
+    // this device's copy of the allocation-free entry point's workspace
+    float *noalloc_ws = nullptr;
     // timing
     bool timing = false;
     hipEvent_t ev[kStages + 1] = {};
@@ -177,6 +188,27 @@ int sync_and_check_build(annf_ctx *c) {
 
 int set_device(annf_ctx *c) {
     HIPCHK(hipSetDevice(c->device));
+    return ANNF_OK;
+}
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float *g_scratch_addr[kMaxDevices] = {};
+
+int scratch_address(int device, float **out) {
+    if (device < 0 || device >= kMaxDevices) return fail(ANNF_EUNSUPPORTED, "device %d above %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(ANNF_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                        hipGetErrorString(e));
+        g_scratch_addr[device] = static_cast<float *>(ptr);
+    }
+    *out = g_scratch_addr[device];
     return ANNF_OK;
 }
 
@@ -402,6 +434,11 @@ int annf_create(annf_ctx **out, int device) {
         c->h_flag = nullptr;
         annf_destroy(c);
         return fail(ANNF_ENOMEM, "hipHostMalloc failed");
+    }
+    const int rc = scratch_address(device, &c->noalloc_ws);
+    if (rc != ANNF_OK) {
+        annf_destroy(c);
+        return rc;
     }
     *out = c;
     return ANNF_OK;
@@ -958,6 +995,25 @@ int annf_mlp_triplet_grad_dev(annf_ctx *c, const uint8_t *x, int B, int D, const
     RCHK(set_device(c));
     RCHK(c->mlp_ws.ensure(mlp_workspace_floats(3LL * B, h1, h2, d) * sizeof(float)));
     HIPCHK(mlp_triplet_grad(x, B, D, params, h1, h2, d, margin, grads, loss, c->mlp_ws.as<float>(), c->stream));
+    return sync_stream(c);
+}
+
+int annf_mlp_triplet_grad_noalloc_dev(annf_ctx *c, const uint8_t *x, int B, int D, const float *params, int h1,
+                                      int h2, int d, float margin, float *grads, float *loss) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (B < 1 || D < 1 || h1 < 1 || h2 < 1 || d < 1) return fail(ANNF_EINVAL, "bad MLP shape or empty batch");
+    if (!x || !params || !grads) return fail(ANNF_EINVAL, "null pointer");
+    if (3LL * B > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "batch too large");
+    const size_t need = mlp_workspace_floats(3LL * B, h1, h2, d);
+    if (need > kNoAllocFloats)
+        return fail(ANNF_EUNSUPPORTED, "batch %d of a %d-%d-%d network needs %zu workspace floats; the "
+                    "allocation-free entry holds %zu (use annf_mlp_triplet_grad_dev)", B, h1, h2, d, need,
+                    kNoAllocFloats);
+    if (!c->noalloc_ws) return fail(ANNF_EHIP, "no workspace address for device %d", c->device);
+    RCHK(set_device(c));
+    // every workspace word the kernels read is written earlier in the same
+    // call, so nothing is cleared
+    HIPCHK(mlp_triplet_grad(x, B, D, params, h1, h2, d, margin, grads, loss, c->noalloc_ws, c->stream));
     return sync_stream(c);
 }
 

@@ -202,6 +202,17 @@ int annf_mlp_forward_dev(annf_ctx *ctx, const uint8_t *x, int n, int D,
 int annf_mlp_triplet_grad_dev(annf_ctx *ctx, const uint8_t *x, int B, int D,
                               const float *params, int h1, int h2, int d,
                               float margin, float *grads, float *loss);
+/* The same gradient with no device allocation (no counterpart in the
+ * reference; for callers that cannot allocate: code under HIP graph capture,
+ * tight training loops).  Its workspace is a fixed per-GPU array of 4 Mi
+ * floats (16 MiB, held until the process ends): batches needing more
+ * (3B x (2 h1 + 2 h2 + 2 d) + B + 1 floats; B <= 2568 at 128-128-16) return
+ * ANNF_EUNSUPPORTED.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (two handles, two threads) race. */
+int annf_mlp_triplet_grad_noalloc_dev(annf_ctx *ctx, const uint8_t *x, int B,
+                                      int D, const float *params, int h1,
+                                      int h2, int d, float margin,
+                                      float *grads, float *loss);
 /* One Adam step (bias-corrected, step >= 1) on n parameters in place. */
 int annf_mlp_adam_dev(annf_ctx *ctx, float *params, const float *grads,
                       float *m, float *v, long long n, float lr, float beta1,
