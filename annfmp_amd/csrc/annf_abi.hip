@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -107,6 +108,8 @@ struct annf_ctx {
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
     DevBuf pca_scratch;
     DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder, proj_pack;
+    // exact BF16 leaf k-NN: centred BF16 images, per-patch norms
+    DevBuf cimg_a, cimg_b, pnorm_a, pnorm_b, pnorm_tmp;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // this device's copy of the allocation-free entry point's workspace
@@ -150,7 +153,8 @@ struct annf_ctx {
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
                           &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch,
-                          &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack};
+                          &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack, &cimg_a, &cimg_b,
+                          &pnorm_a, &pnorm_b, &pnorm_tmp};
         for (DevBuf *b : list) b->release();
         bnd_key = -1;
         has_tree = false;
@@ -744,7 +748,8 @@ struct FieldShape {
 
 int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
                       int psize, const void *basis, const double *mean, int d, int projection, int tree_depth,
-                      int K, float *red_a_out, float *red_b_out, FieldShape &fs) {
+                      int K, float *red_a_out, float *red_b_out, FieldShape &fs,
+                      const std::function<int(hipStream_t)> &side_extra = nullptr) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
     if (!img_a || !img_b) return fail(ANNF_EINVAL, "null image pointer");
     RCHK(set_device(c));
@@ -816,6 +821,8 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     HIPCHK(hipStreamWaitEvent(c->side, c->fork_ev, 0));
     // one encoder block per CU: the build's short kernels keep room beside it
     int rc = project(img_a, ha, wa, ga, n_a, c->qbuf.as<float>(), c->side, 1);
+    // work that needs only the images (queued behind A's projection, beside the build)
+    if (rc == ANNF_OK && side_extra) rc = side_extra(c->side);
     if (rc == ANNF_OK && hipEventRecord(c->join_ev, c->side) != hipSuccess) rc = fail(ANNF_EHIP, "hipEventRecord failed");
     if (rc != ANNF_OK) {
         (void)hipStreamSynchronize(c->side);
@@ -880,18 +887,37 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         return fail(ANNF_EINVAL, "nprobe %d is not a power of two in [1, 2^tree_depth]", nprobe);
     if (propagate > 0 && wa != wb)
         return fail(ANNF_EINVAL, "propagation needs images of equal width (base.c:271)");
+    if (precision == ANNF_PREC_BF16 && (2 * cimg_elems(ha, wa) >= (1LL << 31) || 2 * cimg_elems(hb, wb) >= (1LL << 31)))
+        return fail(ANNF_EUNSUPPORTED, "the BF16 leaf k-NN needs images of fewer than 2^30 elements");
+    if (propagate < 0) return fail(ANNF_EINVAL, "propagate %d is negative", propagate);
+    const int fp8 = precision == ANNF_PREC_FP8;
+    // the exact BF16 pass reads both images as centred BF16 with per-patch
+    // norms: made on the side stream while the tree is built
+    auto prepare_bf16 = [&](hipStream_t st) -> int {
+        const long long na_ = (long long)(ha - psize + 1) * (wa - psize + 1);
+        const long long nb_ = (long long)(hb - psize + 1) * (wb - psize + 1);
+        RCHK(c->cimg_a.ensure(2 * (size_t)cimg_elems(ha, wa) * sizeof(uint16_t)));
+        RCHK(c->cimg_b.ensure(2 * (size_t)cimg_elems(hb, wb) * sizeof(uint16_t)));
+        RCHK(c->pnorm_a.ensure((size_t)na_ * sizeof(uint32_t)));
+        RCHK(c->pnorm_b.ensure((size_t)nb_ * sizeof(uint32_t)));
+        RCHK(c->pnorm_tmp.ensure((size_t)std::max((long long)ha * (wa - psize + 1), (long long)hb * (wb - psize + 1)) *
+                                 sizeof(uint32_t)));
+        HIPCHK(center_image(img_a, ha, wa, c->cimg_a.as<uint16_t>(), st));
+        HIPCHK(center_image(img_b, hb, wb, c->cimg_b.as<uint16_t>(), st));
+        HIPCHK(patch_norms_centered(img_a, ha, wa, psize, c->pnorm_tmp.as<uint32_t>(), c->pnorm_a.as<uint32_t>(), st));
+        HIPCHK(patch_norms_centered(img_b, hb, wb, psize, c->pnorm_tmp.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), st));
+        return ANNF_OK;
+    };
     FieldShape fs;
     RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, projection, tree_depth, K,
-                           red_a_out, red_b_out, fs));
+                           red_a_out, red_b_out, fs, fp8 ? std::function<int(hipStream_t)>() : prepare_bf16));
     const TreeView &t = c->tree;
     const int nleaves = 1 << t.depth;
     const int n_a = (int)fs.n_a;
-    if (propagate < 0) return fail(ANNF_EINVAL, "propagate %d is negative", propagate);
     const bool prop = propagate > 0 && fs.nr_a > 1;
     // the first prop_k entries of the list above propagate (K: all, base.c:255-282)
     const int prop_k = std::min(propagate, K);
     const size_t n_pairs = prop ? (size_t)n_a * prop_k : 0;
-    const int fp8 = precision == ANNF_PREC_FP8;
     c->mark(3);
     RCHK(c->natleaf.ensure((size_t)n_a * sizeof(int)));
     RCHK(c->bcount.ensure((size_t)nleaves * sizeof(int)));
@@ -905,21 +931,34 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     const size_t sort_bytes = sort_scratch_bytes((int)n_sort, nleaves);
     RCHK(c->sort_tmp.ensure(sort_bytes));
     int *skeys = c->sort_keys.as<int>();
-    RCHK(c->mm_blkleaf.ensure(leafmm_scratch_bytes((int)n_sort, nleaves)));
+    RCHK(c->mm_blkleaf.ensure(fp8 ? leafmm_scratch_bytes((int)n_sort, nleaves)
+                                  : leafbf16_scratch_bytes((int)n_sort, nleaves)));
     void *blk_info = c->mm_blkleaf.p;
+    const int chunk = fp8 ? leafmm_chunk() : leafbf16_chunk();
     const int *pts_order = t.orig;
     if (!fp8) {
         RCHK(c->mm_ptorder.ensure((size_t)fs.n_b * sizeof(int)));
         HIPCHK(leaf_point_order(t, c->mm_ptorder.as<int>(), c->stream));
         pts_order = c->mm_ptorder.as<int>();
     }
+    auto leaf_pass = [&](int nq, int np_, int sdiv, uint64_t *keys, int *knn, int *nn, float *dist) -> int {
+        if (fp8) {
+            HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(),
+                           blk_info, nq, np_, shortlist, K, fp8, sdiv, keys, knn, nn, dist, c->stream));
+        } else {
+            HIPCHK(leaf_bf16(t, c->cimg_a.as<uint16_t>(), wa, c->cimg_b.as<uint16_t>(), wb, psize,
+                             c->pnorm_a.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), pts_order, c->order.as<int>(),
+                             c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, nq, np_, K, sdiv, keys, knn, nn,
+                             dist, c->stream, ha, hb));
+        }
+        return ANNF_OK;
+    };
     HIPCHK(natural_leaves(t, c->qbuf.as<float>(), n_a, c->natleaf.as<int>(), nullptr, nullptr, c->stream));
-    HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, leafmm_chunk(), skeys, c->order.as<int>(),
+    HIPCHK(bucket_sorted(c->natleaf.as<int>(), n_a, nleaves, chunk, skeys, c->order.as<int>(),
                          c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
     c->mark(4);
-    HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, n_a,
-                   nprobe, shortlist, K, fp8, 1, c->mm_keys1.as<uint64_t>(), prop ? nullptr : knn_out,
-                   prop ? nullptr : nn_out, prop ? nullptr : dist_out, c->stream));
+    RCHK(leaf_pass(n_a, nprobe, 1, c->mm_keys1.as<uint64_t>(), prop ? nullptr : knn_out, prop ? nullptr : nn_out,
+                   prop ? nullptr : dist_out));
     c->mark(5);
     uint64_t *keys2 = nullptr;
     if (prop) {
@@ -929,12 +968,11 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         keys2 = c->mm_keys2.as<uint64_t>();
         HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, nprobe, c->natleaf.as<int>(), t.pli, fs.nc_a,
                           (int)fs.n_b, c->mm_pairleaf.as<int>(), nullptr, c->stream));
-        HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, leafmm_chunk(), skeys,
+        HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, chunk, skeys,
                              c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p,
                              sort_bytes, c->stream));
         HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
-        HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), blk_info,
-                       (int)n_pairs, 1, shortlist, K, fp8, prop_k, keys2, nullptr, nullptr, nullptr, c->stream));
+        RCHK(leaf_pass((int)n_pairs, 1, prop_k, keys2, nullptr, nullptr, nullptr));
     }
     if (prop)
         HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,

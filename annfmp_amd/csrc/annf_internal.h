@@ -150,6 +150,22 @@ hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st);
 hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *pts_order, const int *order,
                    const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K,
                    int fp8, int slot_div, uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st);
+// exact BF16 full-patch leaf k-NN (annf_leafbf16.hip): queries bucketed with
+// chunks of leafbf16_chunk(); images as two centred BF16 copies
+// (center_image, buffers of 2 * cimg_elems elements) with per-patch norms of
+// x - 128 (patch_norms_centered, tmp: h * (w - p + 1) words); pts_order as
+// for leaf_mm; scratch of leafbf16_scratch_bytes; output slots as leaf_mm.
+int leafbf16_chunk();
+bool leafbf16_supported(int p);
+size_t leafbf16_scratch_bytes(int nq, int nleaves);
+long long cimg_elems(int h, int w);
+hipError_t center_image(const uint8_t *img, int h, int w, uint16_t *out, hipStream_t st);
+hipError_t patch_norms_centered(const uint8_t *img, int h, int w, int p, uint32_t *tmp, uint32_t *norm,
+                                hipStream_t st);
+hipError_t leaf_bf16(const TreeView &t, const uint16_t *ca, int wa, const uint16_t *cb, int wb, int p,
+                     const uint32_t *norm_a, const uint32_t *norm_b, const int *pts_order, const int *order,
+                     const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int K, int slot_div,
+                     uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st, int ha, int hb);
 // propagated (query, leaf) pairs: pairleaf[q * prop_k + k] (-1: none, or a leaf the
 // first pass scanned: natleaf ^ r, r < nprobe), per-leaf counts
 hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, int nprobe, const int *natleaf,
