@@ -16,8 +16,8 @@ LIB_PATH = os.path.join(HERE, "libannf_hip.so")
 
 ANNF_OK = 0
 ERRORS = {-1: "EINVAL", -2: "ENOTREE", -3: "EHIP", -4: "ENOMEM", -5: "EUNSUPPORTED"}
-MAX_K = 32
-MAX_D = 32
+MAX_K = 64
+MAX_D = 64
 
 # every symbol include/annf_hip.h declares
 EXPORTS = (

@@ -881,6 +881,7 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     if (precision != ANNF_PREC_BF16 && precision != ANNF_PREC_FP8)
         return fail(ANNF_EINVAL, "precision %d is not ANNF_PREC_BF16 or ANNF_PREC_FP8", precision);
     const int K = n_neighbors;
+    if (K < 1 || K > 32) return fail(ANNF_EUNSUPPORTED, "n_neighbors %d: the full-patch variants take 1..32", K);
     if (precision == ANNF_PREC_FP8 && (shortlist < K || shortlist > 32))
         return fail(ANNF_EUNSUPPORTED, "shortlist %d outside [n_neighbors, 32]", shortlist);
     if (tree_depth < 0 || tree_depth > 30 || nprobe < 1 || nprobe > (1 << tree_depth) || (nprobe & (nprobe - 1)))

@@ -135,6 +135,8 @@ static hipError_t with_dp(int dpad, Fn &&fn) {
     case 24: return fn(std::integral_constant<int, 24>{});
     case 28: return fn(std::integral_constant<int, 28>{});
     case 32: return fn(std::integral_constant<int, 32>{});
+    case 48: return fn(std::integral_constant<int, 48>{});
+    case 64: return fn(std::integral_constant<int, 64>{});
     default: return hipErrorInvalidValue;
     }
 }
@@ -143,13 +145,16 @@ template <typename Fn>
 static hipError_t with_dp_km(int dpad, int K, Fn &&fn) {
     return with_dp(dpad, [&](auto dp) {
         if (K <= 8) return fn(dp, std::integral_constant<int, 8>{});
-        return fn(dp, std::integral_constant<int, 32>{});
+        if (K <= 32) return fn(dp, std::integral_constant<int, 32>{});
+        return fn(dp, std::integral_constant<int, 64>{});
     });
 }
 
-int dpad_of(int d) { return (d + 3) & ~3; }
-bool dpad_supported(int dpad) { return dpad >= 4 && dpad <= 32 && dpad % 4 == 0; }
-bool k_supported(int K) { return K >= 1 && K <= 32; }
+// padded dimension: a multiple of 4 up to 32, then 48 or 64 (zero padding
+// adds exact +0 terms to every distance)
+int dpad_of(int d) { return d <= 32 ? (d + 3) & ~3 : (d <= 48 ? 48 : 64); }
+bool dpad_supported(int dpad) { return (dpad >= 4 && dpad <= 32 && dpad % 4 == 0) || dpad == 48 || dpad == 64; }
+bool k_supported(int K) { return K >= 1 && K <= 64; }
 
 static inline unsigned blocks_for(long long n, int tpb) {
     return (unsigned)((n + tpb - 1) / tpb);
@@ -409,24 +414,29 @@ __global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts
 // strictly largest FP32 width max - min, starting from 0.0.
 __device__ __forceinline__ int node_axis(const LevelArgs &L, int node, int d, int dpad, const uint32_t *box,
                                          int splitting_type) {
-    // called by a whole wavefront: lane l takes dimension l % 32 (d <= 32)
-    // and every other copy of the box, the halves combined by one shuffle
+    // called by a whole wavefront.  d <= 32: lane l takes dimension l % 32
+    // and every other copy of the box, the halves combined by one shuffle;
+    // d > 32: lane l takes dimension l and every copy
     if (splitting_type == 0) return L.lvl % d;
-    const int lane = threadIdx.x & (kWave - 1), k = lane & 31;
+    const int lane = threadIdx.x & (kWave - 1);
+    const bool halves = d <= 32;
+    const int k = halves ? lane & 31 : lane;
     uint32_t kmn = 0xFFFFFFFFu, kmx = 0u;
     if (k < d)
-        for (int rep = lane >> 5; rep < L.rbox; rep += 2) {
+        for (int rep = halves ? lane >> 5 : 0; rep < L.rbox; rep += halves ? 2 : 1) {
             kmn = min(kmn, box[box_at(node, L.rbox, rep, dpad, k, 0)]);
             kmx = max(kmx, box[box_at(node, L.rbox, rep, dpad, k, 1)]);
         }
-    kmn = min(kmn, (uint32_t)__shfl_xor((int)kmn, 32, kWave));
-    kmx = max(kmx, (uint32_t)__shfl_xor((int)kmx, 32, kWave));
+    if (halves) {
+        kmn = min(kmn, (uint32_t)__shfl_xor((int)kmn, 32, kWave));
+        kmx = max(kmx, (uint32_t)__shfl_xor((int)kmx, 32, kWave));
+    }
     // the first strictly largest FP32 width, starting from 0.0
     const float w = k < d ? key2f(kmx) - key2f(kmn) : 0.0f;
     float wmax = w;
 #pragma unroll
-    for (int m = 16; m >= 1; m >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, m, kWave));
-    const unsigned long long hit = __ballot(k < d && lane < 32 && w == wmax && w > 0.0f);
+    for (int m = 32; m >= 1; m >>= 1) wmax = fmaxf(wmax, __shfl_xor(wmax, m, kWave));
+    const unsigned long long hit = __ballot(k < d && (!halves || lane < 32) && w == wmax && w > 0.0f);
     return hit ? __ffsll(hit) - 1 : 0;
 }
 
@@ -764,10 +774,10 @@ constexpr int kSubTPB = 1024;
 constexpr int kSubNodes = 1 << (kSubLevels - 1);   // sub-nodes of the last level done in a block
 constexpr int kSubMaxAll = 2048;
 template <int DP>
-constexpr int sub_max() { return DP <= 16 ? 2048 : 1024; }   // 128 KiB of points
+constexpr int sub_max() { return DP <= 16 ? 2048 : (DP <= 32 ? 1024 : 512); }   // at most 128 KiB of points
 static_assert(kSubNodes * 256 == kSubMaxAll, "radix histograms alias the eqoff array");
 
-int subtree_max(int dpad) { return dpad <= 16 ? 2048 : 1024; }
+int subtree_max(int dpad) { return dpad <= 16 ? 2048 : (dpad <= 32 ? 1024 : 512); }
 
 __device__ __forceinline__ int sub_of(const int *sb, int nsub, int i) {
     int a = 0, b = nsub - 1;
@@ -1087,7 +1097,7 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
         if (host_maxc[lvl] <= subtree_max(dpad)) { L0 = lvl; break; }
     // the moving pass computes the next level's boxes when DP / 4 is a power
     // of two; boxes alternate between box and box2
-    const bool fused_box = splitting_type != 0 && (dpad == 4 || dpad == 8 || dpad == 16 || dpad == 32);
+    const bool fused_box = splitting_type != 0 && (dpad == 4 || dpad == 8 || dpad == 16 || dpad == 32 || dpad == 64);
     const int *src_idx = nullptr;   // set when the last level moved indices only
     bool orig_identity = true;      // orig is still 0..n-1 (read as the index itself)
     for (int lvl = 0; lvl < L0; ++lvl) {
@@ -1339,12 +1349,17 @@ hipError_t leaf_scan(const TreeView &t, const float *q, int nq, const int *order
                      hipStream_t st) {
     const int nleaves = 1 << t.depth;
     const size_t lds = (size_t)t.max_leaf * t.dpad * sizeof(float);
-    if (lds > 64 * 1024) return hipErrorInvalidValue;
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
     // chunks <= one per leaf + one per kScanChunk queries
     const unsigned nblk = (unsigned)(nleaves + (nq + kScanChunk - 1) / kScanChunk);
     return with_dp_km(t.dpad, K, [&](auto dp, auto km) {
         constexpr int DP = decltype(dp)::value;
         constexpr int KM = decltype(km)::value;
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute((const void *)k_leafscan<DP, KM>,
+                                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
         hipLaunchKernelGGL((k_leafscan<DP, KM>), dim3(nblk), dim3(256), lds, st, t, q, order, offset, chunk_off, K,
                            propleaf, out_d, out_p, out_l);
         return hipGetLastError();
@@ -1527,7 +1542,8 @@ __global__ __launch_bounds__(256) void k_exact(TreeView t, const float *__restri
 constexpr int kBlock = 256;
 constexpr int kBatch = 4;                            // candidates per thread between checks
 constexpr int kSetCap = 32 + kBatch * kBlock;        // live K <= 32 + one batch of appends
-constexpr int set_words(int cap) { return 4 * cap + 4 * 32 + 4; }  // key(2) pos pl | out key(2) pos pl | count
+constexpr int kSetOut = 64;                          // K <= 64 kept by bset_reduce
+constexpr int set_words(int cap) { return 4 * cap + 4 * kSetOut + 4; }  // key(2) pos pl | out key(2) pos pl | count
 constexpr int kSetWords = set_words(kSetCap);
 
 struct BlockSet {
@@ -1546,9 +1562,9 @@ __device__ __forceinline__ BlockSet bset_at(uint32_t *base, int cap = kSetCap) {
     S.pos = reinterpret_cast<int *>(base + 2 * cap);
     S.pl = S.pos + cap;
     S.okey = reinterpret_cast<uint64_t *>(base + 4 * cap);
-    S.opos = reinterpret_cast<int *>(base + 4 * cap + 64);
-    S.opl = S.opos + 32;
-    S.count = S.opl + 32;
+    S.opos = reinterpret_cast<int *>(base + 4 * cap + 2 * kSetOut);
+    S.opl = S.opos + kSetOut;
+    S.count = S.opl + kSetOut;
     return S;
 }
 
@@ -1955,7 +1971,7 @@ hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hi
 // The only dependent memory round trip per row is the candidates' load:
 // leaf ranges are computed (leaf_range) and the next row's query, natural
 // leaf and natural list are requested while the current row is scored.
-constexpr int kSweepCap = 32 + 4 * kWave;
+constexpr int kSweepCap = 64 + 4 * kWave;   // K <= 64 live + four wavefronts of appends
 
 template <int DP>
 __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__restrict__ q, int n_rows, int n_cols,
@@ -1982,13 +1998,14 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
     }
     for (int y = 1; y < n_rows; ++y) {
         const size_t qi = (size_t)y * n_cols + x;
-        // leaf k (lane k): occurrences, first occurrence, natural or not
-        uint32_t occ = 0;
+        // leaf k (lane k): occurrences (a mask over the K <= 64 lanes), first
+        // occurrence, natural or not
+        uint64_t occ = 0;
         bool first = true;
         for (int j = 0; j < K; ++j) {
             const int lj = readlane_i(my_l, j);
             if (lane < K && lj == my_l) {
-                occ |= 1u << j;
+                occ |= 1ull << j;
                 if (j < lane) first = false;
             }
         }
@@ -2001,12 +2018,12 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
         const int ch = uniq ? (c + kWave - 1) / kWave : 0;
         int inc = ch;
 #pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
+        for (int o = 1; o < kWave; o <<= 1) {
             const int v = __shfl_up(inc, o, kWave);
             if (lane >= o) inc += v;
         }
         const int chpre = inc - ch;
-        const int nch = readlane_i(inc, 31);
+        const int nch = readlane_i(inc, kWave - 1);
         const int nat_from = nat >> 1;
         // owner (lane k of a unique leaf) of chunk j, held by lane j
         int own_l = 0;
@@ -2076,9 +2093,9 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                 if (__ballot(take) == 0ull) continue;
                 touched = true;
                 const int pl = plc[u];
-                uint32_t m = (uint32_t)readlane_i((int)occ, own[u]);
+                uint64_t m = readlane_u64(occ, own[u]);
                 while (m) {
-                    const int k = __ffs(m) - 1;
+                    const int k = __builtin_ctzll(m);
                     m &= m - 1;
                     wset_append(S, n, take, make_key(s2, (uint32_t)(1 + K + k * stride + off[u])), pos[u], pl);
                     if (n > kSweepCap - kWave) {
@@ -2098,9 +2115,9 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
         // propagated leaves equal to the natural leaf: re-append the natural
         // list's entries below the bound with those leaves' arrivals
         {
-            uint32_t m = (uint32_t)natmask;
+            uint64_t m = natmask;
             while (m) {
-                const int k = __ffs(m) - 1;
+                const int k = __builtin_ctzll(m);
                 m &= m - 1;
                 const bool take = lane < K && cd < bound;
                 if (__ballot(take) == 0ull) continue;
@@ -2435,8 +2452,8 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL((k_project<DP, double>), dim3(blocks_for(n, 256)), dim3(256), lds, st, img, h, w, p,
                                basis_t64, mean64, d, out);
-        } else if (projection_dk(p) == 0) {
-            // p > 12: the VALU kernel (k_project)
+        } else if (projection_dk(p) == 0 || DP > 32) {
+            // p > 12 or d > 32: the VALU kernel (k_project)
             const size_t lds = (size_t)D * (DP + 1) * sizeof(float);
             if (lds > 160 * 1024) return hipErrorInvalidValue;
             hipError_t e = hipFuncSetAttribute((const void *)k_project<DP, float>,
@@ -2483,13 +2500,12 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
 
 size_t projection_pack_floats(int p, int dpad) {
     const int DK = projection_dk(p);
-    return DK == 0 ? 1 : (size_t)(dpad <= 16 ? 16 : 32) * DK + DK;
+    return (DK == 0 || dpad > 32) ? 1 : (size_t)(dpad <= 16 ? 16 : 32) * DK + DK;
 }
 
 hipError_t pack_projection(const float *basis_t, const float *mean_f, int p, int dpad, float *pack, hipStream_t st) {
     const int DK = projection_dk(p);
-    if (DK == 0) return hipSuccess;   // no table: the VALU kernel projects
-    if (dpad > 32) return hipErrorInvalidValue;
+    if (DK == 0 || dpad > 32) return hipSuccess;   // no table: the VALU kernel projects
     const int ctw = dpad <= 16 ? 16 : 32;
     hipLaunchKernelGGL(k_pack_projection, dim3(blocks_for((long long)ctw * DK, 256)), dim3(256), 0, st, basis_t,
                        mean_f, p, dpad, DK, ctw, pack, pack + (size_t)ctw * DK);

@@ -40,9 +40,12 @@ extern "C" {
 #define ANNF_SPLIT_CYCLIC 0    /* global.h:14 SPLITTING_TYPE_CYCLIC         */
 #define ANNF_SPLIT_LONGEST 1   /* global.h:15 SPLITTING_TYPE_LONGEST_BOX    */
 
-/* Build limits: n_neighbors <= ANNF_MAX_K, reduced dimension <= ANNF_MAX_D. */
-#define ANNF_MAX_K 32
-#define ANNF_MAX_D 32
+/* Build limits: n_neighbors <= ANNF_MAX_K (the full-patch entries
+ * annf_field_mm_dev: <= 32), reduced dimension <= ANNF_MAX_D (beyond them:
+ * ANNF_EUNSUPPORTED; the reference accepts n_neighbors up to 1000,
+ * helper.c:32, and any dimension). */
+#define ANNF_MAX_K 64
+#define ANNF_MAX_D 64
 
 typedef struct annf_ctx annf_ctx;
 

@@ -308,3 +308,15 @@ def test_front_fp64_matches_oracle_1080p(gpu_available):
     out, res, ra, rb = _front_vs_oracle(a, b, 11, 16, 64, 5, 50000)
     assert out["red_differ"] <= 1e-5 * out["red_values"]
     assert out["nn_mismatch"] <= 1e-4 * out["n"]
+
+
+def test_front_runs_the_reference_example_configuration(gpu_available):
+    """The reference's example call (examples/simple.py:16-28: n_neighbors 8,
+    psize 8, dim_reduced 40, leaf_size 128, propagation and best-of-K on)
+    through the HIP front on a crop of its example photographs, FP64
+    projection: the field equals the oracle's on the same images."""
+    from conftest import golden
+    g = golden("example_crop.npz")
+    out, res, ra, rb = _front_vs_oracle(g["a"], g["b"], 8, 40, 128, 8, 1000)
+    assert out["nn_mismatch"] == 0 or out["red_differ"] > 0
+    assert out["nn_mismatch"] <= 1e-4 * out["n"]

@@ -69,6 +69,9 @@ def test_process_rows_matches_reference(name, gpu_available):
     (65536, 10, "ties", 32),
     (40000, 3, "dups", 16),
     (5000, 32, "gauss", 8),
+    (30001, 40, "gauss", 16),
+    (20000, 64, "ties", 16),
+    (12000, 47, "dups", 8),
 ])
 def test_tree_matches_oracle(n, d, kind, leaf, gpu_available):
     rng = np.random.default_rng(n + d)
@@ -122,11 +125,11 @@ def test_cyclic_split_matches_oracle_rule(gpu_available):
         assert np.all(arr["axis"][(1 << lvl) - 1:(1 << (lvl + 1)) - 1] == lvl % 5)
 
 
-@pytest.mark.parametrize("K", [1, 3, 8, 12])
-def test_exact_search_matches_oracle(K, gpu_available):
-    rng = np.random.default_rng(K)
-    X = rng.standard_normal((20000, 8)).astype(np.float32)
-    Q = rng.standard_normal((500, 8)).astype(np.float32)
+@pytest.mark.parametrize("K,d", [(1, 8), (3, 8), (8, 8), (12, 8), (8, 40), (5, 64), (40, 8), (64, 12)])
+def test_exact_search_matches_oracle(K, d, gpu_available):
+    rng = np.random.default_rng(K + d)
+    X = rng.standard_normal((20000, d)).astype(np.float32)
+    Q = rng.standard_normal((500, d)).astype(np.float32)
     depth = O.tree_depth(20000, 32)
     ref = O.OracleTree(X, depth)
     t = _fit(X, depth, K)
@@ -178,27 +181,27 @@ def _field_vs_oracle(a, b, p, d, leaf, K, propagate=True, select_best_nn=True, p
 
 
 @pytest.mark.parametrize("K,propagate,select", [(1, True, True), (5, True, True), (3, False, True),
-                                                (4, True, False)])
+                                                (4, True, False), (40, True, True), (64, True, True)])
 def test_field_256_matches_oracle(K, propagate, select, gpu_available):
     from annfmp_amd import synth
     a, b = synth.make_pair("256x256-shift", seed=0)
     _field_vs_oracle(a, b, 7, 10, 32, K, propagate, select)
 
 
-@pytest.mark.parametrize("p,d", [(5, 12), (9, 24), (13, 8)])
+@pytest.mark.parametrize("p,d", [(5, 12), (9, 24), (13, 8), (7, 40), (5, 64)])
 def test_field_fp32_projection(p, d, gpu_available):
     """FP32 projection: the MFMA GEMM with one (d <= 16) and two (d > 16)
-    component tiles, and the CUDA-core kernel for p > 12."""
+    component tiles, and the VALU kernel for p > 12 or d > 32."""
     from annfmp_amd import synth
     a, b = synth.make_pair(synth.small_spec(96, 128), seed=4)
     _field_vs_oracle(a, b, p, d, 16, 3, projection="fp32")
 
 
-@pytest.mark.parametrize("d", [16, 32])
+@pytest.mark.parametrize("d", [16, 32, 40])
 def test_field_example_photographs_match_oracle(d, gpu_available):
     """Real photographs: a 160 x 240 crop of the reference's example pair,
-    at its example's K = 8, p = 8, leaf 128 (examples/simple.py:16-28); its
-    dim_reduced = 40 exceeds this build's MAX_D, so d = 16 and 32."""
+    at its example's K = 8, p = 8, leaf 128 (examples/simple.py:16-28),
+    with its dim_reduced = 40 and two smaller dimensions."""
     g = golden("example_crop.npz")
     _field_vs_oracle(g["a"], g["b"], 8, d, 128, 8)
 
@@ -210,7 +213,7 @@ def test_field_1080p_matches_oracle(gpu_available):
     _field_vs_oracle(a, b, 11, 16, 64, 5, projection="fp32")
 
 
-@pytest.mark.parametrize("K", [2, 8, 20])
+@pytest.mark.parametrize("K", [2, 8, 20, 48])
 def test_exact_search_ties_match_oracle(K, gpu_available):
     """Many equal distances across leaves: integer points with duplicates and
     integer queries; the tie order must be the reference's visit order."""
@@ -277,3 +280,11 @@ def test_exact_field_p11_matches_brute_force(gpu_available):
     idx, d2 = metrics.exact_knn(a, b, p, q, 8)
     ridx, rd2 = O.exact_field(pa[q], pb, 8)
     assert np.array_equal(idx, ridx) and np.array_equal(d2, rd2)
+
+
+def test_field_high_dimension_many_neighbours_match_oracle(gpu_available):
+    """d = 48 and K = 64 together (the widest lists and points of the build):
+    tree, row 0, propagation sweep and best-of-K against the oracle."""
+    from annfmp_amd import synth
+    a, b = synth.make_pair(synth.small_spec(90, 120), seed=31)
+    _field_vs_oracle(a, b, 6, 48, 128, 64)
