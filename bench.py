@@ -47,6 +47,7 @@ MM_STAGES = ["project_b", "launch_project_a", "tree_build_with_project_a", "natu
 DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz")
 DEFAULT_ENCODER_4K = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p9_d16.npz")
 FP8_PEAK_TFS = 5000.0          # MI355X dense FP8 MFMA, 5.0 PFLOP/s
+FP32_PEAK_TFS = 157.3          # MI355X FP32, 157.3 TFLOP/s
 
 
 def fit_basis(a, p, d, n_samples, seed):
@@ -55,26 +56,104 @@ def fit_basis(a, p, d, n_samples, seed):
     return fit_pca(a, p, d, n_samples, seed)
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+class _quiet_stdout:
+    """Sends C-level stdout to /dev/null (the reference core prints its
+    progress), so that this script's one JSON line stays alone."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+        return self
+
+    def __exit__(self, *exc):
+        sys.stdout.flush()
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+
+
 def cpu_baseline(a, b, p, red_a, red_b, depth, K, gpu_nn):
-    """The oracle (C restatement of the openmp backend, OpenMP) on the same
-    float32 reduced arrays: tree build + rows + conversion + best-of-K."""
+    """The plain method on the host cores on the same float32 reduced arrays
+    (tree build + row 0, propagation, conversion and best-of-K): the
+    reference's own C core (oracle/_ref, compiled from its sources) when it
+    is built, else the C restatement; on every host core of this process and
+    on one thread."""
     from oracle import oracle as O
-    threads = min(len(os.sched_getaffinity(0)), 16)
+    cores = len(os.sched_getaffinity(0))
     pa = O.extract_patches(a, p)
     pb = O.extract_patches(b, p)
     nr, nc = a.shape[0] - p + 1, a.shape[1] - p + 1
-    t0 = time.time()
-    ref = O.OracleTree(red_b, depth)
-    t1 = time.time()
-    final, _, _ = ref.process_rows(red_a, nr, nc, K, True, True, pa, pb, nthreads=threads)
-    t2 = time.time()
     n = red_a.shape[0]
+    kind = "reference" if O.ref_available() else "port"
+
+    def run(threads):
+        t0 = time.time()
+        if kind == "reference":
+            with _quiet_stdout():
+                ref = O.RefTree(red_b, depth, K=K, num_threads=threads)
+                t1 = time.time()
+                final = ref.process_rows(pa, pb, red_a, nr, nc, K, True, True)
+        else:
+            ref = O.OracleTree(red_b, depth)
+            t1 = time.time()
+            final, _, _ = ref.process_rows(red_a, nr, nc, K, True, True, pa, pb, nthreads=threads)
+        t2 = time.time()
+        return final, t1 - t0, t2 - t1
+
+    f_all, fit_all, rows_all = run(cores)
+    f_one, fit_one, rows_one = run(1)
+    parity = bool(np.array_equal(f_all, gpu_nn) and np.array_equal(f_one, gpu_nn))
     return {
-        "value": round(n / (t2 - t0), 1), "unit": "query patches/s", "cores": threads, "kind": "port",
-        "sample": f"plain path on the full 1920x1080 pair, {n} queries: kdtree_fit {t1 - t0:.2f} s (serial, as "
-                  f"the reference) + rows/conversion/best-of-K {t2 - t1:.2f} s on {threads} threads, same float32 "
-                  f"reduced arrays as the GPU run",
-    }, bool(np.array_equal(final, gpu_nn))
+        "value": round(n / (fit_all + rows_all), 1), "unit": "query patches/s", "cores": cores, "kind": kind,
+        "value_1thread": round(n / (fit_one + rows_one), 1), "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+        "sample": ("plain path on the full 1920x1080 pair, %d queries, same float32 reduced arrays as the GPU run: "
+                   "%s; %d threads: kdtree_fit %.2f s (serial, as the reference) + rows/conversion/best-of-K "
+                   "%.2f s; 1 thread: %.2f s + %.2f s"
+                   % (n, "the reference's own openmp C core (fit_extern + process_rows_extern)"
+                      if kind == "reference" else "the C restatement of the openmp core",
+                      cores, fit_all, rows_all, fit_one, rows_one)),
+    }, parity
+
+
+def path_roofline(p, d, n, n_leaf_pts, ms_per_step, learned):
+    """Path-wide roofline of the measured path: t_roof = sum over stages of
+    max(algorithmic bytes / HBM peak, algorithmic flops / MFMA peak) per
+    query, against the measured time per query.  Stages: projection of A
+    and B (encoder MLP 363-128-128-16 on BF16 MFMA, or PCA on FP32), the
+    tree build over B, natural leaf + bucketing, the BF16 full-patch leaf
+    k-NN over one leaf."""
+    D = p * p * 3
+    bw = HBM_PEAK_GBS * 1e9
+    img = 2 * 3.0   # two images, 3 bytes per pixel, about one pixel per patch
+    if learned:
+        s1_flops, s1_peak = 2 * 2 * (D * 128 + 128 * 128 + 128 * d), BF16_PEAK_TFS * 1e12
+    else:
+        s1_flops, s1_peak = 2 * 2 * D * d, FP32_PEAK_TFS * 1e12
+    stages = {
+        "project_a_b": (img + 2 * 4 * d, s1_flops, s1_peak),
+        "tree_build": (4 * d + (4 * d + 4), 0.0, 1.0),
+        "natural_leaf_bucket": (4 * d + 4, 0.0, 1.0),
+        "leaf_knn_bf16": (img + 4 + 4 + 4 * 5, 2.0 * D * n_leaf_pts, BF16_PEAK_TFS * 1e12),
+    }
+    t = {k: max(bts / bw, fl / pk) for k, (bts, fl, pk) in stages.items()}
+    t_roof = sum(t.values())
+    t_meas = ms_per_step * 1e-3 / n
+    return {"t_roof_ps_per_query": round(t_roof * 1e12, 2), "t_measured_ps_per_query": round(t_meas * 1e12, 2),
+            "frac": round(t_roof / t_meas, 4), "ceiling_query_patches_per_s": round(1.0 / t_roof, 1),
+            "stages_ps": {k: round(v * 1e12, 2) for k, v in t.items()},
+            "peaks": {"hbm_GBs": HBM_PEAK_GBS, "bf16_TFs": BF16_PEAK_TFS, "fp32_TFs": FP32_PEAK_TFS}}
 
 
 def measured_traffic(kernel):
@@ -82,7 +161,7 @@ def measured_traffic(kernel):
     summary (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or
     None when the summary has no line for it."""
     import re
-    path = os.path.join(ROOT, "profiles", "field_1080p_pmc.md")
+    path = os.path.join(ROOT, "profiles", "leaf_knn_bf16_1080p_pmc.md")
     if not os.path.exists(path):
         return None
     m = re.search(r"traffic_bytes_per_launch " + re.escape(kernel) + r": (\d+)", open(path).read())
@@ -327,10 +406,11 @@ def main():
     leaf_ms = main_path["stage_ms"]["leaf_knn_mfma"]
     flops = 2.0 * (p * p * 3) * n_a * args.nprobe * (n_b / (1 << depth))
     achieved = flops / (leaf_ms * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_leaf_mm (BF16)", "achieved": round(achieved, 3),
+    roofline = {"bound": "mfma", "kernel": "k_leaf_bf16<28, 5, true> (BF16)", "achieved": round(achieved, 3),
                 "peak": BF16_PEAK_TFS, "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFS, 6),
-                "traffic": measured_traffic("k_leaf_mm<400, 5, false>"), "algorithmic_flops": flops,
+                "traffic": measured_traffic("k_leaf_bf16<28, 5, true>"), "algorithmic_flops": flops,
                 "avg_ms": round(float(leaf_ms), 4)}
+    path_roof = path_roofline(p, d, n_a, args.nprobe * n_b / (1 << depth), ms_per_step, chosen == "learned_bf16")
 
     cpu = None
     parity = None
@@ -354,7 +434,8 @@ def main():
         "speedup_vs_plain_fp32": round(value / plain_res["query_patches_per_s"], 3),
         "recall@5_vs_plain_fp32": [round(main_path["recall@5"], 4), round(plain_rec, 4)],
         "recall_at_least_plain": bool(main_path["recall@5"] >= plain_rec),
-        "paths": paths, "roofline": roofline, "cpu_baseline": cpu, "parity_vs_oracle": parity,
+        "paths": paths, "roofline": roofline, "path_roofline": path_roof, "cpu_baseline": cpu,
+        "parity_vs_oracle": parity,
     }
     print(json.dumps(out), flush=True)
 

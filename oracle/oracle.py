@@ -437,13 +437,14 @@ def ref_lib():
 class RefTree:
     """The reference's own tree (fit_extern), with arrays copied out."""
 
-    def __init__(self, X, depth, K=1, max_leaves=2147483640):
+    def __init__(self, X, depth, K=1, max_leaves=2147483640, num_threads=1):
         L = ref_lib()
         self.X = np.ascontiguousarray(X, dtype=np.float32)   # borrowed by the record
         n, d = self.X.shape
         self.n, self.d, self.depth = n, d, depth
         self.params = _RefParams()
-        L.init_extern(K, depth, max_leaves, 1, 1, 0, C.byref(self.params))
+        # init_extern sets the process-wide OpenMP thread count (base.c:41)
+        L.init_extern(K, depth, max_leaves, num_threads, 1, 0, C.byref(self.params))
         self.rec = _RefRecord()
         L.fit_extern(_ptr(self.X, _fp), n, d, C.byref(self.rec), C.byref(self.params))
         nn = (1 << depth) - 1
