@@ -1583,6 +1583,20 @@ __device__ __forceinline__ void bset_append(const BlockSet &S, bool take, uint64
     }
 }
 
+// Number of keys among key[0, n) below kj, n >= 1; eight keys are read at a
+// time (all in flight before the first compare).
+__device__ __forceinline__ int rank_below(const uint64_t *key, int n, uint64_t kj) {
+    int r = 0;
+    for (int i = 0; i < n; i += 8) {
+        uint64_t kk[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) kk[u] = key[min(i + u, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) r += (i + u < n && kk[u] < kj) ? 1 : 0;
+    }
+    return r;
+}
+
 // Keeps the K smallest keys, ascending, in [0, min(n, K)).  Block-wide.
 __device__ __forceinline__ void bset_reduce(const BlockSet &S, int K) {
     __syncthreads();
@@ -1590,8 +1604,7 @@ __device__ __forceinline__ void bset_reduce(const BlockSet &S, int K) {
     if (n > K) {
         for (int j = threadIdx.x; j < n; j += blockDim.x) {
             const uint64_t kj = S.key[j];
-            int r = 0;
-            for (int i = 0; i < n; ++i) r += (S.key[i] < kj) ? 1 : 0;
+            const int r = rank_below(S.key, n, kj);
             if (r < K) {
                 S.okey[r] = kj;
                 S.opos[r] = S.pos[j];
@@ -1615,7 +1628,7 @@ __device__ __forceinline__ void bset_reduce(const BlockSet &S, int K) {
             kj = S.key[threadIdx.x];
             pj = S.pos[threadIdx.x];
             lj = S.pl[threadIdx.x];
-            for (int i = 0; i < n; ++i) r += (S.key[i] < kj) ? 1 : 0;
+            r = rank_below(S.key, n, kj);
         }
         __syncthreads();
         if (mine) {
@@ -1647,8 +1660,9 @@ __device__ __forceinline__ void wset_append(const BlockSet &S, int &n, bool take
 }
 
 // bset_reduce for a one-wavefront block: with at most 64 items each lane
-// holds one item and ranks it with readlane; lane j < K receives the j-th
-// smallest (returned in opos/opl) and writes it back as item j.
+// holds one item and ranks it against every key (LDS broadcast reads, eight
+// in flight); the items of rank < K move to slots 0..K-1, and lane j < K
+// returns item j's position and leaf (opos/opl; other lanes keep theirs).
 __device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, int lane, int &opos, int &opl) {
     wave_sync();
     if (n > kWave) {
@@ -1662,25 +1676,20 @@ __device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, in
     const bool have = lane < n;
     const uint64_t mk = have ? S.key[lane] : ~0ull;
     const int mp = have ? S.pos[lane] : 0, ml = have ? S.pl[lane] : 0;
-    int r = 0;
-    for (int i = 0; i < n; ++i) r += (readlane_u64(mk, i) < mk) ? 1 : 0;
+    const int r = n > 0 ? rank_below(S.key, n, mk) : 0;
     const int m = min(n, K);
-    uint64_t ok = 0;
-    for (int j = 0; j < m; ++j) {
-        const unsigned long long b = __ballot(have && r == j);
-        const int o = __ffsll(b) - 1;
-        const uint64_t kj = readlane_u64(mk, o);
-        const int pj = readlane_i(mp, o), lj = readlane_i(ml, o);
-        if (lane == j) { ok = kj; opos = pj; opl = lj; }
+    wave_sync();
+    if (have && r < m) {
+        S.key[r] = mk;
+        S.pos[r] = mp;
+        S.pl[r] = ml;
     }
     wave_sync();
     if (lane < m) {
-        S.key[lane] = ok;
-        S.pos[lane] = opos;
-        S.pl[lane] = opl;
+        opos = S.pos[lane];
+        opl = S.pl[lane];
     }
     n = m;
-    wave_sync();
 }
 
 // Exact k-NN search as a bounded, order-free scan (one workgroup per query).
@@ -1692,17 +1701,33 @@ __device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, in
 // (q_a - s)^2 > dK(current) >= D* >= d(p) >= (q_a - p_a)^2 >= (q_a - s)^2
 // (FP32 subtraction, squaring and sequential sums of non-negative terms are
 // monotone), a contradiction.  So the reference's list is the K smallest keys
-// among {p : d(p) <= D*}.  Any radius U >= D* therefore finds a superset: the
-// K-th distance of the natural leaf is such a U.  Arrival order is the
+// among {p : d(p) <= D*}, and any set of scanned points that holds every
+// point with d(p) <= D* gives the same list.  Arrival order is the
 // depth-first leaf order -- near child first -- which is the integer order
 // of the "went far" bits read from the root, followed by the position in the
 // leaf; that pair replaces the arrival number in the key.
 //
+// Which leaves are scanned.  U, the K-th distance of the natural leaf, is
+// >= D*.  A leaf is queued when every far turn on its path passes
+// (q_a - s)^2 <= U (the reference's own test with a radius >= D*).  A
+// queued leaf's cell bound is LB = sum_a g_a^2, g_a the largest |q_a - s|
+// over the far turns on axis a: a point of the leaf lies on the far side of
+// each of those splits (left child <= split <= right child), so
+// |q_a - p_a| >= g_a and its exact distance is >= LB.  The FP32 roundings of
+// LB and of d(p) (sums of at most 64 non-negative terms) are each within
+// 70 units of 2^-24, so d(p) >= LB (1 - 2^-14): a leaf with
+// LB (1 - 2^-14) > U' for any U' >= D* holds no list member.  Queued leaves
+// are scanned in ascending LB, eight at a time; U' is the K-th distance of
+// the list so far, and the scan stops at the first leaf whose bound exceeds
+// it (every later one has a larger bound).
+//
 // Per query: near/pass bits of every internal node go to LDS (coalesced
 // passes over the node arrays); every leaf is tested against its path; the
-// passing leaves are queued in LDS and scanned by the whole workgroup, one
-// leaf per wavefront per step.
+// passing leaves are queued in LDS, bounded, sorted (bitonic) and scanned by
+// the whole workgroup, one leaf per wavefront per step.
 constexpr int kExactQueue = 1024;
+constexpr int kNodeUnroll = 8;
+constexpr float kCellSafe = 1.0f - 0x1p-14f;
 
 template <int DP>
 __global__ __launch_bounds__(kBlock) void k_exact_bound(TreeView t, const float *__restrict__ q, int nq, int K,
@@ -1714,31 +1739,48 @@ __global__ __launch_bounds__(kBlock) void k_exact_bound(TreeView t, const float 
     const BlockSet S = bset_at(smem_u);
     uint32_t *nearb = smem_u + kSetWords;
     uint32_t *passb = nearb + words;
-    int *qfrom = reinterpret_cast<int *>(passb + words);
-    int *qcnt = qfrom + kExactQueue;
-    int *qrank = qcnt + kExactQueue;
-    int *qn_p = qrank + kExactQueue;
+    uint64_t *qkey = reinterpret_cast<uint64_t *>(passb + words);   // (bound bits << 32) | queue slot
+    int *qleaf = reinterpret_cast<int *>(qkey + kExactQueue);
+    int *qrank = qleaf + kExactQueue;
+    float *qs = reinterpret_cast<float *>(qrank + kExactQueue);     // the query, for indexing by axis
+    int *qn_p = reinterpret_cast<int *>(qs + DP);
+    int *nv_p = qn_p + 1;
     const int w = blockIdx.x;
     if (w >= nq) return;
     float qv[DP];
     load_vec<DP>(q + (size_t)w * DP, qv);
     const float qlane = lane < DP ? q[(size_t)w * DP + lane] : 0.0f;
+    if (tid < DP) qs[tid] = q[(size_t)w * DP + tid];
     const int nn = (1 << t.depth) - 1;
     const int nleaves = 1 << t.depth;
 
-    // near bits: right child first iff q[axis] - split >= 0
-    for (int base = 0; base < nn; base += kBlock) {
-        const int n = base + tid;
-        const int a = n < nn ? t.axis[n] : 0;
-        const float sv = n < nn ? t.split[n] : 0.0f;
-        const float diff = __shfl(qlane, a, kWave) - sv;
-        const unsigned long long b = __ballot(n < nn && diff >= 0.0f);
-        const int wb = base + wid * kWave;
-        if (lane == 0 && wb < nn) {
-            nearb[wb >> 5] = (uint32_t)b;
-            nearb[(wb >> 5) + 1] = (uint32_t)(b >> 32);
+    // near bits: right child first iff q[axis] - split >= 0; pass bits: far
+    // child allowed iff (q[axis] - split)^2 <= u.  kNodeUnroll groups of
+    // kBlock nodes a step, their loads issued together.
+    auto node_bits = [&](uint32_t *bits, bool pass, float u) {
+        for (int base = 0; base < nn; base += kNodeUnroll * kBlock) {
+            int a[kNodeUnroll];
+            float sv[kNodeUnroll];
+#pragma unroll
+            for (int g = 0; g < kNodeUnroll; ++g) {
+                const int n = base + g * kBlock + tid;
+                a[g] = n < nn ? t.axis[n] : 0;
+                sv[g] = n < nn ? t.split[n] : 0.0f;
+            }
+#pragma unroll
+            for (int g = 0; g < kNodeUnroll; ++g) {
+                const int n = base + g * kBlock + tid;
+                const float diff = __shfl(qlane, a[g], kWave) - sv[g];
+                const unsigned long long b = __ballot(n < nn && (pass ? diff * diff <= u : diff >= 0.0f));
+                const int wb = base + g * kBlock + wid * kWave;
+                if (lane == 0 && wb < nn) {
+                    bits[wb >> 5] = (uint32_t)b;
+                    bits[(wb >> 5) + 1] = (uint32_t)(b >> 32);
+                }
+            }
         }
-    }
+    };
+    node_bits(nearb, false, 0.0f);
     if (tid == 0) *S.count = 0;
     __syncthreads();
 
@@ -1759,19 +1801,7 @@ __global__ __launch_bounds__(kBlock) void k_exact_bound(TreeView t, const float 
     const float U = __uint_as_float((uint32_t)(S.key[K - 1] >> 32));
     __syncthreads();
 
-    // pass bits: far child allowed iff (q[axis] - split)^2 <= U
-    for (int base = 0; base < nn; base += kBlock) {
-        const int n = base + tid;
-        const int a = n < nn ? t.axis[n] : 0;
-        const float sv = n < nn ? t.split[n] : 0.0f;
-        const float diff = __shfl(qlane, a, kWave) - sv;
-        const unsigned long long b = __ballot(n < nn && diff * diff <= U);
-        const int wb = base + wid * kWave;
-        if (lane == 0 && wb < nn) {
-            passb[wb >> 5] = (uint32_t)b;
-            passb[(wb >> 5) + 1] = (uint32_t)(b >> 32);
-        }
-    }
+    node_bits(passb, true, U);
     // the reference's initial list: K sentinels, arrivals 0..K-1
     const int pl0 = propleaf ? propleaf[0] : 0;
     if (tid < K) {
@@ -1783,63 +1813,112 @@ __global__ __launch_bounds__(kBlock) void k_exact_bound(TreeView t, const float 
     __syncthreads();
     const uint64_t ubound = (U < kFltMax) ? (((uint64_t)__float_as_uint(U) << 32) | 0xFFFFFFFFull) : ~0ull;
     uint64_t bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
+    float ucur = __uint_as_float((uint32_t)(bkey >> 32));   // >= D*: the K-th distance so far, at most U
     const uint32_t seq0 = (uint32_t)K;
 
-    // scan of the queued leaves: wavefront wid takes leaves wid, wid+4, ...;
-    // each lane keeps up to kBatch points in flight
-    auto scan_queue = [&](int count) {
-        for (int j0 = 0; j0 < count; j0 += 4 * (kBatch / 2)) {
-            float pt[kBatch][DP];
-            int pos[kBatch];
-            bool val[kBatch];
-            uint32_t sq[kBatch];
+    // the queued leaves [0, count): cell bounds, sort, scan in ascending bound
+    auto process_queue = [&](int count) {
+        if (tid == 0) *nv_p = 0;
+        __syncthreads();
+        for (int j = tid; j < count; j += kBlock) {
+            const int l = qleaf[j];
+            int ax[16];
+            float sp[16];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                const int j = j0 + wid + 4 * (u >> 1);
-                const int off = lane + (u & 1) * kWave;
-                const bool jv = j < count;
-                const int f = jv ? qfrom[j] : 0, c = jv ? qcnt[j] : 0;
-                const uint32_t rk = jv ? ((uint32_t)qrank[j] << offbits) : 0u;
-                val[u] = off < c;
-                pos[u] = f + (val[u] ? off : 0);
-                sq[u] = seq0 + (rk | (uint32_t)off);
-                if (val[u]) load_vec<DP>(t.pts + (size_t)pos[u] * DP, pt[u]);
+            for (int lev = 0; lev < 16; ++lev) {   // depth <= 15: the path's nodes, loads issued together
+                const int nd = lev < t.depth ? (1 << lev) - 1 + (l >> (t.depth - lev)) : 0;
+                ax[lev] = lev < t.depth ? t.axis[nd] : 0;
+                sp[lev] = lev < t.depth ? t.split[nd] : 0.0f;
             }
+            float gap[DP];
 #pragma unroll
-            for (int u = 0; u < kBatch; ++u) {
-                float s2 = 0.0f;
-                if (val[u]) {
+            for (int jj = 0; jj < DP; ++jj) gap[jj] = 0.0f;
+            int rank = 0;
 #pragma unroll
-                    for (int jj = 0; jj < DP; ++jj) {
-                        const float dd = pt[u][jj] - qv[jj];
-                        s2 = s2 + dd * dd;
+            for (int lev = 0; lev < 16; ++lev) {
+                if (lev < t.depth) {
+                    const int dir = (l >> (t.depth - 1 - lev)) & 1;
+                    const float diff = qs[ax[lev]] - sp[lev];
+                    const int far = dir ^ (int)(diff >= 0.0f);
+                    rank = (rank << 1) | far;
+                    const float g = far ? fabsf(diff) : 0.0f;
+#pragma unroll
+                    for (int jj = 0; jj < DP; ++jj) gap[jj] = (jj == ax[lev]) ? fmaxf(gap[jj], g) : gap[jj];
+                }
+            }
+            float lb = 0.0f;
+#pragma unroll
+            for (int jj = 0; jj < DP; ++jj) lb = lb + gap[jj] * gap[jj];
+            const float lbs = lb * kCellSafe;
+            const bool keep = !(lbs > ucur);
+            qkey[j] = keep ? (((uint64_t)__float_as_uint(lbs) << 32) | (uint32_t)j) : ~0ull;
+            qrank[j] = rank;
+            const unsigned long long m = __ballot(keep);
+            if (lane_rank(m) == 0 && keep) atomicAdd(nv_p, __popcll(m));
+        }
+        int P = 1;
+        while (P < count) P <<= 1;
+        for (int j = count + tid; j < P; j += kBlock) qkey[j] = ~0ull;
+        // bitonic sort of qkey[0, P)
+        for (int k = 2; k <= P; k <<= 1)
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                __syncthreads();
+                for (int i = tid; i < P; i += kBlock) {
+                    const int ixj = i ^ jj;
+                    if (ixj > i) {
+                        const uint64_t x = qkey[i], y = qkey[ixj];
+                        if ((x > y) == ((i & k) == 0)) {
+                            qkey[i] = y;
+                            qkey[ixj] = x;
+                        }
                     }
                 }
-                const uint64_t key = make_key(s2, sq[u]);
-                const bool take = val[u] && key < bkey;
-                bset_append(S, take, key, pos[u], (take && propleaf) ? propleaf[pos[u]] : 0);
             }
-            __syncthreads();
-            if (*S.count > kSetCap - kBatch * kBlock) {
-                bset_reduce(S, K);
-                bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
+        __syncthreads();
+        const int nvalid = *nv_p;
+        // eight leaves a step: wavefront wid takes sorted leaves j0 + wid and
+        // j0 + wid + 4; each lane keeps up to kBatch points in flight
+        for (int j0 = 0; j0 < nvalid; j0 += 4 * (kBatch / 2)) {
+            if (__uint_as_float((uint32_t)(qkey[j0] >> 32)) > ucur) break;   // every later leaf is farther
+            int lf[kBatch / 2], lc[kBatch / 2];
+            uint32_t lr[kBatch / 2];
+#pragma unroll
+            for (int h = 0; h < kBatch / 2; ++h) {
+                const int j = j0 + wid + 4 * h;
+                const bool jv = j < nvalid;
+                const int slot = jv ? (int)(uint32_t)qkey[j] : 0;
+                int f = 0, e = 0;
+                if (jv) leaf_range(qleaf[slot], t.n, t.depth, f, e);
+                lf[h] = f;
+                lc[h] = e - f;
+                lr[h] = jv ? ((uint32_t)qrank[slot] << offbits) : 0u;
             }
-            // leaves longer than 2 * 64 points continue here
-            for (int ob = 2 * kWave; ob < t.max_leaf; ob += 2 * kWave) {
+            for (int ob = 0; ob < t.max_leaf; ob += 2 * kWave) {
+                float pt[kBatch][DP];
+                int pos[kBatch];
+                bool val[kBatch];
+                uint32_t sq[kBatch];
 #pragma unroll
                 for (int u = 0; u < kBatch; ++u) {
-                    const int j = j0 + wid + 4 * (u >> 1);
                     const int off = ob + lane + (u & 1) * kWave;
-                    const bool jv = j < count;
-                    const int f = jv ? qfrom[j] : 0, c = jv ? qcnt[j] : 0;
-                    const uint32_t rk = jv ? ((uint32_t)qrank[j] << offbits) : 0u;
-                    const bool v = off < c;
+                    val[u] = off < lc[u >> 1];
+                    pos[u] = lf[u >> 1] + (val[u] ? off : 0);
+                    sq[u] = seq0 + (lr[u >> 1] | (uint32_t)off);
+                    if (val[u]) load_vec<DP>(t.pts + (size_t)pos[u] * DP, pt[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < kBatch; ++u) {
                     float s2 = 0.0f;
-                    const int pp = f + (v ? off : 0);
-                    if (v) s2 = dist_sq<DP>(t.pts + (size_t)pp * DP, qv);
-                    const uint64_t key = make_key(s2, seq0 + (rk | (uint32_t)off));
-                    const bool take = v && key < bkey;
-                    bset_append(S, take, key, pp, (take && propleaf) ? propleaf[pp] : 0);
+                    if (val[u]) {
+#pragma unroll
+                        for (int jj = 0; jj < DP; ++jj) {
+                            const float dd = pt[u][jj] - qv[jj];
+                            s2 = s2 + dd * dd;
+                        }
+                    }
+                    const uint64_t key = make_key(s2, sq[u]);
+                    const bool take = val[u] && key < bkey;
+                    bset_append(S, take, key, pos[u], (take && propleaf) ? propleaf[pos[u]] : 0);
                 }
                 __syncthreads();
                 if (*S.count > kSetCap - kBatch * kBlock) {
@@ -1847,45 +1926,44 @@ __global__ __launch_bounds__(kBlock) void k_exact_bound(TreeView t, const float 
                     bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
                 }
             }
+            // the bound after this step
+            bset_reduce(S, K);
+            bkey = ubound < S.key[K - 1] ? ubound : S.key[K - 1];
+            ucur = __uint_as_float((uint32_t)(bkey >> 32));
         }
+        __syncthreads();
     };
-    for (int lb = 0; lb < nleaves; lb += kBlock) {
-        const int l = lb + tid;
-        bool ok = l < nleaves;
-        int rank = 0, nd = 0, lo = 0, hi = t.n;
-        for (int lev = t.depth - 1; lev >= 0; --lev) {
-            const int dir = (l >> lev) & 1;
-            const int nr = (int)((nearb[nd >> 5] >> (nd & 31)) & 1u);
-            const int pb = (int)((passb[nd >> 5] >> (nd & 31)) & 1u);
-            const int far = dir ^ nr;
-            ok = ok && (!far || pb);
-            rank = (rank << 1) | far;
-            nd = 2 * nd + 1 + dir;
-            const int mid = lo + ((hi - lo) >> 1);
-            if (dir) lo = mid; else hi = mid;
-        }
-        const unsigned long long m = __ballot(ok);
-        const int cnt = __popcll(m);
-        int base = 0;
-        if (lane == 0 && cnt) base = atomicAdd(qn_p, cnt);
-        base = __builtin_amdgcn_readlane(base, 0);
-        if (ok) {
-            const int slot = base + lane_rank(m);
-            qfrom[slot] = lo;
-            qcnt[slot] = hi - lo;
-            qrank[slot] = rank;
+    // every leaf against its path, two leaves per thread a step
+    for (int lb = 0; lb < nleaves; lb += 2 * kBlock) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int l = lb + h * kBlock + tid;
+            bool ok = l < nleaves;
+            int nd = 0;
+            for (int lev = t.depth - 1; lev >= 0; --lev) {
+                const int dir = (l >> lev) & 1;
+                const int nr = (int)((nearb[nd >> 5] >> (nd & 31)) & 1u);
+                const int pb = (int)((passb[nd >> 5] >> (nd & 31)) & 1u);
+                ok = ok && (dir == nr || pb);
+                nd = 2 * nd + 1 + dir;
+            }
+            const unsigned long long m = __ballot(ok);
+            const int cnt = __popcll(m);
+            int base = 0;
+            if (lane == 0 && cnt) base = atomicAdd(qn_p, cnt);
+            base = __builtin_amdgcn_readlane(base, 0);
+            if (ok) qleaf[base + lane_rank(m)] = l;
         }
         __syncthreads();
         const int qn = *qn_p;
-        if (qn > kExactQueue - kBlock) {
-            scan_queue(qn);
-            __syncthreads();
+        if (qn > kExactQueue - 2 * kBlock) {
+            process_queue(qn);
             if (tid == 0) *qn_p = 0;
             __syncthreads();
         }
     }
     __syncthreads();
-    scan_queue(*qn_p);
+    process_queue(*qn_p);
     bset_reduce(S, K);
     if (tid < K) {
         od[(size_t)w * K + tid] = __uint_as_float((uint32_t)(S.key[tid] >> 32));
@@ -1906,7 +1984,7 @@ hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long l
     if (bounded) {
         const int nn = (1 << t.depth) - 1;
         const int words = 2 * ((nn + kWave - 1) / kWave) + 2;
-        const size_t lds = (size_t)(kSetWords + 2 * words + 3 * kExactQueue + 1) * sizeof(uint32_t);
+        const size_t lds = (size_t)(kSetWords + 2 * words + 4 * kExactQueue + kWave + 2) * sizeof(uint32_t);
         if (lds > 160 * 1024) return hipErrorInvalidValue;
         return with_dp(t.dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
@@ -1968,25 +2046,38 @@ hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hi
 //     its points closer than the natural list's K-th distance are exactly
 //     the natural list's entries, which are re-appended with that leaf's
 //     arrivals.
-// The only dependent memory round trip per row is the candidates' load:
-// leaf ranges are computed (leaf_range) and the next row's query, natural
-// leaf and natural list are requested while the current row is scored.
+// A row's unique leaves are cut into chunks of 64 points, listed in an LDS
+// table (point range, owner entry, offset in the leaf); the loads of UNR
+// chunks are issued together, so a row whose leaves fit UNR chunks (K = 5
+// with leaves of up to 128 points: 10) costs one dependent memory round
+// trip.  Leaf ranges come from the leaf descriptors (propleaf), and the next
+// row's query, natural leaf and natural list are requested while the current
+// row is scored.  Consecutive workgroups go to different XCDs, so the
+// columns are dealt out in bands: XCD i walks columns [i B, (i + 1) B), and
+// neighbouring columns, whose propagated leaves overlap, share an L2.
 constexpr int kSweepCap = 64 + 4 * kWave;   // K <= 64 live + four wavefronts of appends
+constexpr int kSweepTab = 256;              // chunk table entries (a window of a row's chunks)
+constexpr int kXcds = 8;
+constexpr int kRowBatch = 16;             // rows whose results are stored together
 
-template <int DP>
+template <int DP, int UNR>
 __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__restrict__ q, int n_rows, int n_cols,
                                                  int K, const int *__restrict__ propleaf,
                                                  const float *__restrict__ p1d, const int *__restrict__ p1p,
                                                  const int *__restrict__ p1l, const int *__restrict__ natleaf,
-                                                 int *__restrict__ fl_pos, const int *__restrict__ row0_l, int stride) {
-    constexpr int UNR = 4;
+                                                 int *__restrict__ fl_pos, const int *__restrict__ row0_l, int stride,
+                                                 int band, const float4 *__restrict__ head) {
     extern __shared__ uint32_t smem_w[];
     const int lane = threadIdx.x;
     const BlockSet S = bset_at(smem_w, kSweepCap);
-    const int x = blockIdx.x;
+    int4 *tab = reinterpret_cast<int4 *>(smem_w + set_words(kSweepCap));
+    int *rowbuf = reinterpret_cast<int *>(tab + kSweepTab);   // kRowBatch rows of K results
+    const int x = (int)(blockIdx.x % kXcds) * band + (int)(blockIdx.x / kXcds);
     if (x >= n_cols) return;
     const int mlo = t.n >> t.depth;
-    int my_l = (lane < K) ? row0_l[(size_t)x * K + lane] : 0;
+    const int chmax = (mlo + kWave) / kWave;   // chunks of the largest leaf (mlo + 1 points)
+    const bool act = lane < K;
+    int my_l = act ? row0_l[(size_t)x * K + lane] : 0;
     float qv[DP];
     float cd = 0.0f;
     int cp = 0, cl = 0, nat = 0;
@@ -1994,45 +2085,45 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
         const size_t qi = (size_t)n_cols + x;
         load_vec<DP>(q + qi * DP, qv);
         nat = natleaf[qi];
-        if (lane < K) { cd = p1d[qi * K + lane]; cp = p1p[qi * K + lane]; cl = p1l[qi * K + lane]; }
+        if (act) { cd = p1d[qi * K + lane]; cp = p1p[qi * K + lane]; cl = p1l[qi * K + lane]; }
     }
+    // rows [ybase, y) wait in rowbuf; their stores go out together
+    int ybase = 1;
+    auto flush_rows = [&](int y_end) {
+        wave_sync();
+        const int cnt = (y_end - ybase) * K;
+        for (int i = lane; i < cnt; i += kWave) {
+            const int r = i / K, k = i - r * K;
+            fl_pos[((size_t)(ybase + r) * n_cols + x) * K + k] = rowbuf[i];
+        }
+        ybase = y_end;
+    };
     for (int y = 1; y < n_rows; ++y) {
         const size_t qi = (size_t)y * n_cols + x;
-        // leaf k (lane k): occurrences (a mask over the K <= 64 lanes), first
-        // occurrence, natural or not
+        // occ: the entries (lanes) listing this lane's leaf; the first of
+        // them scans it, unless it is the natural leaf
         uint64_t occ = 0;
-        bool first = true;
         for (int j = 0; j < K; ++j) {
             const int lj = readlane_i(my_l, j);
-            if (lane < K && lj == my_l) {
-                occ |= 1ull << j;
-                if (j < lane) first = false;
-            }
+            const uint64_t b = __ballot(act && my_l == lj);
+            if (lane == j) occ = b;
         }
-        const bool is_nat = lane < K && my_l == nat;
-        const bool uniq = lane < K && first && !is_nat;
+        const bool is_nat = act && my_l == nat;
+        const bool uniq = act && (occ & ((1ull << lane) - 1ull)) == 0ull && !is_nat;
         const unsigned long long natmask = __ballot(is_nat);
         const int f = uniq ? (my_l >> 1) : 0;
         const int c = uniq ? (mlo + (my_l & 1)) : 0;
-        // chunks of 64 points per unique leaf, prefix over lanes
-        const int ch = uniq ? (c + kWave - 1) / kWave : 0;
-        int inc = ch;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int v = __shfl_up(inc, o, kWave);
-            if (lane >= o) inc += v;
+        const int ch = (c + kWave - 1) / kWave;
+        // exclusive prefix of ch over the lanes and its total, bit by bit
+        int chpre = 0, nch = 0;
+        for (int b = 0; (1 << b) <= chmax; ++b) {
+            const unsigned long long m = __ballot((ch >> b) & 1);
+            chpre += lane_rank(m) << b;
+            nch += __popcll(m) << b;
         }
-        const int chpre = inc - ch;
-        const int nch = readlane_i(inc, kWave - 1);
         const int nat_from = nat >> 1;
-        // owner (lane k of a unique leaf) of chunk j, held by lane j
-        int own_l = 0;
-        for (int k = 0; k < K; ++k) {
-            const int pk = readlane_i(chpre, k), ck = readlane_i(ch, k);
-            if (lane >= pk && lane < pk + ck) own_l = k;
-        }
         // the natural list as the first K items
-        if (lane < K) {
+        if (act) {
             S.key[lane] = make_key(cd, (uint32_t)(1 + lane));
             S.pos[lane] = cp;
             S.pl[lane] = cl;
@@ -2046,71 +2137,108 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
         const bool more = y + 1 < n_rows;
         bool fetched = false;
         bool touched = false;   // any append this row (the list may have changed)
-        for (int c0 = 0; c0 < nch; c0 += UNR) {
-            float pt[UNR][DP];
-            int pos[UNR], off[UNR], own[UNR], plc[UNR];
-            bool val[UNR];
+        for (int w0 = 0; w0 < nch; w0 += kSweepTab) {
+            const int wn = min(nch - w0, kSweepTab);
+            if (uniq)
+                for (int cc = max(chpre, w0); cc < min(chpre + ch, w0 + wn); ++cc) {
+                    const int o = (cc - chpre) * kWave;
+                    tab[cc - w0] = make_int4(f + o, min(kWave, c - o), lane, o);
+                }
+            wave_sync();
+            for (int c0 = 0; c0 < wn; c0 += UNR) {
+                // the chunks' table entries, then the head (dimensions 0-3) of
+                // every point, then the rest of the points whose partial sum is
+                // below the bound; every read is issued unconditionally (lanes
+                // without a point read point 0), so each group is in flight at once
+                constexpr int TL = DP - 4;
+                int4 e[UNR];
 #pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                const int cc = c0 + u;
-                // owner lane of chunk cc (uniform)
-                int o = readlane_i(own_l, cc & (kWave - 1));
-                if (cc >= kWave) {
-                    for (int k = 0; k < K; ++k) {
-                        const int pk = readlane_i(chpre, k), ck = readlane_i(ch, k);
-                        if (cc >= pk && cc < pk + ck) o = k;
+                for (int u = 0; u < UNR; ++u) e[u] = tab[min(c0 + u, kSweepTab - 1)];
+                float4 hd[UNR];
+                int pos[UNR], meta[UNR];
+                bool val[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    val[u] = c0 + u < wn && lane < e[u].y;
+                    pos[u] = val[u] ? e[u].x + lane : 0;
+                    meta[u] = (e[u].z << 24) | e[u].w;   // owner entry, offset of the chunk in its leaf
+                    hd[u] = head[pos[u]];
+                }
+                if (!fetched && more) {
+                    const size_t nqi = qi + n_cols;
+                    load_vec<DP>(q + nqi * DP, nqv);
+                    nnat = natleaf[nqi];
+                    const int kl = act ? lane : 0;
+                    nd0 = p1d[nqi * K + kl];
+                    np0 = p1p[nqi * K + kl];
+                    nl0 = p1l[nqi * K + kl];
+                    fetched = true;
+                }
+                float s2[UNR];
+                float tl[UNR][TL > 0 ? TL : 1];
+                int plc[UNR];
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    float dd, a = 0.0f;
+                    dd = hd[u].x - qv[0]; a = a + dd * dd;
+                    dd = hd[u].y - qv[1]; a = a + dd * dd;
+                    dd = hd[u].z - qv[2]; a = a + dd * dd;
+                    dd = hd[u].w - qv[3]; a = a + dd * dd;
+                    s2[u] = a;
+                    // the sum only grows: a partial sum at or above the bound rejects
+                    val[u] = val[u] && a < bound;
+                    const int tp = val[u] ? pos[u] : 0;
+                    if constexpr (TL > 0) {
+                        const float4 *src = reinterpret_cast<const float4 *>(t.pts + (size_t)tp * DP + 4);
+#pragma unroll
+                        for (int c4 = 0; c4 < TL / 4; ++c4) {
+                            const float4 v = src[c4];
+                            tl[u][4 * c4 + 0] = v.x;
+                            tl[u][4 * c4 + 1] = v.y;
+                            tl[u][4 * c4 + 2] = v.z;
+                            tl[u][4 * c4 + 3] = v.w;
+                        }
+                    }
+                    plc[u] = propleaf[tp];
+                }
+#pragma unroll
+                for (int u = 0; u < UNR; ++u) {
+                    float a = s2[u];
+                    if constexpr (TL > 0) {
+#pragma unroll
+                        for (int jj = 0; jj < TL; ++jj) {
+                            const float dd = tl[u][jj] - qv[4 + jj];
+                            a = a + dd * dd;
+                        }
+                    }
+                    const bool take = val[u] && a < bound;
+                    if (__ballot(take) == 0ull) continue;
+                    touched = true;
+                    const int mt = __builtin_amdgcn_readfirstlane(meta[u]);
+                    const uint32_t arr0 = (uint32_t)(1 + K + (mt & 0xFFFFFF) + lane);
+                    uint64_t m = readlane_u64(occ, mt >> 24);
+                    while (m) {
+                        const int k = __builtin_ctzll(m);
+                        m &= m - 1;
+                        wset_append(S, n, take, make_key(a, arr0 + (uint32_t)(k * stride)), pos[u], plc[u]);
+                        if (n > kSweepCap - kWave) {
+                            int dp_ = 0, dl_ = 0;
+                            wset_reduce(S, n, K, lane, dp_, dl_);
+                            bound = __uint_as_float((uint32_t)(S.key[K - 1] >> 32));
+                        }
                     }
                 }
-                own[u] = o;
-                const int fo = readlane_i(f, o), co = readlane_i(c, o), po = readlane_i(chpre, o);
-                off[u] = (cc - po) * kWave + lane;
-                val[u] = cc < nch && off[u] < co;
-                pos[u] = val[u] ? fo + off[u] : 0;
-                plc[u] = 0;
-                if (val[u]) {
-                    load_vec<DP>(t.pts + (size_t)pos[u] * DP, pt[u]);
-                    plc[u] = propleaf[pos[u]];
-                }
             }
-            if (!fetched && more) {
-                const size_t nqi = qi + n_cols;
-                load_vec<DP>(q + nqi * DP, nqv);
-                nnat = natleaf[nqi];
-                if (lane < K) { nd0 = p1d[nqi * K + lane]; np0 = p1p[nqi * K + lane]; nl0 = p1l[nqi * K + lane]; }
-                fetched = true;
-            }
-#pragma unroll
-            for (int u = 0; u < UNR; ++u) {
-                float s2 = 0.0f;
-                if (val[u]) {
-#pragma unroll
-                    for (int jj = 0; jj < DP; ++jj) {
-                        const float dd = pt[u][jj] - qv[jj];
-                        s2 = s2 + dd * dd;
-                    }
-                }
-                const bool take = val[u] && s2 < bound;
-                if (__ballot(take) == 0ull) continue;
-                touched = true;
-                const int pl = plc[u];
-                uint64_t m = readlane_u64(occ, own[u]);
-                while (m) {
-                    const int k = __builtin_ctzll(m);
-                    m &= m - 1;
-                    wset_append(S, n, take, make_key(s2, (uint32_t)(1 + K + k * stride + off[u])), pos[u], pl);
-                    if (n > kSweepCap - kWave) {
-                        int dp_ = 0, dl_ = 0;
-                        wset_reduce(S, n, K, lane, dp_, dl_);
-                        bound = __uint_as_float((uint32_t)(S.key[K - 1] >> 32));
-                    }
-                }
-            }
+            wave_sync();   // the next window rewrites the table
         }
         if (!fetched && more) {
             const size_t nqi = qi + n_cols;
             load_vec<DP>(q + nqi * DP, nqv);
             nnat = natleaf[nqi];
-            if (lane < K) { nd0 = p1d[nqi * K + lane]; np0 = p1p[nqi * K + lane]; nl0 = p1l[nqi * K + lane]; }
+            const int kl = act ? lane : 0;
+            nd0 = p1d[nqi * K + kl];
+            np0 = p1p[nqi * K + kl];
+            nl0 = p1l[nqi * K + kl];
         }
         // propagated leaves equal to the natural leaf: re-append the natural
         // list's entries below the bound with those leaves' arrivals
@@ -2119,7 +2247,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
             while (m) {
                 const int k = __builtin_ctzll(m);
                 m &= m - 1;
-                const bool take = lane < K && cd < bound;
+                const bool take = act && cd < bound;
                 if (__ballot(take) == 0ull) continue;
                 touched = true;
                 wset_append(S, n, take, make_key(cd, (uint32_t)(1 + K + k * stride + (cp - nat_from))), cp, cl);
@@ -2132,10 +2260,11 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
         }
         int rpos = cp, rl = cl;     // unchanged list when nothing was appended
         if (touched) wset_reduce(S, n, K, lane, rpos, rl);
-        if (lane < K) {
-            fl_pos[qi * K + lane] = rpos;
+        if (act) {
+            rowbuf[(y - ybase) * K + lane] = rpos;
             my_l = rl;
         }
+        if (y + 1 - ybase == kRowBatch || !more) flush_rows(y + 1);
         if (more) {
 #pragma unroll
             for (int j = 0; j < DP; ++j) qv[j] = nqv[j];
@@ -2148,16 +2277,36 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
     }
 }
 
+// head[i] = dimensions 0-3 of tree point i (16 bytes a point, contiguous)
+__global__ void k_point_heads(const float *__restrict__ pts, int n, int dpad, float4 *__restrict__ head) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = *reinterpret_cast<const float4 *>(pts + (size_t)i * dpad);
+}
+
 hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K, const int *propleaf,
                      const float *p1d, const int *p1p, const int *p1l, const int *natleaf, int *fl_pos,
-                     const int *row0_l, hipStream_t st) {
+                     const int *row0_l, float *head, hipStream_t st) {
     if (n_rows < 2) return hipSuccess;
-    const size_t lds = (size_t)set_words(kSweepCap) * sizeof(uint32_t);
+    // offsets of a leaf's chunks (below the leaf size) share a key field with the owner entry
+    if (t.max_leaf >= (1 << 24)) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_point_heads, dim3(blocks_for(t.n, 256)), dim3(256), 0, st, t.pts, t.n, t.dpad,
+                       reinterpret_cast<float4 *>(head));
+    const size_t lds = (size_t)set_words(kSweepCap) * sizeof(uint32_t) + kSweepTab * sizeof(int4) +
+                       (size_t)kRowBatch * kWave * sizeof(int);
+    const int band = (n_cols + kXcds - 1) / kXcds;
+    // chunks of the largest row: K leaves of up to max_leaf points
+    const long long chunks = (long long)K * ((t.max_leaf + kWave - 1) / kWave);
     return with_dp(t.dpad, [&](auto dp) {
         constexpr int DP = decltype(dp)::value;
-        hipLaunchKernelGGL((k_sweep<DP>), dim3(n_cols), dim3(kWave), lds, st, t, q, n_rows, n_cols, K, propleaf, p1d,
-                           p1p, p1l, natleaf, fl_pos, row0_l, t.max_leaf);
-        return hipGetLastError();
+        auto launch = [&](auto unr) {
+            constexpr int UNR = decltype(unr)::value;
+            hipLaunchKernelGGL((k_sweep<DP, UNR>), dim3((unsigned)(band * kXcds)), dim3(kWave), lds, st, t, q,
+                               n_rows, n_cols, K, propleaf, p1d, p1p, p1l, natleaf, fl_pos, row0_l, t.max_leaf, band,
+                               reinterpret_cast<const float4 *>(head));
+            return hipGetLastError();
+        };
+        if (DP <= 16 && chunks > 4) return launch(std::integral_constant<int, 10>{});
+        return launch(std::integral_constant<int, 4>{});
     });
 }
 

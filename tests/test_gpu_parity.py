@@ -139,6 +139,25 @@ def test_exact_search_matches_oracle(K, d, gpu_available):
     assert np.array_equal(bits(d), bits(rd))
 
 
+@pytest.mark.parametrize("K,d,shift", [(16, 6, 40.0), (32, 3, 8.0), (5, 16, 0.0)])
+def test_exact_search_far_queries_match_oracle(K, d, shift, gpu_available):
+    """Queries far outside clustered data: almost every leaf passes the
+    natural leaf's radius, so the queue of candidate leaves fills and is
+    bounded, sorted and scanned several times per query."""
+    rng = np.random.default_rng(70 + K)
+    centres = rng.standard_normal((12, d)).astype(np.float32) * 4
+    X = (centres[rng.integers(0, 12, 24000)] + rng.standard_normal((24000, d)).astype(np.float32) * 0.5)
+    X = X.astype(np.float32)
+    Q = (rng.standard_normal((300, d)) * 3 + shift).astype(np.float32)
+    depth = O.tree_depth(24000, 8)
+    ref = O.OracleTree(X, depth)
+    t = _fit(X, depth, K)
+    dd, idx = t.kneighbors(Q)
+    rd, ridx = ref.neighbors(Q, K)
+    assert np.array_equal(idx, ridx)
+    assert np.array_equal(bits(dd), bits(rd))
+
+
 def test_max_leaves_limits_the_search(gpu_available):
     from annfmp_amd import KDTree
     rng = np.random.default_rng(9)
