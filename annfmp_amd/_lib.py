@@ -27,6 +27,7 @@ EXPORTS = (
     "annf_exact_field_dev",
     "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
     "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev", "annf_mlp_triplet_grad_noalloc_dev",
+    "annf_pca_fit_dev",
 )
 
 PROJ_PCA32 = 0
@@ -91,6 +92,7 @@ def _declare(L):
     L.annf_set_encoder.argtypes = [_vp, _vp, i, i, i, i]
     L.annf_encode_image_dev.argtypes = [_vp, _vp, i, i, i, _vp]
     L.annf_pca_cov_dev.argtypes = [_vp, _vp, i, i, _vp, i, _vp, i, i, _vp, i, i, _vp, _vp]
+    L.annf_pca_fit_dev.argtypes = [_vp, _vp, i, i, _vp, i, _vp, i, i, _vp, i, i, i, _vp, _vp, _vp]
     for name in EXPORTS:
         if name not in ("annf_last_error", "annf_stream", "annf_norm_array"):
             getattr(L, name).restype = C.c_int

@@ -26,7 +26,11 @@ query for "bf16" / "fp8") and ``encoder_check`` (with an encoder: when > 0,
 the field is also computed with the PCA projection, recall@K of both fields is
 measured on that many sampled A patches against the exact field, and the
 better one is returned -- PCA is the fallback when the encoder's recall is
-lower).  Extra outputs after ``fit``: ``nn_distances`` (full-patch L2 of each
+lower), ``pca_fit`` ("sklearn": scikit-learn's ``PCA.fit`` as the
+reference, the default; "gpu": the same sampled patches, FP64 covariance and
+eigensolve on the GPU through ``annf_pca_fit_dev`` -- scikit-learn's
+covariance_eigh solution whatever solver sklearn would pick).  Extra outputs
+after ``fit``: ``nn_distances`` (full-patch L2 of each
 chosen match, float32), ``knn_indices`` (the K candidates of every patch before
 best-of-K, original indices), ``projection_used`` ("pca" or "encoder") and,
 with ``encoder_check``, ``encoder_check_result`` (both recalls).
@@ -47,7 +51,7 @@ class ANNFieldPropKDTreeHIP:
     def __init__(self, n_neighbors=3, psize=8, dim_reduced=5, n_subset=1000, leaf_size=64, n_jobs=-1,
                  propagate=True, select_best_nn=True, float_type="float", verbose=1, seed=0, device=0,
                  projection="fp32", leaf_distance="reduced", encoder=None, shortlist=20, nprobe=1,
-                 encoder_check=0):
+                 encoder_check=0, pca_fit="sklearn"):
         self.n_neighbors = n_neighbors
         self.psize = psize
         self.dim_reduced = dim_reduced
@@ -73,6 +77,9 @@ class ANNFieldPropKDTreeHIP:
         self.shortlist = shortlist
         self.nprobe = nprobe
         self.encoder_check = int(encoder_check)
+        if pca_fit not in ("sklearn", "gpu"):
+            raise Exception("pca_fit must be 'sklearn' or 'gpu', not {!r}".format(pca_fit))
+        self.pca_fit = pca_fit
         self._handle = None
 
     # ------------------------------------------------------------------
@@ -167,7 +174,17 @@ class ANNFieldPropKDTreeHIP:
         return patches.reshape((-1, self.psize * self.psize * n_channels))
 
     def _fit_pca_model(self, image_a, image_b):
-        """openmp/base.py:168-191 (scikit-learn, FP64)."""
+        """openmp/base.py:168-191 (scikit-learn, FP64); with pca_fit="gpu" the
+        same samples fitted by annf_pca_fit_dev."""
+        if self.pca_fit == "gpu":
+            from .pca import fit_pca_gpu, sklearn_sample_indices
+            n = int(self.n_subset / 2)
+            p = self.psize
+            ia = sklearn_sample_indices(image_a.shape[0], image_a.shape[1], p, n, self.seed)
+            ib = sklearn_sample_indices(image_b.shape[0], image_b.shape[1], p, n, self.seed + 1)
+            comp, mean, var = fit_pca_gpu(image_a, ia, p, self.dim_reduced, image_b, ib, device=self.device,
+                                          handle=self._get_handle(), return_variance=True)
+            return _GpuPCA(comp, mean, var)
         from sklearn.feature_extraction import image
         from sklearn.decomposition import PCA
         subset_a = image.extract_patches_2d(image_a, (self.psize, self.psize), max_patches=int(self.n_subset / 2),
@@ -195,6 +212,16 @@ class ANNFieldPropKDTreeHIP:
     @property
     def patches_b(self):
         return self._create_patches(self.image_b)
+
+
+class _GpuPCA:
+    """The attributes of a fitted sklearn PCA the backend reads."""
+
+    def __init__(self, components, mean, variance):
+        self.components_ = components
+        self.mean_ = mean
+        self.explained_variance_ = variance
+        self.n_components_ = components.shape[0]
 
 
 def _projection_args(projection, components, mean, psize, handle, device):

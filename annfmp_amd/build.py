@@ -31,6 +31,8 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+# rocSOLVER (FP64 eigensolve of the GPU PCA fit) and the rocBLAS handle it runs on
+LINK_LIBS = ["-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"]
 COMPILE_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
                  "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
@@ -68,7 +70,7 @@ def build(verbose=False, out=OUT, jobs=None):
             results = list(pool.map(compile_one, range(len(SOURCES))))
         for src, res in zip(SOURCES, results):
             _report(res, src, verbose)
-        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp"] + objs
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out + ".tmp"] + objs + LINK_LIBS
         if verbose:
             print(" ".join(cmd), flush=True)
         _report(subprocess.run(cmd, capture_output=True, text=True), "link", verbose)

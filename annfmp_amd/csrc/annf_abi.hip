@@ -3,6 +3,9 @@
 #include "annf_internal.h"
 #include "../../include/annf_hip.h"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -106,7 +109,9 @@ struct annf_ctx {
     DevBuf mlp_ws, enc_w1, enc_b1, enc_w2, enc_b2, enc_w3, enc_b3;
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
-    DevBuf pca_scratch;
+    DevBuf pca_scratch, pca_eig;
+    // rocSOLVER (the FP64 eigensolve of annf_pca_fit_dev), created on first use
+    rocblas_handle blas = nullptr;
     DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder, proj_pack;
     // exact BF16 leaf k-NN: centred BF16 images, per-patch norms
     DevBuf cimg_a, cimg_b, pnorm_a, pnorm_b, pnorm_tmp;
@@ -152,7 +157,7 @@ struct annf_ctx {
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &ptshead, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
-                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch,
+                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch, &pca_eig,
                           &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack, &cimg_a, &cimg_b,
                           &pnorm_a, &pnorm_b, &pnorm_tmp};
         for (DevBuf *b : list) b->release();
@@ -460,6 +465,7 @@ int annf_destroy(annf_ctx *c) {
     if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
     if (c->join_ev) (void)hipEventDestroy(c->join_ev);
     if (c->h_flag) (void)hipHostFree(c->h_flag);
+    if (c->blas) (void)rocblas_destroy_handle(c->blas);
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1126,6 +1132,41 @@ int annf_pca_cov_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const in
     RCHK(c->pca_scratch.ensure(pca_cov_scratch(D, n_a + n_b)));
     HIPCHK(pca_cov(ga, idx_a, n_a, gb, idx_b, n_b, D, c->pca_scratch.as<uint8_t>(), mean_out, cov_out, c->stream));
     return sync_stream(c);
+}
+
+int annf_pca_fit_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const int *idx_a, int n_a,
+                     const uint8_t *img_b, int hb, int wb, const int *idx_b, int n_b, int psize, int d,
+                     double *components_out, double *mean_out, double *variance_out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!components_out) return fail(ANNF_EINVAL, "null output");
+    const int D = psize * psize * 3;
+    if (d < 1 || d > D) return fail(ANNF_EINVAL, "n_components %d outside [1, %d]", d, D);
+    RCHK(set_device(c));
+    // covariance into the first D*D doubles of pca_eig, eigenvalues and the
+    // off-diagonal workspace after it, then rocSOLVER's status word
+    const size_t dd = (size_t)D * D;
+    RCHK(c->pca_eig.ensure((dd + 2 * (size_t)D) * sizeof(double) + 256));
+    double *C = c->pca_eig.as<double>(), *w = C + dd, *e = w + D;
+    rocblas_int *info = reinterpret_cast<rocblas_int *>(e + D);
+    RCHK(annf_pca_cov_dev(c, img_a, ha, wa, idx_a, n_a, img_b, hb, wb, idx_b, n_b, psize, mean_out, C));
+    if (!c->blas) {
+        if (rocblas_create_handle(&c->blas) != rocblas_status_success) {
+            c->blas = nullptr;
+            return fail(ANNF_EHIP, "rocblas_create_handle failed");
+        }
+    }
+    if (rocblas_set_stream(c->blas, c->stream) != rocblas_status_success)
+        return fail(ANNF_EHIP, "rocblas_set_stream failed");
+    // C is symmetric, so its row-major layout is also the column-major one
+    // rocSOLVER reads; on return its columns are the eigenvectors
+    const rocblas_status rs = rocsolver_dsyevd(c->blas, rocblas_evect_original, rocblas_fill_upper, D, C, D, w, e, info);
+    if (rs != rocblas_status_success) return fail(ANNF_EHIP, "rocsolver_dsyevd failed (status %d)", (int)rs);
+    HIPCHK(pca_components(C, w, D, d, components_out, variance_out, c->stream));
+    rocblas_int hinfo = 0;
+    HIPCHK(hipMemcpyAsync(&hinfo, info, sizeof hinfo, hipMemcpyDeviceToHost, c->stream));
+    RCHK(sync_stream(c));
+    if (hinfo != 0) return fail(ANNF_EHIP, "rocsolver_dsyevd did not converge (info %d)", (int)hinfo);
+    return ANNF_OK;
 }
 
 int annf_set_timing(annf_ctx *c, int on) {

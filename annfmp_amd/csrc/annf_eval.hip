@@ -318,3 +318,51 @@ hipError_t pca_cov(const PatchGeom &ga, const int *ia, int na, const PatchGeom &
 }
 
 }  // namespace annf
+
+namespace annf {
+
+// Components from the eigenvectors of the covariance (columns of V, D x D
+// column-major, eigenvalues w ascending, as LAPACK's and rocSOLVER's syevd
+// return them): component c is eigenvector D-1-c (descending eigenvalue,
+// scikit-learn's flip), multiplied by the sign of its first entry of largest
+// magnitude (svd_flip with u_based_decision=False, utils/extmath.py), and its
+// explained variance max(w, 0).  One block per component.
+__global__ __launch_bounds__(256) void k_pca_components(const double *__restrict__ V, const double *__restrict__ w,
+                                                        int D, int d, double *__restrict__ comp,
+                                                        double *__restrict__ var) {
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    const int c = blockIdx.x, col = D - 1 - c;
+    const double *v = V + (size_t)col * D;
+    double best = -1.0;
+    int bi = D;
+    for (int f = threadIdx.x; f < D; f += 256) {
+        const double a = fabs(v[f]);
+        if (a > best) { best = a; bi = f; }   // f ascending: keeps the first largest
+    }
+    sv[threadIdx.x] = best;
+    si[threadIdx.x] = bi;
+    __syncthreads();
+    for (int s = 128; s >= 1; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double ob = sv[threadIdx.x + s];
+            const int oi = si[threadIdx.x + s];
+            if (ob > sv[threadIdx.x] || (ob == sv[threadIdx.x] && oi < si[threadIdx.x])) {
+                sv[threadIdx.x] = ob;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const double pv = v[si[0]];
+    const double sgn = pv > 0.0 ? 1.0 : (pv < 0.0 ? -1.0 : 0.0);
+    for (int f = threadIdx.x; f < D; f += 256) comp[(size_t)c * D + f] = v[f] * sgn;
+    if (var && threadIdx.x == 0) var[c] = w[col] > 0.0 ? w[col] : 0.0;
+}
+
+hipError_t pca_components(const double *V, const double *w, int D, int d, double *comp, double *var, hipStream_t st) {
+    hipLaunchKernelGGL(k_pca_components, dim3(d), dim3(256), 0, st, V, w, D, d, comp, var);
+    return hipGetLastError();
+}
+
+}  // namespace annf

@@ -207,6 +207,9 @@ hipError_t exact_sample(const uint8_t *am, const uint32_t *an, const int *qidx, 
 size_t pca_cov_scratch(int D, int ns);
 hipError_t pca_cov(const PatchGeom &ga, const int *ia, int na, const PatchGeom &gb, const int *ib, int nb, int D,
                    uint8_t *scratch, double *mean, double *C, hipStream_t st);
+// d leading components (d x D row-major) and explained variances from the
+// eigenvectors V (columns, D x D column-major) and ascending eigenvalues w
+hipError_t pca_components(const double *V, const double *w, int D, int d, double *comp, double *var, hipStream_t st);
 
 int dpad_of(int d);
 bool dpad_supported(int dpad);

@@ -168,14 +168,22 @@ def train(encoder, triplets, epochs=2, batch=1024, lr=1e-3, margin=0.1, beta1=0.
     return encoder
 
 
-def fit_pca(a, p, d, n_samples=50000, seed=0):
-    """PCA basis (d x D) and mean from n_samples uniformly sampled A patches:
-    FP64 covariance and eigensolve, sklearn's sign rule (largest |loading|
-    of each component positive)."""
+def fit_pca(a, p, d, n_samples=50000, seed=0, where="gpu", handle=None, device=0):
+    """PCA basis (d x D) and mean from n_samples uniformly sampled A patches
+    (positions from default_rng(seed + 1000)): FP64 covariance and
+    eigensolve, sklearn's sign rule (largest |loading| of each component
+    positive).  where="gpu": the whole fit on the GPU (annf_pca_fit_dev);
+    "host": numpy on the host (for CPU-only scripts)."""
     rng = np.random.default_rng(seed + 1000)
     H, W, _ = a.shape
     ys = rng.integers(0, H - p + 1, n_samples)
     xs = rng.integers(0, W - p + 1, n_samples)
+    if where == "gpu":
+        from .pca import fit_pca_gpu
+        idx = (ys * (W - p + 1) + xs).astype(np.int32)
+        return fit_pca_gpu(a, idx, p, d, handle=handle, device=device)
+    if where != "host":
+        raise ValueError("where must be 'gpu' or 'host'")
     X = np.stack([a[y:y + p, x:x + p, :].reshape(-1) for y, x in zip(ys, xs)]).astype(np.float64)
     mu = X.mean(0)
     C = (X - mu).T @ (X - mu) / (n_samples - 1)
@@ -198,7 +206,7 @@ def compare_with_pca(encoder, seeds, spec="1080p-warp", leaf=64, K=5, nprobe=1, 
     out = []
     for s in seeds:
         a, b = synth.make_pair(spec, seed=s)
-        comp, mu = fit_pca(a, p, encoder.d, seed=s)
+        comp, mu = fit_pca(a, p, encoder.d, seed=s, handle=h)
         n_a = (a.shape[0] - p + 1) * (a.shape[1] - p + 1)
         n_b = (b.shape[0] - p + 1) * (b.shape[1] - p + 1)
         depth = tree_depth_for(n_b, leaf)

@@ -20,7 +20,8 @@ from device-resident uint8 images to the int32 field, for one of two paths:
 W untimed steps, then K timed steps bracketed by device synchronisation,
 for each path.  Quality (recall@5 and mean distance ratio against the
 exact field on 8192 sampled A patches) is measured outside the timed
-region.  The PCA basis is fitted once on 50,000 sampled A patches (FP64).
+region.  The PCA basis is fitted once on 50,000 sampled A patches (FP64
+covariance and eigensolve on the GPU).
 Prints one JSON line.
 """
 from __future__ import annotations
@@ -51,7 +52,8 @@ FP32_PEAK_TFS = 157.3          # MI355X FP32, 157.3 TFLOP/s
 
 
 def fit_basis(a, p, d, n_samples, seed):
-    """PCA basis from n_samples uniformly sampled A patches (FP64 covariance)."""
+    """PCA basis from n_samples uniformly sampled A patches: FP64 covariance
+    and eigensolve on the GPU (annf_pca_fit_dev)."""
     from annfmp_amd.encoder import fit_pca
     return fit_pca(a, p, d, n_samples, seed)
 

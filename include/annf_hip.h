@@ -235,12 +235,25 @@ int annf_encode_image_dev(annf_ctx *ctx, const uint8_t *img, int h, int w,
  * idx_a (n_a device int32 indices into img_a's patches) followed by idx_b
  * (n_b into img_b's), D = p*p*3: mean = sum / n; C = X^T X;
  * C -= (n * mean_col) * mean_row; C /= n - 1 -- sklearn's expression; the
- * sums are exact, so the result equals it bit for bit.  The eigensolve stays
- * with the caller (LAPACK eigh, as sklearn).  n_a + n_b <= 2^24. */
+ * sums are exact, so the result equals it bit for bit.  n_a + n_b <= 2^24. */
 int annf_pca_cov_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                      const int *idx_a, int n_a, const uint8_t *img_b, int hb,
                      int wb, const int *idx_b, int n_b, int psize,
                      double *mean_out, double *cov_out);
+/* The whole PCA fit on the GPU: the covariance above, its eigendecomposition
+ * in FP64 (rocSOLVER dsyevd, divide and conquer; scikit-learn calls LAPACK's
+ * eigh), the d leading eigenvectors in descending eigenvalue order with
+ * scikit-learn's sign rule (svd_flip, u_based_decision=False: the first
+ * entry of largest magnitude of each component positive).  Writes
+ * components_out (d x D doubles, row-major, PCA.components_), mean_out (D
+ * doubles, PCA.mean_) and, when not NULL, variance_out (d doubles,
+ * PCA.explained_variance_).  Equal to scikit-learn's fit up to the
+ * eigensolver's rounding (a few ulps of the leading components). */
+int annf_pca_fit_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
+                     const int *idx_a, int n_a, const uint8_t *img_b, int hb,
+                     int wb, const int *idx_b, int n_b, int psize, int d,
+                     double *components_out, double *mean_out,
+                     double *variance_out);
 
 /* ---- exact field (evaluation reference; no reference counterpart) -----
  * For the nq A patches listed in query_idx (device int32 patch indices),

@@ -33,7 +33,7 @@ def workload(name):
     else:
         a, b = synth.make_pair("1080p-warp", seed=0)
         p, d, leaf, K, ns = 11, 16, 64, 5, 50000
-    comp, mu = fit_pca(a, p, d, ns, seed=0)
+    comp, mu = fit_pca(a, p, d, ns, seed=0, where="host")
     pa, pb = O.extract_patches(a, p), O.extract_patches(b, p)
     ra, rb = O.project_f64(pa, comp, mu), O.project_f64(pb, comp, mu)
     nr, nc = a.shape[0] - p + 1, a.shape[1] - p + 1

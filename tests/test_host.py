@@ -134,3 +134,20 @@ def test_leaf_range_formula_matches_oracle_tree():
         t = O.OracleTree(rng.standard_normal((n, 2)).astype(np.float32), depth)
         for l in range(1 << depth):
             assert leaf_range(l, n, depth) == (t.leaves[2 * l], t.leaves[2 * l + 1])
+
+
+def test_sklearn_sample_indices_match_extract_patches_2d():
+    """The GPU PCA fit samples the positions extract_patches_2d(max_patches=n,
+    random_state=seed) samples (openmp/base.py:170-177)."""
+    import numpy as np
+    from sklearn.feature_extraction import image
+    from annfmp_amd.pca import sklearn_sample_indices
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (37, 53, 3), dtype=np.uint8)
+    p = 7
+    nc = 53 - p + 1
+    for n, seed in ((500, 0), (123, 9), (5000, 3)):
+        want = image.extract_patches_2d(img, (p, p), max_patches=n, random_state=seed).reshape(-1, p * p * 3)
+        idx = sklearn_sample_indices(37, 53, p, n, seed)
+        got = np.stack([img[i // nc:i // nc + p, i % nc:i % nc + p].reshape(-1) for i in idx])
+        assert np.array_equal(got, want)
