@@ -250,8 +250,10 @@ void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off
     }
 }
 
-// Builds the tree from n x dpad padded points already in c->pts_a.
-int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
+// Builds the tree from n x dpad padded points already in c->pts_a.  Without
+// need_pts the points are not written in tree order (the tree then has no
+// pts: the full-patch path reads only its nodes, order and leaves).
+int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth, bool need_pts = true) {
     std::vector<int> &bnd = c->h_bnd, &off = c->h_off, &cpn = c->h_cpn, &maxc = c->h_maxc;
     // the level bounds depend on (n, depth) alone: computed and uploaded once
     const long long bkey = ((long long)n << 6) | depth;
@@ -320,7 +322,7 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth) {
     TreeView t;
     HIPCHK(build_tree(n, d, dpad, depth, c->splitting_type, bnd.data(), off.data(), cpn.data(), maxc.data(), s,
                       c->axis.as<int>(), c->split.as<float>(), c->leaves.as<int>(), c->pli.as<int>(), t,
-                      c->stream));
+                      c->stream, need_pts));
     t.max_leaf = max_leaf;
     c->tree = t;
     c->has_tree = true;
@@ -522,6 +524,8 @@ int annf_neighbors(annf_ctx *c, const float *Xtest, int nXtest, int dXtest, floa
                    int ddistances, int *indices, int nindices, int dindices) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
     if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    if (!c->tree.pts)
+        return fail(ANNF_ENOTREE, "the tree of the last full-patch field holds no points: fit a tree first");
     RCHK(set_device(c));
     const TreeView &t = c->tree;
     const int K = ddistances;
@@ -555,6 +559,8 @@ int annf_process_rows(annf_ctx *c, int *finalindices, int nfinalindices, const u
                       int select_best_nn) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
     if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    if (!c->tree.pts)
+        return fail(ANNF_ENOTREE, "the tree of the last full-patch field holds no points: fit a tree first");
     RCHK(set_device(c));
     const TreeView &t = c->tree;
     const int K = n_neighbors;
@@ -632,6 +638,8 @@ int annf_tree_info(annf_ctx *c, int *n, int *d, int *depth) {
 int annf_tree_export(annf_ctx *c, int *axis, float *split, int *leaves, int *xi_orig, float *xi_pts, int *pli) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
     if (!c->has_tree) return fail(ANNF_ENOTREE, "no tree fitted");
+    if (xi_pts && !c->tree.pts)
+        return fail(ANNF_ENOTREE, "the tree of the last full-patch field holds no points: fit a tree first");
     RCHK(set_device(c));
     const TreeView &t = c->tree;
     const size_t nn = ((size_t)1 << t.depth) - 1;
@@ -756,7 +764,7 @@ struct FieldShape {
 int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b, int hb, int wb,
                       int psize, const void *basis, const double *mean, int d, int projection, int tree_depth,
                       int K, float *red_a_out, float *red_b_out, FieldShape &fs,
-                      const std::function<int(hipStream_t)> &side_extra = nullptr) {
+                      const std::function<int(hipStream_t)> &side_extra = nullptr, bool need_pts = true) {
     if (!c) return fail(ANNF_EINVAL, "null handle");
     if (!img_a || !img_b) return fail(ANNF_EINVAL, "null image pointer");
     RCHK(set_device(c));
@@ -836,7 +844,7 @@ int project_and_build(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         return rc;
     }
     c->mark(2);
-    rc = build_from_padded(c, (int)n_b, d, dpad, tree_depth);
+    rc = build_from_padded(c, (int)n_b, d, dpad, tree_depth, need_pts);
     if (hipStreamWaitEvent(c->stream, c->join_ev, 0) != hipSuccess && rc == ANNF_OK)
         rc = fail(ANNF_EHIP, "hipStreamWaitEvent failed");
     if (rc != ANNF_OK) {
@@ -918,7 +926,8 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     };
     FieldShape fs;
     RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, projection, tree_depth, K,
-                           red_a_out, red_b_out, fs, fp8 ? std::function<int(hipStream_t)>() : prepare_bf16));
+                           red_a_out, red_b_out, fs, fp8 ? std::function<int(hipStream_t)>() : prepare_bf16,
+                           false));
     const TreeView &t = c->tree;
     const int nleaves = 1 << t.depth;
     const int n_a = (int)fs.n_a;

@@ -95,7 +95,7 @@ size_t level_box_words(int nodes, int cpn, int dpad);
 hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
                       const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
-                      TreeView &t, hipStream_t st);
+                      TreeView &t, hipStream_t st, bool need_pts);
 
 // Natural leaf of every query (leaf), optional per-leaf counts, and optional
 // packed leaf ranges (desc: (from << 1) | (size - (n >> depth))).

@@ -806,6 +806,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
     __shared__ int hist_eq[kSubMaxAll];                 // radix histograms, then eqoff
     __shared__ uint32_t sbox[kSubNodes][DP][2];
     __shared__ int sb[kSubNodes + 1];                   // sub-node bounds (local offsets)
+    __shared__ uint8_t sid[SM];                         // sub-node of each local position
     __shared__ int sax[kSubNodes];
     __shared__ uint32_t spre[kSubNodes];
     __shared__ int srank[kSubNodes];
@@ -843,6 +844,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
             sbox[tid / DP][tid % DP][0] = 0xFFFFFFFFu;
             sbox[tid / DP][tid % DP][1] = 0u;
         }
+        for (int i = tid; i < c0; i += kSubTPB) sid[i] = (uint8_t)sub_of(sb, nsub, i);
         __syncthreads();
         // (A) data boxes: every (sub-node, dimension) pair gets T threads
         // (a power of two, T | 64) that scan its points; no atomics
@@ -892,14 +894,14 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         __syncthreads();
         // (C) split-axis keys
         for (int i = tid; i < c0; i += kSubTPB)
-            keys[i] = f2key(dataf[at(perm[cur][i], sax[sub_of(sb, nsub, i)])]);
+            keys[i] = f2key(dataf[at(perm[cur][i], sax[sid[i]])]);
         // (D) radix select of the rank-c/2 key of every sub-node
         for (int pass = 0; pass < 4; ++pass) {
             for (int i = tid; i < nsub * 256; i += kSubTPB) hist_eq[i] = 0;
             __syncthreads();
             const int dsh = 24 - 8 * pass;
             for (int i = tid; i < c0; i += kSubTPB) {
-                const int s2 = sub_of(sb, nsub, i);
+                const int s2 = sid[i];
                 const uint32_t k = keys[i];
                 if (pass == 0 || ((k ^ spre[s2]) >> (dsh + 8)) == 0u)
                     atomicAdd(&hist_eq[s2 * 256 + ((k >> dsh) & 255u)], 1);
@@ -947,7 +949,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
             const int i = EPT * tid + u;
             fl[u] = fe[u] = 0;
             if (i < c0) {
-                const uint32_t v = spre[sub_of(sb, nsub, i)];
+                const uint32_t v = spre[sid[i]];
                 fl[u] = keys[i] < v;
                 fe[u] = keys[i] == v;
             }
@@ -983,7 +985,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         for (int u = 0; u < EPT; ++u) {
             const int i = EPT * tid + u;
             if (i >= c0 || !fe[u]) continue;
-            const int s2 = sub_of(sb, nsub, i);
+            const int s2 = sid[i];
             const int c = sb[s2 + 1] - sb[s2], off = i - sb[s2];
             const int Ls = PL[s2 + 1] - PL[s2], Es = PE[s2 + 1] - PE[s2];
             const int rl = pl[u] - PL[s2], re = pe[u] - PE[s2];
@@ -998,7 +1000,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         for (int u = 0; u < EPT; ++u) {
             const int i = EPT * tid + u;
             if (i >= c0) continue;
-            const int s2 = sub_of(sb, nsub, i);
+            const int s2 = sid[i];
             const int c = sb[s2 + 1] - sb[s2], off = i - sb[s2];
             const int Ls = PL[s2 + 1] - PL[s2], Es = PE[s2 + 1] - PE[s2];
             const int rl = pl[u] - PL[s2], re = pe[u] - PE[s2];
@@ -1039,8 +1041,10 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         cur ^= 1;
         __syncthreads();
     }
-    float4 *dst = reinterpret_cast<float4 *>(pts_out + (size_t)lo * DP);
-    for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) {
+    // the points in final order (pts_out may be null: the full-patch path
+    // never reads them)
+    float4 *dst = pts_out ? reinterpret_cast<float4 *>(pts_out + (size_t)lo * DP) : nullptr;
+    for (int e = tid; dst && e < c0 * (DP / 4); e += kSubTPB) {
         const int i = e / (DP / 4), j = 4 * (e - i * (DP / 4));
         const int pi = perm[cur][i];
         dst[e] = make_float4(dataf[at(pi, j + 0)], dataf[at(pi, j + 1)], dataf[at(pi, j + 2)], dataf[at(pi, j + 3)]);
@@ -1054,7 +1058,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
         const int o = orig_in ? orig_in[src] : src;
         orig_out[lo + i] = o;
         if (pli) {
-            const int s2 = sub_of(sb, nlast, i);
+            const int s2 = sid[i];
             const int half = (sb[s2 + 1] - sb[s2]) / 2;
             pli[o] = (node << R) + 2 * s2 + (i - sb[s2] >= half ? 1 : 0);
         }
@@ -1085,7 +1089,7 @@ __global__ void k_leaves(const int *bnd, int nleaves, const int *orig, int n, in
 hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_bnd, const int *host_bnd_off, const int *host_cpn,
                       const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
-                      TreeView &t, hipStream_t st) {
+                      TreeView &t, hipStream_t st, bool need_pts) {
     (void)host_bnd;
     float *pin = s.pts_a, *pout = s.pts_b;
     int *oin = s.orig_a, *oout = s.orig_b;
@@ -1163,11 +1167,13 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
             constexpr int DP = decltype(dp)::value;
             hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin,
                                orig_identity ? (const int *)nullptr : oin, src_idx, s.bnd + host_bnd_off[L0], L0,
-                               depth - L0, d, splitting_type, pout, oout, axis, split, s.err, leaves, pli);
+                               depth - L0, d, splitting_type, need_pts ? pout : (float *)nullptr, oout, axis, split,
+                               s.err, leaves, pli);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
         std::swap(pin, pout);
+        if (!need_pts) pin = nullptr;   // no tree-order points written
         std::swap(oin, oout);
         orig_identity = false;
     } else {

@@ -173,6 +173,10 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
  * Outputs as annf_field_dev: nn_out is the first list entry, dist_out its
  * exact L2 (correctly rounded), knn_out the K list (-1 past the candidates).
  * No best-of-K pass: the list is already ranked by the full patch distance.
+ * The tree this call builds keeps no points in tree order (the full-patch
+ * path reads only its nodes, order and leaves): annf_neighbors,
+ * annf_process_rows and annf_tree_export of xi_pts on the handle return
+ * ANNF_ENOTREE until a tree is fitted again.
  * Stage times (annf_stage_times): 0 project B, 1 launch of A's projection,
  * 2 tree build (ends once A is projected too), 3 natural leaves +
  * bucketing, 4 MFMA leaf k-NN, 5 propagated pass + merge. */
