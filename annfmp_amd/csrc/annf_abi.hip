@@ -99,7 +99,7 @@ struct annf_ctx {
     // tree storage
     DevBuf pts_a, pts_b, orig_a, orig_b, axis, split, leaves, pli;
     // build scratch
-    DevBuf keys, eqoff, box, box2, nax, prefix, rank, hist, cnt, cnt_off, src_idx, tot, bnd, errflag;
+    DevBuf keys, eqoff, box, box2, nax, prefix, rank, hist, cnt, cnt_off, tot, bnd, errflag;
     // query workspaces
     DevBuf qbuf, natleaf, natdesc, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf, ptshead;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
@@ -153,7 +153,7 @@ struct annf_ctx {
 
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box, &box2,
-                          &nax, &prefix, &rank, &hist, &cnt, &cnt_off, &src_idx, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
+                          &nax, &prefix, &rank, &hist, &cnt, &cnt_off, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &ptshead, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
@@ -288,7 +288,6 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth, bool need_
     RCHK(c->hist.ensure(3 * hist_words * sizeof(uint32_t)));
     RCHK(c->cnt.ensure(chunks_max * sizeof(int2)));
     RCHK(c->cnt_off.ensure(chunks_max * sizeof(int2)));
-    RCHK(c->src_idx.ensure((size_t)n * sizeof(int)));
     RCHK(c->tot.ensure(nodes_max * sizeof(int2)));
     const void *bnd_was = c->bnd.p;
     RCHK(c->bnd.ensure(bnd.size() * sizeof(int)));
@@ -315,7 +314,6 @@ int build_from_padded(annf_ctx *c, int n, int d, int dpad, int depth, bool need_
     s.nodes_max = nodes_max;
     s.cnt = c->cnt.as<int2>();
     s.cnt_off = c->cnt_off.as<int2>();
-    s.src_idx = c->src_idx.as<int>();
     s.tot = c->tot.as<int2>();
     s.bnd = c->bnd.as<int>();
     s.err = c->errflag.as<int>();

@@ -49,7 +49,8 @@ struct TreeView {
 
 // Device scratch owned by the caller of the build.
 struct BuildScratch {
-    float *pts_a, *pts_b;             // n * dpad each (ping-pong)
+    float *pts_a;                     // n * dpad: the input rows, read in original order
+    float *pts_b;                     // n * dpad: the rows in tree order (written at the end)
     int *orig_a, *orig_b;             // n each
     uint32_t *keys;                   // n
     int *eqoff;                       // n
@@ -61,7 +62,6 @@ struct BuildScratch {
     size_t hist_words;                // max over the levels of level_hist_words(nodes, cpn)
     size_t nodes_max;
     int2 *cnt, *cnt_off;              // total chunks of the widest level each
-    int *src_idx;                     // n: record position of each slot (level before the subtree)
     int2 *tot;                        // nodes_max
     int *bnd;                         // all levels' bounds
     int *err;                         // 1 int, non-zero if a kernel saw an impossible state

@@ -6,7 +6,9 @@
 //     reduction), first longest axis, radix select (11/11/10-bit LDS
 //     histograms) of the rank-c/2 value, and an order-exact partition
 //     computed from block scans (the closed form of helper.c:63-113, see
-//     k_lvl_eqoff); the last levels run one subtree per block in LDS;
+//     k_lvl_eqoff); the rows never move: a level moves the map from tree
+//     positions to original rows and gathers rows through it; the last
+//     levels run one subtree per block in LDS;
 //   * the natural-leaf scan of every query is one pass over queries bucketed
 //     by leaf, the leaf's points staged in LDS;
 //   * row 0's exact search and the per-row propagation run one query per
@@ -365,9 +367,12 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_pick(LevelArgs L, const uint3
     }
 }
 
-// Data box of each node (kdtree.c:379-394), as min/max of ordered keys.
+// Data box of each node (kdtree.c:379-394), as min/max of ordered keys.  The
+// rows stay in their original order: tree position i is row orig[i] (i
+// itself while orig is null).
 template <int DP>
-__global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts, LevelArgs L, int d, uint32_t *box) {
+__global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts, const int *__restrict__ orig,
+                                                   LevelArgs L, int d, uint32_t *box) {
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
     if (s >= e) return;
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(kBuildTPB) void k_box(const float *__restrict__ pts
     for (int j = 0; j < DP; ++j) { mn[j] = 0xFFFFFFFFu; mx[j] = 0u; }
     for (int i = s + threadIdx.x; i < e; i += blockDim.x) {
         float v[DP];
-        load_vec<DP>(pts + (size_t)i * DP, v);
+        load_vec<DP>(pts + (size_t)(orig ? orig[i] : i) * DP, v);
 #pragma unroll
         for (int j = 0; j < DP; ++j) {
             uint32_t k = f2key(v[j]);
@@ -445,7 +450,8 @@ __device__ __forceinline__ int node_axis(const LevelArgs &L, int node, int d, in
 // third-pass histograms and resets the next level's boxes (nbox, filled by
 // k_move) when given.
 template <int DP>
-__global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict__ pts, LevelArgs L, int d, int dpad,
+__global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict__ pts, const int *__restrict__ orig,
+                                                         LevelArgs L, int d, int dpad,
                                                          const uint32_t *box, int splitting_type, int *nax,
                                                          int *axis_out, uint32_t *keys, uint32_t *H0, uint32_t *H1,
                                                          uint32_t *H2, size_t hwords, uint32_t *nbox) {
@@ -479,9 +485,22 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_hist0(const float *__restrict
     }
     __syncthreads();
     const int ax = sax;
-    for (int i = s + threadIdx.x; i < e; i += kBuildTPB) {
-        const uint32_t k = f2key(pts[(size_t)i * DP + ax]);
-        keys[i] = k;
+    // tree position i is row orig[i]: every row index of the thread is read
+    // before the gathers, every gather before the first key
+    int row[kBuildEPT];
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) {
+        const int i = s + (int)threadIdx.x + u * kBuildTPB;
+        row[u] = i < e ? (orig ? orig[i] : i) : -1;
+    }
+    float v[kBuildEPT];
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) v[u] = row[u] >= 0 ? pts[(size_t)row[u] * DP + ax] : 0.0f;
+#pragma unroll
+    for (int u = 0; u < kBuildEPT; ++u) {
+        if (row[u] < 0) continue;
+        const uint32_t k = f2key(v[u]);
+        keys[s + (int)threadIdx.x + u * kBuildTPB] = k;
         atomicAdd(&h[k >> pass_shift(0)], 1u);
     }
     __syncthreads();
@@ -621,25 +640,24 @@ __global__ __launch_bounds__(kBuildTPB) void k_lvl_eqoff(LevelArgs L, const uint
     }
 }
 
-// RECORDS = false (the level before k_subtree): only orig moves, and
-// src_out[slot] records the position of the slot's record, which the subtree
-// blocks gather directly.
-// The moving pass of the partition (the slots of k_lvl_eqoff), coalesced: the
-// block first computes every point's slot (eight consecutive points per
-// thread, so the segment-order ranks come from one block scan), then copies
-// the records one float4 per lane.  With nbox, the same copy accumulates the two
-// children's data boxes (the next level's k_box), for DP/4 a power of two.
-template <int DP, bool RECORDS>
-__global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pts_in, const int *__restrict__ orig_in,
+// The moving pass of the partition (the slots of k_lvl_eqoff).  The rows never
+// move: every level reads them through orig (tree position -> original
+// row), so this pass moves orig alone.  The block first computes every
+// point's slot (eight consecutive points per thread, so the segment-order
+// ranks come from one block scan) and writes orig in the new order; with
+// nbox (DP/4 a power of two) it then gathers the rows, one float4 per lane,
+// and accumulates the two children's data boxes (the next level's k_box).
+template <int DP>
+__global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pts, const int *__restrict__ orig_in,
                                                     LevelArgs L, const uint32_t *keys, const uint32_t *vkey,
                                                     const int2 *cnt_off, const int2 *tot, const int *eqoff,
-                                                    float *__restrict__ pts_out, int *__restrict__ orig_out,
-                                                    const int *nax, float *split, int *err, uint32_t *nbox, int d,
-                                                    int *__restrict__ src_out) {
+                                                    int *__restrict__ orig_out, const int *nax, float *split,
+                                                    int *err, uint32_t *nbox, int d) {
     constexpr int P4 = DP / 4;
     constexpr bool kBoxOk = (P4 & (P4 - 1)) == 0;
     __shared__ int2 wsum[kBuildTPB / kWave];
     __shared__ int slots[kChunk];
+    __shared__ int srow[kChunk];
     __shared__ uint32_t sbox[2][DP][2];
     int node, lo, hi, s, e;
     chunk_range(L, node, lo, hi, s, e);
@@ -690,30 +708,37 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
                 atomicOr(err, 1);
                 slot = -1;
             } else {
-                orig_out[lo + slot] = orig_in ? orig_in[i] : i;
-                if (!RECORDS) src_out[lo + slot] = i;
-                if (slot == half) split[L.node0 + node] = pts_in[(size_t)i * DP + ax];
+                const int row = orig_in ? orig_in[i] : i;
+                orig_out[lo + slot] = row;
+                srow[i - s] = row;
+                if (slot == half) split[L.node0 + node] = pts[(size_t)row * DP + ax];
             }
         }
         slots[i - s] = slot;
     }
-    if (!RECORDS) return;   // the records stay; src_out says where each slot's record is
+    if (!boxes) return;
     __syncthreads();
-    const float4 *src4 = reinterpret_cast<const float4 *>(pts_in) + (size_t)s * P4;
-    float4 *dst4 = reinterpret_cast<float4 *>(pts_out) + (size_t)lo * P4;
+    const float4 *src4 = reinterpret_cast<const float4 *>(pts);
     uint32_t mn0[4], mx0[4], mn1[4], mx1[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { mn0[q] = mn1[q] = 0xFFFFFFFFu; mx0[q] = mx1[q] = 0u; }
     const int nf = (e - s) * P4;
-    for (int f = threadIdx.x; f < nf; f += kBuildTPB) {
-        const int el = f / P4, part = f - el * P4;
-        const int slot = slots[el];
-        if (slot < 0) continue;
-        const float4 v = src4[f];
-        dst4[(size_t)slot * P4 + part] = v;
-        if (boxes) {
-            const uint32_t k0 = f2key(v.x), k1 = f2key(v.y), k2 = f2key(v.z), k3 = f2key(v.w);
-            if (slot < half) {
+    constexpr int kUn = 8;   // gathers in flight per thread
+    for (int f0 = threadIdx.x; f0 < nf; f0 += kUn * kBuildTPB) {
+        int sl[kUn];
+        float4 v[kUn];
+#pragma unroll
+        for (int u = 0; u < kUn; ++u) {
+            const int f = f0 + u * kBuildTPB;
+            const int el = f / P4, part = f - el * P4;
+            sl[u] = f < nf ? slots[el] : -1;
+            if (sl[u] >= 0) v[u] = src4[(size_t)srow[el] * P4 + part];
+        }
+#pragma unroll
+        for (int u = 0; u < kUn; ++u) {
+            if (sl[u] < 0) continue;
+            const uint32_t k0 = f2key(v[u].x), k1 = f2key(v[u].y), k2 = f2key(v[u].z), k3 = f2key(v[u].w);
+            if (sl[u] < half) {
                 mn0[0] = min(mn0[0], k0); mx0[0] = max(mx0[0], k0);
                 mn0[1] = min(mn0[1], k1); mx0[1] = max(mx0[1], k1);
                 mn0[2] = min(mn0[2], k2); mx0[2] = max(mx0[2], k2);
@@ -726,8 +751,8 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
             }
         }
     }
-    if (!boxes) return;
     // lanes with the same part (lane % P4) hold the same four dimensions
+    // (kUn * kBuildTPB is a multiple of P4, so a lane's part never changes)
     const int lane = threadIdx.x & (kWave - 1);
 #pragma unroll
     for (int m = P4; m < kWave; m <<= 1) {
@@ -756,6 +781,17 @@ __global__ __launch_bounds__(kBuildTPB) void k_move(const float *__restrict__ pt
         atomicMin(&nbox[o], sbox[ch][j][0]);
         atomicMax(&nbox[o + 1], sbox[ch][j][1]);   // copy blockIdx.x % rbox_next
     }
+}
+
+// Tree-order rows when no subtree pass writes them: out[i] = pts[orig[i]].
+template <int DP>
+__global__ void k_gather_rows(const float *__restrict__ pts, const int *__restrict__ orig, int n,
+                              float *__restrict__ out) {
+    const long long f = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= (long long)n * (DP / 4)) return;
+    const long long i = f / (DP / 4);
+    const int part = (int)(f - i * (DP / 4));
+    reinterpret_cast<float4 *>(out)[f] = reinterpret_cast<const float4 *>(pts)[(size_t)orig[i] * (DP / 4) + part];
 }
 
 // ---------------------------------------------------------------------------
@@ -818,7 +854,7 @@ __global__ __launch_bounds__(kSubTPB) void k_subtree(const float *__restrict__ p
     const int node = blockIdx.x;
     const int lo = bnd0[node], c0 = bnd0[node + 1] - lo;
     {
-        // point i of the node is record src_idx[lo + i] (or lo + i)
+        // point i of the node is row src_idx[lo + i] (or lo + i)
         for (int e = tid; e < c0 * (DP / 4); e += kSubTPB) {
             const int i = e / (DP / 4), j = 4 * (e - i * (DP / 4));
             const size_t g = src_idx ? (size_t)src_idx[lo + i] : (size_t)(lo + i);
@@ -1091,7 +1127,11 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                       const int *host_maxc, BuildScratch &s, int *axis, float *split, int *leaves, int *pli,
                       TreeView &t, hipStream_t st, bool need_pts) {
     (void)host_bnd;
-    float *pin = s.pts_a, *pout = s.pts_b;
+    // the n x dpad rows stay in s.pts_a in their original order; the levels
+    // move orig (tree position -> row) and read the rows through it; the
+    // tree-order rows are written once at the end, into s.pts_b
+    const float *rows = s.pts_a;
+    float *rows_out = s.pts_b;
     int *oin = s.orig_a, *oout = s.orig_b;
     hipError_t err = hipSuccess;
     // first level whose nodes all fit one subtree block, at most kSubLevels
@@ -1102,7 +1142,6 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
     // the moving pass computes the next level's boxes when DP / 4 is a power
     // of two; boxes alternate between box and box2
     const bool fused_box = splitting_type != 0 && (dpad == 4 || dpad == 8 || dpad == 16 || dpad == 32 || dpad == 64);
-    const int *src_idx = nullptr;   // set when the last level moved indices only
     bool orig_identity = true;      // orig is still 0..n-1 (read as the index itself)
     for (int lvl = 0; lvl < L0; ++lvl) {
         LevelArgs L;
@@ -1129,15 +1168,14 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
                                need_kbox ? box : (uint32_t *)nullptr, lvl == 0 ? H0 : (uint32_t *)nullptr, hw);
         }
         const int R = hist_rep(L.cpn);
-        // the last level before the subtree blocks moves indices, not records
-        const bool index_only = lvl + 1 == L0 && L0 < depth;
+        const int *oin_k = orig_identity ? (const int *)nullptr : oin;
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
             if (need_kbox)
-                hipLaunchKernelGGL(k_box<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, pin, L, d, box);
+                hipLaunchKernelGGL(k_box<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, rows, oin_k, L, d, box);
             const dim3 g(nblk), b(kBuildTPB), gn((unsigned)L.nodes);
-            hipLaunchKernelGGL(k_lvl_hist0<DP>, g, b, 0, st, pin, L, d, dpad, box, splitting_type, s.nax, axis,
-                               s.keys, H0, H1, H2, hw, nbox);
+            hipLaunchKernelGGL(k_lvl_hist0<DP>, g, b, 0, st, rows, oin_k, L, d, dpad, box, splitting_type, s.nax,
+                               axis, s.keys, H0, H1, H2, hw, nbox);
             hipLaunchKernelGGL(k_lvl_pick<0>, gn, b, 0, st, L, H0, R, (const uint32_t *)nullptr,
                                (const int *)nullptr, P1, R1);
             hipLaunchKernelGGL(k_lvl_pass<1>, g, b, 0, st, L, s.keys, (const uint32_t *)nullptr, H1, P1,
@@ -1146,43 +1184,47 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
             hipLaunchKernelGGL(k_lvl_count, g, b, 0, st, L, s.keys, H2, P2, R2, vkey, s.cnt);
             hipLaunchKernelGGL(k_lvl_eqoff<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, L, s.keys, vkey, s.cnt,
                                s.cnt_off, s.tot, s.eqoff, H0, hw_next, s.err);
-            const int *oin_k = orig_identity ? (const int *)nullptr : oin;
-            if (index_only)
-                hipLaunchKernelGGL((k_move<DP, false>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin_k, L, s.keys, vkey,
-                                   s.cnt_off, s.tot, s.eqoff, pout, oout, s.nax, split, s.err, (uint32_t *)nullptr, d,
-                                   s.src_idx);
-            else
-                hipLaunchKernelGGL((k_move<DP, true>), dim3(nblk), dim3(kBuildTPB), 0, st, pin, oin_k, L, s.keys, vkey,
-                                   s.cnt_off, s.tot, s.eqoff, pout, oout, s.nax, split, s.err, nbox, d, (int *)nullptr);
+            hipLaunchKernelGGL(k_move<DP>, dim3(nblk), dim3(kBuildTPB), 0, st, rows, oin_k, L, s.keys, vkey,
+                               s.cnt_off, s.tot, s.eqoff, oout, s.nax, split, s.err, nbox, d);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
-        if (!index_only) std::swap(pin, pout);
         std::swap(oin, oout);
         orig_identity = false;
-        if (index_only) src_idx = s.src_idx;
     }
+    const float *tree_rows = nullptr;
     if (L0 < depth) {
+        // the subtree blocks gather their rows through orig and write them in
+        // final order (when asked: the full-patch path never reads them)
+        const int *oin_k = orig_identity ? (const int *)nullptr : oin;
         err = with_dp(dpad, [&](auto dp) {
             constexpr int DP = decltype(dp)::value;
-            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, pin,
-                               orig_identity ? (const int *)nullptr : oin, src_idx, s.bnd + host_bnd_off[L0], L0,
-                               depth - L0, d, splitting_type, need_pts ? pout : (float *)nullptr, oout, axis, split,
-                               s.err, leaves, pli);
+            hipLaunchKernelGGL(k_subtree<DP>, dim3(1u << L0), dim3(kSubTPB), 0, st, rows, oin_k, oin_k,
+                               s.bnd + host_bnd_off[L0], L0, depth - L0, d, splitting_type,
+                               need_pts ? rows_out : (float *)nullptr, oout, axis, split, s.err, leaves, pli);
             return hipGetLastError();
         });
         if (err != hipSuccess) return err;
-        std::swap(pin, pout);
-        if (!need_pts) pin = nullptr;   // no tree-order points written
         std::swap(oin, oout);
         orig_identity = false;
+        if (need_pts) tree_rows = rows_out;
     } else {
         // no subtree pass (or no level at all): leaf bounds and point -> leaf map here
         if (orig_identity) {
             err = iota(oin, n, st);
             if (err != hipSuccess) return err;
-            orig_identity = false;
+            tree_rows = rows;   // no level moved anything: tree order is the original order
+        } else if (need_pts) {
+            err = with_dp(dpad, [&](auto dp) {
+                constexpr int DP = decltype(dp)::value;
+                hipLaunchKernelGGL(k_gather_rows<DP>, dim3(blocks_for((long long)n * (DP / 4), 256)), dim3(256), 0,
+                                   st, rows, (const int *)oin, n, rows_out);
+                return hipGetLastError();
+            });
+            if (err != hipSuccess) return err;
+            tree_rows = rows_out;
         }
+        orig_identity = false;
         const int nleaves = 1 << depth;
         hipLaunchKernelGGL(k_leaves, dim3(blocks_for(std::max(n, nleaves), 256)), dim3(256), 0, st,
                            s.bnd + host_bnd_off[depth], nleaves, oin, n, leaves, pli);
@@ -1192,7 +1234,7 @@ hipError_t build_tree(int n, int d, int dpad, int depth, int splitting_type,
     t.d = d;
     t.dpad = dpad;
     t.depth = depth;
-    t.pts = pin;
+    t.pts = tree_rows;
     t.orig = oin;
     t.axis = axis;
     t.split = split;
