@@ -29,8 +29,11 @@ better one is returned -- PCA is the fallback when the encoder's recall is
 lower), ``pca_fit`` ("sklearn": scikit-learn's ``PCA.fit`` as the
 reference, the default; "gpu": the same sampled patches, FP64 covariance and
 eigensolve on the GPU through ``annf_pca_fit_dev`` -- scikit-learn's
-covariance_eigh solution whatever solver sklearn would pick).  Extra outputs
-after ``fit``: ``nn_distances`` (full-patch L2 of each
+covariance_eigh solution whatever solver sklearn would pick) and
+``semantics`` ("openmp": the openmp backend's tree and rows, the default;
+"openclopt": the openclopt backend's, annfmp/openclopt/base.py:47-130 --
+padded power-of-two tree, K fixed to 8, propagation and best-of-K always on,
+see ``annfmp_amd.openclopt``).  Extra outputs after ``fit``: ``nn_distances`` (full-patch L2 of each
 chosen match, float32), ``knn_indices`` (the K candidates of every patch before
 best-of-K, original indices), ``projection_used`` ("pca" or "encoder") and,
 with ``encoder_check``, ``encoder_check_result`` (both recalls).
@@ -51,7 +54,7 @@ class ANNFieldPropKDTreeHIP:
     def __init__(self, n_neighbors=3, psize=8, dim_reduced=5, n_subset=1000, leaf_size=64, n_jobs=-1,
                  propagate=True, select_best_nn=True, float_type="float", verbose=1, seed=0, device=0,
                  projection="fp32", leaf_distance="reduced", encoder=None, shortlist=20, nprobe=1,
-                 encoder_check=0, pca_fit="sklearn"):
+                 encoder_check=0, pca_fit="sklearn", semantics="openmp"):
         self.n_neighbors = n_neighbors
         self.psize = psize
         self.dim_reduced = dim_reduced
@@ -80,6 +83,9 @@ class ANNFieldPropKDTreeHIP:
         if pca_fit not in ("sklearn", "gpu"):
             raise Exception("pca_fit must be 'sklearn' or 'gpu', not {!r}".format(pca_fit))
         self.pca_fit = pca_fit
+        if semantics not in ("openmp", "openclopt"):
+            raise Exception("semantics must be 'openmp' or 'openclopt', not {!r}".format(semantics))
+        self.semantics = semantics
         self._handle = None
 
     # ------------------------------------------------------------------
@@ -97,6 +103,9 @@ class ANNFieldPropKDTreeHIP:
         need_pca = self.encoder is None or check
         self._pca_model = self._fit_pca_model(self.image_a, self.image_b) if need_pca else None
         self._timers["pca_fit"] = time.time() - tstart
+
+        if self.semantics == "openclopt":
+            return self._fit_openclopt(image_a, image_b)
 
         # since there is just one leaf visit, this must hold (openmp/base.py:113)
         assert self.leaf_size > self.n_neighbors
@@ -151,6 +160,32 @@ class ANNFieldPropKDTreeHIP:
                                                       "kdtree_fit", "remaining_rows"))))
             print("-----------------------------------------")
         return nn_indices
+
+    def _fit_openclopt(self, image_a, image_b):
+        """openclopt/base.py:47-130 after the PCA fit: the reference asserts
+        select_best_nn and fixes K = 8 whatever n_neighbors says."""
+        from .openclopt import field_openclopt
+        assert self.select_best_nn
+        if self.encoder is not None:
+            raise Exception("semantics='openclopt' projects with PCA (the reference's reducePatchDim)")
+        tstart = time.time()
+        res = field_openclopt(image_a, image_b, self.psize, self._pca_model.components_, self._pca_model.mean_,
+                              self.leaf_size, projection=self.projection, handle=self._get_handle(),
+                              device=self.device)
+        self._timers["futhark"] = time.time() - tstart
+        self.projection_used = "pca"
+        self.encoder_check_result = None
+        self.knn_indices = res["knn"]
+        self.nn_distances = None
+        if self.verbose > 0:
+            print("-----------------------------------------")
+            print("--------------- RUNTIMES ----------------")
+            print("-----------------------------------------")
+            print("PCA fit:\t\t{}".format(self._timers["pca_fit"]))
+            print("Futhark:\t\t{}".format(self._timers["futhark"]))
+            print("Total runtime:\t\t{}".format(self._timers["pca_fit"] + self._timers["futhark"]))
+            print("-----------------------------------------")
+        return res["nn"]
 
     def _field(self, image_a, image_b, comp, mean, depth, proj):
         if self.leaf_distance == "reduced":

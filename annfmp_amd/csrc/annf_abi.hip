@@ -110,6 +110,11 @@ struct annf_ctx {
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
     DevBuf mm_keys1, mm_keys2, mm_pairleaf;
     DevBuf pca_scratch, pca_eig;
+    // the openclopt-semantics mode: its tree and row lists
+    DevBuf ocl_lub, ocl_indir, ocl_indir2, ocl_keys, ocl_keys2, ocl_dims, ocl_clanc, ocl_vals, ocl_leafpts,
+        ocl_o2l, ocl_sort, ocl_nat, ocl_pos, ocl_dst;
+    OclTree ocl;
+    bool has_ocl = false;
     // rocSOLVER (the FP64 eigensolve of annf_pca_fit_dev), created on first use
     rocblas_handle blas = nullptr;
     DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder, proj_pack;
@@ -157,13 +162,16 @@ struct annf_ctx {
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &ptshead, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
-                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch, &pca_eig,
+                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch, &pca_eig, &ocl_lub, &ocl_indir, &ocl_indir2, &ocl_keys,
+                          &ocl_keys2, &ocl_dims, &ocl_clanc, &ocl_vals, &ocl_leafpts, &ocl_o2l, &ocl_sort, &ocl_nat,
+                          &ocl_pos, &ocl_dst,
                           &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack, &cimg_a, &cimg_b,
                           &pnorm_a, &pnorm_b, &pnorm_tmp};
         for (DevBuf *b : list) b->release();
         bnd_key = -1;
         has_tree = false;
         has_encoder = false;
+        has_ocl = false;
     }
 };
 
@@ -1174,6 +1182,135 @@ int annf_pca_fit_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const in
     RCHK(sync_stream(c));
     if (hinfo != 0) return fail(ANNF_EHIP, "rocsolver_dsyevd did not converge (info %d)", (int)hinfo);
     return ANNF_OK;
+}
+
+int annf_project_dev(annf_ctx *c, const uint8_t *img, int h, int w, int psize, const void *basis,
+                     const double *mean, int d, int projection, float *out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!img || !out) return fail(ANNF_EINVAL, "null pointer");
+    const int p = psize;
+    if (p < 1 || h < p || w < p) return fail(ANNF_EINVAL, "patch size %d larger than the image", p);
+    if (d < 1 || d > ANNF_MAX_D) return fail(ANNF_EUNSUPPORTED, "dimension %d outside [1, %d]", d, ANNF_MAX_D);
+    if (projection != ANNF_PROJ_PCA32 && projection != ANNF_PROJ_PCA64 && projection != ANNF_PROJ_ENCODER)
+        return fail(ANNF_EINVAL, "projection %d is not ANNF_PROJ_PCA32, _PCA64 or _ENCODER", projection);
+    const bool enc = projection == ANNF_PROJ_ENCODER;
+    if (!enc && (!basis || !mean)) return fail(ANNF_EINVAL, "null basis or mean");
+    if (enc && (!c->has_encoder || c->enc_p != p || c->enc_d != d))
+        return fail(ANNF_EINVAL, "no encoder for patch size %d and dimension %d on this handle", p, d);
+    RCHK(set_device(c));
+    const int D = p * p * 3, dpad = dpad_of(d), nc = w - p + 1;
+    const long long n = (long long)(h - p + 1) * nc;
+    const PatchGeom g{img, 1, nc, w, p, 3 * p, 3LL * w, 3LL * w * h};
+    RCHK(c->stage_in.ensure((size_t)n * dpad * sizeof(float)));
+    float *padded = c->stage_in.as<float>();
+    if (enc) {
+        HIPCHK(encode_patches(g, n, c->enc_w1.as<__bf16>(), c->enc_b1.as<float>(), c->enc_w2.as<__bf16>(),
+                              c->enc_b2.as<float>(), c->enc_w3.as<__bf16>(), c->enc_b3.as<float>(), d, dpad, padded,
+                              c->stream, 2));
+    } else {
+        RCHK(c->basis32.ensure((size_t)D * dpad * sizeof(float)));
+        RCHK(c->mean32.ensure((size_t)D * sizeof(float)));
+        RCHK(c->basis64.ensure((size_t)D * dpad * sizeof(double)));
+        RCHK(c->mean64.ensure((size_t)D * sizeof(double)));
+        hipLaunchKernelGGL(k_prep_basis, dim3((D * dpad + 255) / 256), dim3(256), 0, c->stream, basis,
+                           (int)(projection == ANNF_PROJ_PCA64), mean, d, D, dpad, c->basis32.as<float>(),
+                           c->mean32.as<float>(), c->basis64.as<double>(), c->mean64.as<double>());
+        HIPCHK(hipGetLastError());
+        if (projection == ANNF_PROJ_PCA32) {
+            RCHK(c->proj_pack.ensure(projection_pack_floats(p, dpad) * sizeof(float)));
+            HIPCHK(pack_projection(c->basis32.as<float>(), c->mean32.as<float>(), p, dpad, c->proj_pack.as<float>(),
+                                   c->stream));
+        }
+        HIPCHK(project_patches(img, h, w, p, c->basis32.as<float>(), c->mean32.as<float>(), c->basis64.as<double>(),
+                               c->mean64.as<double>(), d, dpad, projection == ANNF_PROJ_PCA64, padded,
+                               c->proj_pack.as<float>(), 2, c->stream));
+    }
+    hipLaunchKernelGGL(k_unpad, dim3((unsigned)((n * d + 255) / 256)), dim3(256), 0, c->stream, padded, n, d, dpad,
+                       out);
+    HIPCHK(hipGetLastError());
+    return sync_stream(c);
+}
+
+int annf_field_openclopt_dev(annf_ctx *c, const float *red_a, int n_rows, int n_cols, const float *red_b, int n_b,
+                             int d, int leaf_size, const uint8_t *img_a, int ha, int wa, const uint8_t *img_b,
+                             int hb, int wb, int psize, int *nn_out, int *knn_out, float *knd_out) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!red_a || !red_b) return fail(ANNF_EINVAL, "null reduced patches");
+    if (n_rows < 1 || n_cols < 1 || n_b < 1) return fail(ANNF_EINVAL, "empty patch set");
+    if ((long long)n_rows * n_cols > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "too many patches");
+    if (d < 1 || d > ANNF_MAX_D) return fail(ANNF_EUNSUPPORTED, "dimension %d outside [1, %d]", d, ANNF_MAX_D);
+    if (leaf_size < 1) return fail(ANNF_EINVAL, "leaf_size must be positive");
+    const int n_a = n_rows * n_cols;
+    // the row chain reads B indices on A's patch grid (driverKNN.fut:250-261)
+    if (n_rows > 1 && n_b != n_a)
+        return fail(ANNF_EUNSUPPORTED, "propagation needs B on A's patch grid (%d vs %d patches)", n_b, n_a);
+    if (nn_out) {
+        const int p = psize;
+        if (!img_a || !img_b) return fail(ANNF_EINVAL, "the best of the 8 candidates needs both images");
+        if (p < 1 || ha < p || wa < p || hb < p || wb < p) return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
+        if ((ha - p + 1) != n_rows || (wa - p + 1) != n_cols || (long long)(hb - p + 1) * (wb - p + 1) != n_b)
+            return fail(ANNF_EINVAL, "images do not match the patch grids");
+    }
+    OclTree &t = c->ocl;
+    ocl_tree_shape(n_b, leaf_size, t.height, t.q, t.ppl, t.mp);
+    if (t.height > 29) return fail(ANNF_EUNSUPPORTED, "tree height %d: the traversal keeps its path in 31 bits",
+                                   t.height);
+    if ((long long)t.mp * d > 0x7FFFFFFFLL) return fail(ANNF_EUNSUPPORTED, "padded tree too large");
+    RCHK(set_device(c));
+    c->has_ocl = false;
+    const size_t nq = (size_t)std::max(t.q, 1), mp = (size_t)t.mp;
+    RCHK(c->ocl_lub.ensure(2 * (size_t)d * sizeof(uint32_t)));
+    RCHK(c->ocl_indir.ensure(mp * sizeof(int)));
+    RCHK(c->ocl_indir2.ensure(mp * sizeof(int)));
+    RCHK(c->ocl_keys.ensure(mp * sizeof(unsigned long long)));
+    RCHK(c->ocl_keys2.ensure(mp * sizeof(unsigned long long)));
+    RCHK(c->ocl_dims.ensure(nq * sizeof(int)));
+    RCHK(c->ocl_clanc.ensure(nq * sizeof(int)));
+    RCHK(c->ocl_vals.ensure(nq * sizeof(float)));
+    RCHK(c->ocl_leafpts.ensure(mp * d * sizeof(float)));
+    RCHK(c->ocl_o2l.ensure(mp * sizeof(int)));
+    const size_t sort_bytes = ocl_sort_bytes(t.mp);
+    RCHK(c->ocl_sort.ensure(sort_bytes));
+    RCHK(c->ocl_nat.ensure((size_t)n_a * sizeof(int)));
+    RCHK(c->ocl_pos.ensure((size_t)n_a * 8 * sizeof(int)));
+    RCHK(c->ocl_dst.ensure((size_t)n_a * 8 * sizeof(float)));
+    t.lub = c->ocl_lub.as<uint32_t>();
+    t.indir = c->ocl_indir.as<int>();
+    t.indir2 = c->ocl_indir2.as<int>();
+    t.keys = c->ocl_keys.as<unsigned long long>();
+    t.keys2 = c->ocl_keys2.as<unsigned long long>();
+    t.dims = c->ocl_dims.as<int>();
+    t.clanc = c->ocl_clanc.as<int>();
+    t.vals = c->ocl_vals.as<float>();
+    t.leaf_pts = c->ocl_leafpts.as<float>();
+    t.orig2leaf = c->ocl_o2l.as<int>();
+    HIPCHK(ocl_build(red_b, n_b, d, leaf_size, t, c->ocl_sort.p, sort_bytes, c->stream));
+    HIPCHK(ocl_queries(t, red_a, n_rows, n_cols, d, c->ocl_nat.as<int>(), c->ocl_pos.as<int>(),
+                       c->ocl_dst.as<float>(), c->stream));
+    HIPCHK(ocl_best(c->ocl_pos.as<int>(), t.indir, n_a, img_a, wa, wa - psize + 1, img_b, wb, wb - psize + 1, psize,
+                    knn_out, nn_out, c->stream));
+    if (knd_out)
+        HIPCHK(hipMemcpyAsync(knd_out, c->ocl_dst.p, (size_t)n_a * 8 * sizeof(float), hipMemcpyDeviceToDevice,
+                              c->stream));
+    RCHK(sync_stream(c));
+    c->has_ocl = true;
+    return ANNF_OK;
+}
+
+int annf_openclopt_tree_export(annf_ctx *c, int *shape, int *dims, float *vals, int *clanc, int *indir,
+                               int *orig2leaf) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!c->has_ocl) return fail(ANNF_ENOTREE, "no openclopt-semantics field computed on this handle");
+    RCHK(set_device(c));
+    const OclTree &t = c->ocl;
+    if (shape) { shape[0] = t.height; shape[1] = t.q; shape[2] = t.ppl; shape[3] = t.mp; }
+    const size_t q = (size_t)t.q, mp = (size_t)t.mp;
+    if (dims && q) HIPCHK(hipMemcpyAsync(dims, t.dims, q * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (vals && q) HIPCHK(hipMemcpyAsync(vals, t.vals, q * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (clanc && q) HIPCHK(hipMemcpyAsync(clanc, t.clanc, q * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (indir) HIPCHK(hipMemcpyAsync(indir, t.indir, mp * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (orig2leaf) HIPCHK(hipMemcpyAsync(orig2leaf, t.orig2leaf, mp * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    return sync_stream(c);
 }
 
 int annf_set_timing(annf_ctx *c, int on) {

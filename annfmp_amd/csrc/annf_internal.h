@@ -211,6 +211,31 @@ hipError_t pca_cov(const PatchGeom &ga, const int *ia, int na, const PatchGeom &
 // eigenvectors V (columns, D x D column-major) and ascending eigenvalues w
 hipError_t pca_components(const double *V, const double *w, int D, int d, double *comp, double *var, hipStream_t st);
 
+// The openclopt backend's semantics (annf_ocl.hip): the tree of
+// buildKDtree.fut on m points, padded to mp = (q + 1) * ppl with +inf rows.
+struct OclTree {
+    int height = -1, q = 0, ppl = 0, mp = 0;
+    uint32_t *lub = nullptr;                  // 2d: global bounds (ordered keys)
+    int *indir = nullptr, *indir2 = nullptr;  // mp: tree position -> point (and sort ping-pong)
+    unsigned long long *keys = nullptr, *keys2 = nullptr;   // mp sort keys
+    int *dims = nullptr, *clanc = nullptr;    // q
+    float *vals = nullptr;                    // q
+    float *leaf_pts = nullptr;                // mp x d, rows in tree order
+    int *orig2leaf = nullptr;                 // mp
+};
+void ocl_tree_shape(int m, int defppl, int &height, int &q, int &ppl, int &mp);
+size_t ocl_sort_bytes(int mp);
+hipError_t ocl_build(const float *X, int m, int d, int defppl, const OclTree &t, void *sort_tmp, size_t sort_bytes,
+                     hipStream_t st);
+// natural-leaf lists of every query, exact lists for row 0, the row chain:
+// pos / dst are n_rows * n_cols x 8 (tree positions, squared distances)
+hipError_t ocl_queries(const OclTree &t, const float *qa, int n_rows, int n_cols, int d, int *nat, int *pos,
+                       float *dst, hipStream_t st);
+// positions -> original indices (knn, may be null) and the best of the 8 by
+// exact full-patch distance (nn, may be null; needs the images)
+hipError_t ocl_best(const int *pos, const int *indir, int n_a, const uint8_t *img_a, int wa, int nca,
+                    const uint8_t *img_b, int wb, int ncb, int p, int *knn, int *nn, hipStream_t st);
+
 int dpad_of(int d);
 bool dpad_supported(int dpad);
 bool k_supported(int K);

@@ -190,6 +190,50 @@ int annf_field_mm_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                       int *nn_out, float *dist_out, int *knn_out,
                       float *red_a_out, float *red_b_out);
 
+/* Projection alone (openmp/base.py:193-200's transform, or the handle's
+ * encoder): every p x p patch of a device image (h x w x 3 uint8) into out
+ * (n x d float32, patch order of the reference's extract_patches_2d), with
+ * the same arithmetic as annf_field_dev's projection of that image. */
+int annf_project_dev(annf_ctx *ctx, const uint8_t *img, int h, int w,
+                     int psize, const void *basis, const double *mean, int d,
+                     int projection, float *out);
+
+/* ---- the openclopt backend's semantics (replaces
+ *      annfmp/openclopt/src/base.c fit_extern:257-520, which runs
+ *      futhark/driverKNN.fut's buildKDtree, findNaturalLeavesFixK,
+ *      exactKnnFixK, propagateFixK and selectBestNN) --------------------
+ * On reduced patches (device float32, A: n_rows * n_cols x d, B: n_b x d):
+ * the tree of buildKDtree.fut:19-144 (ceil(n_b / leaf_size) rounded to the
+ * nearest power of two leaves of ppl = ceil(n_b / leaves) points, padded
+ * with +inf rows; split dimension = the longest side of the box the
+ * ancestors' medians cut from the global bounds; split value = the mean of
+ * the sorted node's middle pair; stable sort), descent left on <=
+ * (kd-traverse.fut:10), K fixed to 8 (driverKNN.fut:16): natural-leaf lists,
+ * the exact 8-NN for row 0 (the stackless traverseOnce, one query per
+ * lane), then row by row the leaves of the B patches one row below the row
+ * above's list, without the natural leaf and repeats (driverKNN.fut:326-349).
+ * Outputs (device, any may be NULL): knn_out (n_a x 8 original B indices,
+ * -1 where a list is unfilled), knd_out (n_a x 8 squared reduced
+ * distances), nn_out (n_a: the first strict minimum of the exact full-patch
+ * distance over the 8, selectBestNN; needs both images, h x w x 3 uint8,
+ * whose patch grids match the reduced sets).  n_rows > 1 needs n_b = n_a (the
+ * row chain reads B indices on A's grid).  FP32 operations are rounded one
+ * by one. */
+int annf_field_openclopt_dev(annf_ctx *ctx, const float *red_a, int n_rows,
+                             int n_cols, const float *red_b, int n_b, int d,
+                             int leaf_size, const uint8_t *img_a, int ha,
+                             int wa, const uint8_t *img_b, int hb, int wb,
+                             int psize, int *nn_out, int *knn_out,
+                             float *knd_out);
+/* The last openclopt-semantics tree of the handle, to host buffers (any may
+ * be NULL): shape = {height, internal nodes q, points per leaf, padded
+ * count m'}, dims / vals / clanc (q: split dimension, median, closest
+ * ancestor splitting the same dimension or -1), indir (m': tree position ->
+ * point, >= n_b for pads), orig2leaf (m': point -> leaf). */
+int annf_openclopt_tree_export(annf_ctx *ctx, int *shape, int *dims,
+                               float *vals, int *clanc, int *indir,
+                               int *orig2leaf);
+
 /* ---- learned patch encoder (no reference counterpart; replaces the PCA
  *      projection of openmp/base.py:193-200 as the tree's feature map) ----
  * An MLP D -> h1 -> h2 -> d (D = p*p*3), GELU (tanh form) after the hidden

@@ -26,6 +26,10 @@ Plus numpy restatements of the Python-side stages:
 * ``leaf_knn_propagated`` (and ``_rows`` for a sample of the queries) -- the
                           same over the probed leaves plus the leaves the row
                           above's matches propagate (base.c:255-282)
+
+and the openclopt backend's semantics (``ocl_oracle.c``, restating
+annfmp/openclopt/src/futhark/*.fut): ``ocl_tree_shape``, ``OclTree``,
+``ocl_brute_force`` / ``ocl_brute_force_par`` and ``ocl_field``.
 """
 from __future__ import annotations
 
@@ -274,6 +278,16 @@ def lib():
         L.orc_best_of_k_images.argtypes = [_u8p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, _ip, C.c_int,
                                            _ip, _fp, C.c_int]
         L.orc_best_of_k_images.restype = C.c_int
+        i = C.c_int
+        L.ocl_tree_shape.argtypes = [i, i, _ip, _ip, _ip, _ip]
+        L.ocl_build_tree.argtypes = [_fp, i, i, i, _ip, _fp, _ip, _ip, _fp, _ip]
+        L.ocl_find_leaf.argtypes = [_ip, _fp, i, _fp]
+        L.ocl_find_leaf.restype = i
+        L.ocl_brute_force.argtypes = [_fp, i, _fp, i, i, _ip, _fp]
+        L.ocl_brute_force_par.argtypes = [_fp, i, _fp, i, i, _ip, _fp]
+        L.ocl_exact_knn.argtypes = [i, _ip, _fp, _ip, _fp, i, i, _fp, i, _ip, _fp]
+        L.ocl_field.argtypes = [_fp, i, i, _fp, i, i, i, _u8p, _u8p, i, _ip, _ip, _fp, _ip, _ip, _fp, _ip, _ip, _ip,
+                                i]
         _lib = L
     return _lib
 
@@ -515,3 +529,123 @@ def ref_norm_array(a, b):
     a = np.ascontiguousarray(a, dtype=np.uint8)
     b = np.ascontiguousarray(b, dtype=np.uint8)
     return ref_lib().norm_array(_ptr(a, _u8p), _ptr(b, _u8p), a.size)
+
+
+# --------------------------------------------------------------------------
+# the openclopt backend's semantics (ocl_oracle.c)
+# --------------------------------------------------------------------------
+
+OCL_K = 8
+
+
+def ocl_tree_shape(m, leaf_size):
+    """buildKDtree.fut:19-30: (height, internal nodes, points per leaf, padded count)."""
+    h, q, ppl, mp = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    lib().ocl_tree_shape(int(m), int(leaf_size), C.byref(h), C.byref(q), C.byref(ppl), C.byref(mp))
+    return h.value, q.value, ppl.value, mp.value
+
+
+class OclTree:
+    """The openclopt tree of X (m x d float32): dims / vals / clanc per
+    internal node, indir (tree position -> padded point), leaf_pts (rows in
+    tree order, +inf pads) and orig2leaf."""
+
+    def __init__(self, X, leaf_size):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        m, d = X.shape
+        self.m, self.d = m, d
+        self.height, self.q, self.ppl, self.mp = ocl_tree_shape(m, leaf_size)
+        nq = max(self.q, 1)
+        self.dims = np.zeros(nq, np.int32)
+        self.vals = np.zeros(nq, np.float32)
+        self.clanc = np.zeros(nq, np.int32)
+        self.indir = np.empty(self.mp, np.int32)
+        self.leaf_pts = np.empty((self.mp, d), np.float32)
+        self.orig2leaf = np.empty(self.mp, np.int32)
+        lib().ocl_build_tree(_ptr(X, _fp), m, d, int(leaf_size), _ptr(self.dims, _ip), _ptr(self.vals, _fp),
+                             _ptr(self.clanc, _ip), _ptr(self.indir, _ip), _ptr(self.leaf_pts, _fp),
+                             _ptr(self.orig2leaf, _ip))
+        self.dims, self.vals, self.clanc = self.dims[:self.q], self.vals[:self.q], self.clanc[:self.q]
+
+    def find_leaf(self, qv):
+        qv = np.ascontiguousarray(qv, dtype=np.float32)
+        dims = self.dims if self.q else np.zeros(1, np.int32)
+        vals = self.vals if self.q else np.zeros(1, np.float32)
+        return lib().ocl_find_leaf(_ptr(dims, _ip), _ptr(vals, _fp), self.height, _ptr(qv, _fp))
+
+    def leaf(self, l):
+        return self.leaf_pts[l * self.ppl:(l + 1) * self.ppl]
+
+    def natural_list(self, qv):
+        """findNaturalLeaves for one query: (positions, squared distances)."""
+        ki = np.full(OCL_K, -1, np.int32)
+        kd = np.full(OCL_K, np.finfo(np.float32).max, np.float32)
+        l = self.find_leaf(qv)
+        ocl_brute_force(qv, self.leaf(l), l * self.ppl, ki, kd)
+        return ki, kd
+
+    def exact_knn(self, qv):
+        """exactKnn for one query from its natural-leaf list."""
+        qv = np.ascontiguousarray(qv, dtype=np.float32)
+        ki, kd = self.natural_list(qv)
+        dims = self.dims if self.q else np.zeros(1, np.int32)
+        vals = self.vals if self.q else np.zeros(1, np.float32)
+        clanc = self.clanc if self.q else np.full(1, -1, np.int32)
+        lib().ocl_exact_knn(self.height, _ptr(dims, _ip), _ptr(vals, _fp), _ptr(clanc, _ip),
+                            _ptr(self.leaf_pts, _fp), self.d, self.ppl, _ptr(qv, _fp), self.find_leaf(qv),
+                            _ptr(ki, _ip), _ptr(kd, _fp))
+        return ki, kd
+
+
+def ocl_brute_force(qv, pts, beg, ki, kd):
+    """brute-force.fut:5-22 on (ki, kd) in place."""
+    qv = np.ascontiguousarray(qv, dtype=np.float32)
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    lib().ocl_brute_force(_ptr(qv, _fp), qv.size, _ptr(pts, _fp), int(beg), pts.shape[0], _ptr(ki, _ip),
+                          _ptr(kd, _fp))
+
+
+def ocl_brute_force_par(qv, pts, beg, ki, kd):
+    """brute-force.fut:25-66 on (ki, kd) in place."""
+    qv = np.ascontiguousarray(qv, dtype=np.float32)
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    lib().ocl_brute_force_par(_ptr(qv, _fp), qv.size, _ptr(pts, _fp), int(beg), pts.shape[0], _ptr(ki, _ip),
+                              _ptr(kd, _fp))
+
+
+def ocl_field(red_a, n_rows, n_cols, red_b, leaf_size, pa=None, pb=None, rows=0):
+    """The openclopt pipeline on reduced patches: dict of nn (when pa/pb
+    given), knn / knd (n_a x 8: original indices, squared reduced
+    distances), nat (natural leaves) and the tree arrays.  rows > 0: only
+    the first `rows` rows (outputs cut to them)."""
+    red_a = np.ascontiguousarray(red_a, dtype=np.float32)
+    red_b = np.ascontiguousarray(red_b, dtype=np.float32)
+    n_a, d = red_a.shape
+    m = red_b.shape[0]
+    assert n_a == n_rows * n_cols and red_b.shape[1] == d
+    h, q, ppl, mp = ocl_tree_shape(m, leaf_size)
+    out = {"knn": np.empty((n_a, OCL_K), np.int32), "knd": np.empty((n_a, OCL_K), np.float32),
+           "nat": np.empty(n_a, np.int32), "dims": np.zeros(max(q, 1), np.int32),
+           "vals": np.zeros(max(q, 1), np.float32), "clanc": np.zeros(max(q, 1), np.int32),
+           "indir": np.empty(mp, np.int32), "orig2leaf": np.empty(mp, np.int32)}
+    nn = np.full(n_a, -1, np.int32)
+    D = 0
+    if pa is not None:
+        pa = np.ascontiguousarray(pa, dtype=np.uint8)
+        pb = np.ascontiguousarray(pb, dtype=np.uint8)
+        D = pa.shape[1]
+    lib().ocl_field(_ptr(red_a, _fp), n_rows, n_cols, _ptr(red_b, _fp), m, d, int(leaf_size),
+                    _ptr(pa, _u8p) if pa is not None else None, _ptr(pb, _u8p) if pb is not None else None, D,
+                    _ptr(nn, _ip), _ptr(out["knn"], _ip), _ptr(out["knd"], _fp), _ptr(out["nat"], _ip),
+                    _ptr(out["dims"], _ip), _ptr(out["vals"], _fp), _ptr(out["clanc"], _ip), _ptr(out["indir"], _ip),
+                    _ptr(out["orig2leaf"], _ip), int(rows))
+    if 0 < rows < n_rows:
+        for k in ("knn", "knd", "nat"):
+            out[k] = out[k][:rows * n_cols]
+        nn = nn[:rows * n_cols]
+    for k in ("dims", "vals", "clanc"):
+        out[k] = out[k][:q]
+    if pa is not None:
+        out["nn"] = nn
+    out["shape"] = (h, q, ppl, mp)
+    return out
