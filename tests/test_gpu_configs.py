@@ -237,15 +237,15 @@ def test_4k_learned_fp8_top32_propagation_properties(pair_4k, gpu_available):
 # the image-level front against the oracle on the same images
 # ---------------------------------------------------------------------------
 
-def _front_vs_oracle(a, b, p, d, leaf, K, n_subset):
-    """ANNField(algorithm="hip", projection="fp64") end to end on the two
+def _front_vs_oracle(a, b, p, d, leaf, K, n_subset, projection="fp64"):
+    """ANNField(algorithm="hip", projection=projection) end to end on the two
     images, and the oracle on the same images and the same fitted PCA:
     oracle.project_f64 (scikit-learn's FP64 transform then astype(float32),
     openmp/base.py:193-200), the oracle's tree and rows.  Returns the
     comparison figures."""
     from annfmp_amd import ANNField
     f = ANNField(n_neighbors=K, psize=p, dim_reduced=d, n_subset=n_subset, algorithm="hip", leaf_size=leaf,
-                 verbose=0, projection="fp64")
+                 verbose=0, projection=projection)
     nn = f.fit(a, b)
     pca = f._model._pca_model
     pa, pb = O.extract_patches(a, p), O.extract_patches(b, p)
@@ -257,7 +257,7 @@ def _front_vs_oracle(a, b, p, d, leaf, K, n_subset):
     final, _, _ = ref.process_rows(ra, nr, nc, K, True, True, pa, pb)
     # the front's own projected arrays, through the device entry with the same basis
     from annfmp_amd import field_on_device
-    res = field_on_device(a, b, p, pca.components_, pca.mean_, depth, K, projection="fp64", return_reduced=True)
+    res = field_on_device(a, b, p, pca.components_, pca.mean_, depth, K, projection=projection, return_reduced=True)
     assert np.array_equal(res["nn"], nn)
     def ulp(x, y):   # |x - y| in units in the last place of the larger magnitude (float32)
         big = np.maximum(np.abs(x), np.abs(y)).astype(np.float32)
@@ -268,7 +268,8 @@ def _front_vs_oracle(a, b, p, d, leaf, K, n_subset):
     # a large value, a few ulps of a value near zero
     scale = float(max(np.abs(ra).max(), np.abs(rb).max()))
     absd = max(np.abs(res["red_a"].astype(np.float64) - ra).max(), np.abs(res["red_b"].astype(np.float64) - rb).max())
-    assert absd <= 1e-7 * scale, (absd, scale)
+    if projection == "fp64":
+        assert absd <= 1e-7 * scale, (absd, scale)
     mism = np.nonzero(nn != final)[0]
     out = {"n": nn.size, "nn_mismatch": int(mism.size), "red_ulp_max": float(max(ua.max(), ub.max())),
            "red_differ": int((ua > 0).sum() + (ub > 0).sum()), "red_values": int(ua.size + ub.size),
@@ -281,7 +282,7 @@ def _front_vs_oracle(a, b, p, d, leaf, K, n_subset):
     if out["red_differ"] == 0:
         # identical reduced arrays: identical tree and field
         assert mism.size == 0
-    print("front vs oracle:", out)
+    print("front (%s) vs oracle:" % projection, out)
     return out, res, ra, rb
 
 
@@ -308,6 +309,30 @@ def test_front_fp64_matches_oracle_1080p(gpu_available):
     out, res, ra, rb = _front_vs_oracle(a, b, 11, 16, 64, 5, 50000)
     assert out["red_differ"] <= 1e-5 * out["red_values"]
     assert out["nn_mismatch"] <= 1e-4 * out["n"]
+
+
+@pytest.mark.parametrize("workload", ["256", "1080p"])
+def test_front_fp32_projection_against_the_reference_transform(workload, gpu_available):
+    """The default front projects in FP32 (summation order of the MFMA GEMM),
+    the reference in FP64 then casts (openmp/base.py:193-200): how many NNF
+    entries that changes on the same images and basis.  A reduced value moves
+    by a few FP32 ulps, which reorders near-equal keys of the tree build and
+    near-equal candidates, and the row chain carries a changed match down its
+    column.  Measured: 256x256 0 of 62,500 patches, 1080p 15,194 of 2,043,700
+    (0.74 %), where the two chosen matches' full-patch distances differ by up
+    to 0.81 relative (a changed match travels down the column); the reduced
+    values differ by at most 1.0e-6 of the largest."""
+    from annfmp_amd import synth
+    if workload == "256":
+        a, b = synth.make_pair("256x256-shift", seed=0)
+        out, res, ra, rb = _front_vs_oracle(a, b, 7, 10, 32, 1, 10000, projection="fp32")
+    else:
+        a, b = synth.make_pair("1080p-warp", seed=0)
+        out, res, ra, rb = _front_vs_oracle(a, b, 11, 16, 64, 5, 50000, projection="fp32")
+    rate = out["nn_mismatch"] / out["n"]
+    print(f"fp32 front, {workload}: {out['nn_mismatch']} of {out['n']} NNF entries differ ({100 * rate:.3f} %)")
+    assert out["abs_diff_max_over_scale"] <= 2e-5
+    assert rate <= 0.05
 
 
 def test_front_runs_the_reference_example_configuration(gpu_available):
