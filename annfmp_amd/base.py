@@ -328,7 +328,10 @@ def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_
 
     precision: "bf16" (exact distances) or "fp8" (shortlist re-ranked
     exactly); nprobe: leaves per query (a power of two); propagate: False,
-    True (all K entries of the list above propagate) or a count."""
+    True (all K entries of the list above propagate) or a count, from the
+    first-pass lists of the row above (one parallel pass), or "chain" (all K,
+    from the row above's final lists, rows in order: the reference's
+    dependence; BF16 only)."""
     torch = dev.torch()
     h = handle or _lib.Handle(device)
     L = _lib.lib()
@@ -353,7 +356,7 @@ def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_
     _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
         h.h, dev.dptr(ia), ha, wa, dev.dptr(ib), hb, wb, int(psize), dev.dptr(comp), dev.dptr(mu), int(d),
         proj, int(tree_depth), K, int(nprobe), prec, int(shortlist),
-        (K if propagate is True else int(propagate or 0)), dev.dptr(nn),
+        (K if propagate is True else (-K if propagate == "chain" else int(propagate or 0))), dev.dptr(nn),
         dev.dptr(dist), dev.dptr(knn), dev.dptr(ra), dev.dptr(rb)))
     wall = time.time() - t0
     st = numpy.zeros(7, numpy.float32)

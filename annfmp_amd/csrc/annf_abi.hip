@@ -108,7 +108,7 @@ struct annf_ctx {
     // learned encoder: training workspace and the packed inference weights
     DevBuf mlp_ws, enc_w1, enc_b1, enc_w2, enc_b2, enc_w3, enc_b3;
     // full-patch leaf k-NN: per-query and per-pair exact keys, pair leaves
-    DevBuf mm_keys1, mm_keys2, mm_pairleaf;
+    DevBuf mm_keys1, mm_keys2, mm_pairleaf, mm_fkeys;
     DevBuf pca_scratch, pca_eig;
     // the openclopt-semantics mode: its tree and row lists
     DevBuf ocl_lub, ocl_indir, ocl_indir2, ocl_keys, ocl_keys2, ocl_dims, ocl_clanc, ocl_vals, ocl_leafpts,
@@ -162,7 +162,7 @@ struct annf_ctx {
                           &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &ptshead, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
-                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &pca_scratch, &pca_eig, &ocl_lub, &ocl_indir, &ocl_indir2, &ocl_keys,
+                          &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &mm_fkeys, &pca_scratch, &pca_eig, &ocl_lub, &ocl_indir, &ocl_indir2, &ocl_keys,
                           &ocl_keys2, &ocl_dims, &ocl_clanc, &ocl_vals, &ocl_leafpts, &ocl_o2l, &ocl_sort, &ocl_nat,
                           &ocl_pos, &ocl_dst,
                           &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack, &cimg_a, &cimg_b,
@@ -907,11 +907,16 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         return fail(ANNF_EUNSUPPORTED, "shortlist %d outside [n_neighbors, 32]", shortlist);
     if (tree_depth < 0 || tree_depth > 30 || nprobe < 1 || nprobe > (1 << tree_depth) || (nprobe & (nprobe - 1)))
         return fail(ANNF_EINVAL, "nprobe %d is not a power of two in [1, 2^tree_depth]", nprobe);
-    if (propagate > 0 && wa != wb)
+    if (propagate != 0 && wa != wb)
         return fail(ANNF_EINVAL, "propagation needs images of equal width (base.c:271)");
+    // propagate < 0: the reference's row chain (rows in order, each from the
+    // final lists of the row above), exact BF16 only
+    const bool chain = propagate < 0;
+    if (chain && precision != ANNF_PREC_BF16)
+        return fail(ANNF_EUNSUPPORTED, "the row chain (propagate < 0) runs with ANNF_PREC_BF16");
+    if (propagate < -1000000) return fail(ANNF_EINVAL, "propagate %d out of range", propagate);
     if (precision == ANNF_PREC_BF16 && (2 * cimg_elems(ha, wa) >= (1LL << 31) || 2 * cimg_elems(hb, wb) >= (1LL << 31)))
         return fail(ANNF_EUNSUPPORTED, "the BF16 leaf k-NN needs images of fewer than 2^30 elements");
-    if (propagate < 0) return fail(ANNF_EINVAL, "propagate %d is negative", propagate);
     const int fp8 = precision == ANNF_PREC_FP8;
     // the exact BF16 pass reads both images as centred BF16 with per-patch
     // norms: made on the side stream while the tree is built
@@ -937,10 +942,11 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     const TreeView &t = c->tree;
     const int nleaves = 1 << t.depth;
     const int n_a = (int)fs.n_a;
-    const bool prop = propagate > 0 && fs.nr_a > 1;
+    const bool prop = propagate != 0 && fs.nr_a > 1;
     // the first prop_k entries of the list above propagate (K: all, base.c:255-282)
-    const int prop_k = std::min(propagate, K);
-    const size_t n_pairs = prop ? (size_t)n_a * prop_k : 0;
+    const int prop_k = std::min(chain ? -propagate : propagate, K);
+    const int nc_a = fs.nc_a, nr_a = fs.nr_a;
+    const size_t n_pairs = prop ? (size_t)(chain ? nc_a : n_a) * prop_k : 0;
     c->mark(3);
     RCHK(c->natleaf.ensure((size_t)n_a * sizeof(int)));
     RCHK(c->bcount.ensure((size_t)nleaves * sizeof(int)));
@@ -989,17 +995,40 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         RCHK(c->mm_pairleaf.ensure(n_pairs * sizeof(int)));
         RCHK(c->mm_keys2.ensure(n_pairs * K * sizeof(uint64_t)));
         keys2 = c->mm_keys2.as<uint64_t>();
-        HIPCHK(prop_pairs(c->mm_keys1.as<uint64_t>(), n_a, K, prop_k, nprobe, c->natleaf.as<int>(), t.pli, fs.nc_a,
-                          (int)fs.n_b, c->mm_pairleaf.as<int>(), nullptr, c->stream));
-        HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)n_pairs, nleaves, chunk, skeys,
-                             c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p,
-                             sort_bytes, c->stream));
-        HIPCHK(fill_keys(keys2, (long long)n_pairs * K, c->stream));
-        RCHK(leaf_pass((int)n_pairs, 1, prop_k, keys2, nullptr, nullptr, nullptr));
+        // pairs of queries q0 .. q0 + nq - 1 from `above` (keys by query),
+        // scanned into keys2
+        auto pair_pass = [&](const uint64_t *above, int q0, int nq) -> int {
+            const size_t np = (size_t)nq * prop_k;
+            HIPCHK(prop_pairs(above, q0, nq, K, prop_k, nprobe, c->natleaf.as<int>(), t.pli, nc_a, (int)fs.n_b,
+                              c->mm_pairleaf.as<int>(), nullptr, c->stream));
+            HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)np, nleaves, chunk, skeys, c->order.as<int>(),
+                                 c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
+            HIPCHK(fill_keys(keys2, (long long)np * K, c->stream));
+            if (fp8) return leaf_pass((int)np, 1, prop_k, keys2, nullptr, nullptr, nullptr);
+            HIPCHK(leaf_bf16(t, c->cimg_a.as<uint16_t>(), wa, c->cimg_b.as<uint16_t>(), wb, psize,
+                             c->pnorm_a.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), pts_order, c->order.as<int>(),
+                             c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, (int)np, 1, K, prop_k, keys2,
+                             nullptr, nullptr, nullptr, c->stream, ha, hb, q0));
+            return ANNF_OK;
+        };
+        if (!chain) {
+            RCHK(pair_pass(c->mm_keys1.as<uint64_t>(), 0, n_a));
+            HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, 0, n_a, K, prop_k, knn_out, nn_out, dist_out,
+                                nullptr, c->stream));
+        } else {
+            // row by row: row y's pairs come from row y - 1's final keys
+            // (row 0's are its first-pass keys), the reference's dependence
+            RCHK(c->mm_fkeys.ensure((size_t)n_a * K * sizeof(uint64_t)));
+            uint64_t *fk = c->mm_fkeys.as<uint64_t>();
+            HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), nullptr, 0, nc_a, K, 1, knn_out, nn_out, dist_out, fk,
+                                c->stream));
+            for (int y = 1; y < nr_a; ++y) {
+                RCHK(pair_pass(fk, y * nc_a, nc_a));
+                HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, y * nc_a, nc_a, K, prop_k, knn_out, nn_out,
+                                    dist_out, fk, c->stream));
+            }
+        }
     }
-    if (prop)
-        HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, n_a, K, std::max(prop_k, 1), knn_out, nn_out, dist_out,
-                        c->stream));
     c->mark(6);
     RCHK(sync_and_check_build(c));
     c->collect_marks();

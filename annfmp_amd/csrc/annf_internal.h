@@ -165,14 +165,20 @@ hipError_t patch_norms_centered(const uint8_t *img, int h, int w, int p, uint32_
 hipError_t leaf_bf16(const TreeView &t, const uint16_t *ca, int wa, const uint16_t *cb, int wb, int p,
                      const uint32_t *norm_a, const uint32_t *norm_b, const int *pts_order, const int *order,
                      const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int K, int slot_div,
-                     uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st, int ha, int hb);
-// propagated (query, leaf) pairs: pairleaf[q * prop_k + k] (-1: none, or a leaf the
-// first pass scanned: natleaf ^ r, r < nprobe), per-leaf counts
-hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, int nprobe, const int *natleaf,
+                     uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st, int ha, int hb,
+                     int a0 = 0);
+// propagated (query, leaf) pairs of the queries q0 .. q0 + nq - 1 from the
+// keys of the row above (keys1, by query): pairleaf[(q - q0) * prop_k + k]
+// (-1: none, or a leaf the first pass scanned: natleaf ^ r, r < nprobe),
+// per-leaf counts
+hipError_t prop_pairs(const uint64_t *keys1, int q0, int nq, int K, int prop_k, int nprobe, const int *natleaf,
                       const int *pli, int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st);
 hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st);
-hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, int K, int prop_k, int *knn, int *nn,
-                        float *dist, hipStream_t st);
+// the K smallest of each query's first-pass keys (keys1, by query) and its
+// pairs' keys (keys2, by query - q0) for queries q0 .. q0 + nq - 1; any
+// output may be null; keys_out: the final keys, by query
+hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int q0, int nq, int K, int prop_k, int *knn,
+                        int *nn, float *dist, uint64_t *keys_out, hipStream_t st);
 
 // learned encoder (annf_mlp.hip): parameters W1 (h1 x D), b1, W2 (h2 x h1),
 // b2, W3 (d x h2), b3 in one FP32 buffer; inputs (x - 128) / 128

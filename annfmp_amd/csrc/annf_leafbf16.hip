@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_bf16(TreeView t, CImg ia, 
                                                          const int *__restrict__ order,
                                                          const int *__restrict__ chunk_off,
                                                          const int4 *__restrict__ blk_info, int nprobe, int K,
-                                                         int slot_div, Out out) {
+                                                         int slot_div, int a0, Out out) {
     constexpr int NGM = 2 * KSM;
     constexpr int GPT = (NGM + kGroupSets - 1) / kGroupSets;   // groups per staging thread and tile
     __shared__ __align__(16) uint8_t bt[NGM * kSlot];        // [group][point][8 BF16]
@@ -247,7 +247,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_bf16(TreeView t, CImg ia, 
     uint32_t na = 0;
     bf16x8 af[KSM];
     {
-        const int apatch = slot / slot_div;
+        const int apatch = a0 + slot / slot_div;
         if (qv) na = norm_a[apatch];
         const uint32_t e0 = patch_base(ia, apatch);
 #pragma unroll
@@ -539,7 +539,7 @@ hipError_t patch_norms_centered(const uint8_t *img, int h, int w, int p, uint32_
 hipError_t leaf_bf16(const TreeView &t, const uint16_t *ca, int wa, const uint16_t *cb, int wb, int p,
                      const uint32_t *norm_a, const uint32_t *norm_b, const int *pts_order, const int *order,
                      const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int K, int slot_div,
-                     uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st, int ha, int hb) {
+                     uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st, int ha, int hb, int a0) {
     if (nq == 0) return hipSuccess;
     if (!leafbf16_supported(p)) return hipErrorInvalidValue;
     const int nleaves = 1 << t.depth;
@@ -558,13 +558,13 @@ hipError_t leaf_bf16(const TreeView &t, const uint16_t *ca, int wa, const uint16
             // p = 1-4 and 6 run these with the steps past their own skipped
             if (pk.ks != KSM) {
                 hipLaunchKernelGGL((k_leaf_bf16<KSM, LM, false>), dim3(nblk), dim3(kThreads), 0, st, t, ia, ib, pk,
-                                   norm_a, norm_b, pts_order, order, chunk_off, blk_info, nprobe, K, slot_div, out);
+                                   norm_a, norm_b, pts_order, order, chunk_off, blk_info, nprobe, K, slot_div, a0, out);
                 return hipGetLastError();
             }
         }
         if (pk.ks != KSM) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_leaf_bf16<KSM, LM, true>), dim3(nblk), dim3(kThreads), 0, st, t, ia, ib, pk, norm_a,
-                           norm_b, pts_order, order, chunk_off, blk_info, nprobe, K, slot_div, out);
+                           norm_b, pts_order, order, chunk_off, blk_info, nprobe, K, slot_div, a0, out);
         return hipGetLastError();
     });
 }

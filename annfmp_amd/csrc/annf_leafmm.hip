@@ -628,12 +628,12 @@ __global__ __launch_bounds__(kThreads, (FP8 && LM > 16) ? 1 : 2) void k_leaf_mm(
 // match of the patch above (first-pass keys), its leaf pli[.]; leaves the
 // first pass already scanned (natleaf[q] ^ r, r < nprobe) or equal to an
 // earlier k's are dropped (-1).  Counts the kept (q, k) pairs per leaf.
-__global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int n_a, int K, int prop_k, int nprobe,
+__global__ void k_prop_pairs(const uint64_t *__restrict__ keys1, int q0, int nq, int K, int prop_k, int nprobe,
                              const int *__restrict__ natleaf, const int *__restrict__ pli, int n_cols, int n_b,
                              int *__restrict__ pairleaf, int *__restrict__ count) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)n_a * prop_k) return;
-    const int q = (int)(t / prop_k), k = (int)(t - (long long)q * prop_k);
+    if (t >= (long long)nq * prop_k) return;
+    const int ql = (int)(t / prop_k), k = (int)(t - (long long)ql * prop_k), q = q0 + ql;
     int leaf = -1;
     if (q >= n_cols) {
         auto leaf_of = [&](int kk) -> int {
@@ -709,10 +709,12 @@ __global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
 // repeats, so the leaves are disjoint and no B index comes twice) -> knn,
 // nn, dist.
 template <int KM>
-__global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__restrict__ keys2, int n_a, int K,
-                         int prop_k, int *__restrict__ knn, int *__restrict__ nn, float *__restrict__ dist) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n_a) return;
+__global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__restrict__ keys2, int q0, int nq,
+                         int K, int prop_k, int *__restrict__ knn, int *__restrict__ nn, float *__restrict__ dist,
+                         uint64_t *__restrict__ keys_out) {
+    const int ql = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ql >= nq) return;
+    const int q = q0 + ql;
     uint64_t l[KM];
     klist_init(l, K);
     for (int s = 0; s < K; ++s) {
@@ -720,7 +722,7 @@ __global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__r
         if (key < l[KM - 1]) klist_insert(l, key);
     }
     if (keys2) {
-        const uint64_t *p2 = keys2 + (size_t)q * prop_k * K;
+        const uint64_t *p2 = keys2 + (size_t)ql * prop_k * K;
         for (int s = 0; s < prop_k * K; ++s) {
             const uint64_t key = p2[s];
             if (key < l[KM - 1]) klist_insert(l, key);
@@ -733,8 +735,9 @@ __global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__r
         const int j = key == kNoKey ? -1 : (int)(uint32_t)key;
         const int r = s - (KM - K);
         if (knn) knn[(size_t)q * K + r] = j;
+        if (keys_out) keys_out[(size_t)q * K + r] = key;
         if (r == 0) {
-            nn[q] = j;
+            if (nn) nn[q] = j;
             if (dist) dist[q] = key == kNoKey ? kFltMax : sqrt_f((uint32_t)(key >> 32));
         }
     }
@@ -768,11 +771,11 @@ int leafmm_dk(int p) {
 
 bool leafmm_supported(int p) { return p >= 1 && leafmm_dk(p) <= 512; }
 
-hipError_t prop_pairs(const uint64_t *keys1, int n_a, int K, int prop_k, int nprobe, const int *natleaf,
+hipError_t prop_pairs(const uint64_t *keys1, int q0, int nq, int K, int prop_k, int nprobe, const int *natleaf,
                       const int *pli, int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st) {
-    const long long n = (long long)n_a * prop_k;
+    const long long n = (long long)nq * prop_k;
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prop_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys1, n_a, K, prop_k,
+    hipLaunchKernelGGL(k_prop_pairs, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys1, q0, nq, K, prop_k,
                        nprobe, natleaf, pli, n_cols, n_b, pairleaf, count);
     return hipGetLastError();
 }
@@ -783,14 +786,14 @@ hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int n_a, int K, int prop_k, int *knn, int *nn,
-                        float *dist, hipStream_t st) {
-    if (n_a == 0) return hipSuccess;
-    const dim3 g((unsigned)((n_a + 255) / 256)), b(256);
+hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int q0, int nq, int K, int prop_k, int *knn,
+                        int *nn, float *dist, uint64_t *keys_out, hipStream_t st) {
+    if (nq == 0) return hipSuccess;
+    const dim3 g((unsigned)((nq + 255) / 256)), b(256);
     if (K <= 8)
-        hipLaunchKernelGGL(k_finish<8>, g, b, 0, st, keys1, keys2, n_a, K, prop_k, knn, nn, dist);
+        hipLaunchKernelGGL(k_finish<8>, g, b, 0, st, keys1, keys2, q0, nq, K, prop_k, knn, nn, dist, keys_out);
     else
-        hipLaunchKernelGGL(k_finish<32>, g, b, 0, st, keys1, keys2, n_a, K, prop_k, knn, nn, dist);
+        hipLaunchKernelGGL(k_finish<32>, g, b, 0, st, keys1, keys2, q0, nq, K, prop_k, knn, nn, dist, keys_out);
     return hipGetLastError();
 }
 

@@ -287,7 +287,9 @@ def main():
     ap.add_argument("--nprobe", type=int, default=1, help="leaves probed per query on the BF16 path")
     ap.add_argument("--encoder", default=DEFAULT_ENCODER, help="trained encoder (.npz); 'none' forces PCA")
     ap.add_argument("--paths", default="plain,pca,learned",
-                    help="paths to run (profiling runs select one); the value needs pca and learned")
+                    help="paths to run (profiling runs select one); the value needs pca and learned; "
+                         "learned_prop / learned_chain add the learned BF16 form with the parallel propagated "
+                         "pass / the row chain")
     ap.add_argument("--no-quality", action="store_true", help="skip the exact-field quality evaluation")
     ap.add_argument("--workload", default="1080p", choices=["1080p", "4k"],
                     help="1080p: the metric (default); 4k: the combined form on the 3840x2160 pair")
@@ -395,6 +397,22 @@ def main():
         if chosen is None or args.no_quality or \
                 paths["learned_bf16"]["recall@5"] >= paths["pca_bf16"]["recall@5"]:
             chosen = "learned_bf16"
+    if encoder is not None and ("learned_prop" in run_paths or "learned_chain" in run_paths):
+        # the propagated pass beside the metric's path: one parallel pass from
+        # the first-pass lists of the row above, or the reference's row chain
+        # (rows in order from the row above's final lists); not the value
+        for name, prop in (("learned_prop", K), ("learned_chain", -K)):
+            if name not in run_paths:
+                continue
+            def mm_prop(knn_out=None, prop=prop):
+                _lib.check("annf_field_mm_dev", L.annf_field_mm_dev(
+                    h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d,
+                    _lib.PROJ_ENCODER, depth, K, args.nprobe, _lib.PREC_BF16, K, prop, dev.dptr(nn), None,
+                    dev.dptr(knn_out), None, None))
+            ms, stage = timed(mm_prop, max(1, args.steps // 5), 1, torch, _lib, h, 6)
+            mm_prop(knn)
+            paths[name] = {"ms_per_step": round(ms, 4), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                           "stage_ms": {s: round(float(v), 4) for s, v in zip(MM_STAGES, stage)}, **quality()}
     if chosen is None:
         print(json.dumps({"paths": paths}), flush=True)
         return

@@ -166,8 +166,10 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
  * patch above (m >= K: all of them, the reference's rule; dropping its
  * natural leaf and repeats) are scanned the same way, and the K smallest
  * keys of both passes are kept.  The candidates come from the
- * first pass of the row above, not from its propagated result (the
- * reference sweeps rows in order); this needs wa == wb.  A propagated leaf
+ * first pass of the row above, not from its propagated result (one parallel
+ * pass); propagate = -m < 0 (ANNF_PREC_BF16 only) is the reference's row
+ * chain instead: rows in order, each taking its candidates from the final
+ * lists of the row above.  Either needs wa == wb.  A propagated leaf
  * the first pass already scanned (natural leaf ^ r, r < nprobe) is left
  * out, so no B index is listed twice.
  * Outputs as annf_field_dev: nn_out is the first list entry, dist_out its

@@ -162,14 +162,16 @@ def leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k):
     return idx, dst
 
 
-def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=1, prop_k=None):
+def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=1, prop_k=None, chain=False):
     """The full-patch leaf k-NN with the parallel propagated pass: pass one is
     leaf_knn_exact over the leaves qleaf ^ 0 .. qleaf ^ (nprobe - 1); a patch
     below the first row then also scans the leaves pli[o'] of the first
     prop_k (default k) pass-one matches o of its row-above neighbour, shifted
     one row down (o' = o + n_cols when that is a B patch, base.c:265-276),
     leaving out the leaves pass one scanned and repeats, and keeps the k
-    smallest (squared distance, B index) over all scanned points.
+    smallest (squared distance, B index) over all scanned points.  chain:
+    the matches of the row above are its final ones (rows in order, the
+    reference's row sweep, base.c:227-306) instead of its pass-one ones.
     Returns (idx [n_a, k], sqdist [n_a, k]) int64, -1 past the candidates."""
     prop_k = k if prop_k is None else min(prop_k, k)
     i1, d1 = leaf_knn_exact(pa, pb, qleaf, leaves, orig, nprobe, k)
@@ -180,8 +182,9 @@ def leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=1, p
     orig = np.asarray(orig, np.int64)
     for q in range(n_cols, n_a):
         ls = [int(qleaf[q]) ^ r for r in range(nprobe)]
+        above = out_i if chain else i1
         for j in range(prop_k):
-            o = int(i1[q - n_cols, j])
+            o = int(above[q - n_cols, j])
             if o < 0:
                 continue
             if o + n_cols < n_b:

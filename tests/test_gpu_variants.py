@@ -183,6 +183,30 @@ def test_bf16_propagation_with_several_probed_leaves(nprobe, prop, gpu_available
     assert all(len(set(r)) == K for r in knn.tolist())
 
 
+@pytest.mark.parametrize("nprobe,seed", [(1, 71), (2, 72)])
+def test_bf16_row_chain_matches_oracle(nprobe, seed, gpu_available):
+    """propagate="chain": rows in order, each row's propagated leaves from the
+    row above's final lists (the reference's dependence, base.c:227-306);
+    equal to the oracle's chain bit for bit, and it reaches matches the
+    parallel pass does not."""
+    from annfmp_amd import synth, field_mm_on_device
+    a, b = synth.make_pair(synth.small_spec(96, 128), seed=seed)
+    p, K = 7, 5
+    pa, pb = O.extract_patches(a, p), O.extract_patches(b, p)
+    comp, mu = _basis(pa, pb, 8)
+    depth = O.tree_depth(pb.shape[0], 32)
+    res = field_mm_on_device(a, b, p, comp, mu, depth, K, nprobe=nprobe, precision="bf16", propagate="chain",
+                             return_reduced=True)
+    tree = O.OracleTree(res["red_b"], depth)
+    qleaf = tree.natural_leaves(res["red_a"])
+    n_cols = a.shape[1] - p + 1
+    wi, wd = O.leaf_knn_propagated(pa, pb, qleaf, tree.leaves, tree.orig, tree.pli, n_cols, K, nprobe=nprobe,
+                                   chain=True)
+    _check_bf16(res, np.arange(pa.shape[0]), wi, wd)
+    pi, pd = O.leaf_knn_propagated(pa, pb, qleaf, tree.leaves, tree.orig, tree.pli, n_cols, K, nprobe=nprobe)
+    assert (wd[:, 0] < pd[:, 0]).any()
+
+
 def test_fp8_propagation_properties(gpu_available):
     from annfmp_amd import synth, field_mm_on_device
     a, b = synth.make_pair(synth.small_spec(96, 128), seed=44)

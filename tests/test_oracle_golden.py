@@ -252,6 +252,41 @@ def test_leaf_knn_propagated_is_the_best_of_its_leaves(nprobe, prop_k):
     assert (ip[:n_cols] == i1[:n_cols]).all()
 
 
+@pytest.mark.parametrize("nprobe,prop_k", [(1, None), (2, 2)])
+def test_leaf_knn_chain_is_the_best_of_its_leaves(nprobe, prop_k):
+    """With chain=True a query's propagated leaves come from the final list
+    of the patch above (rows in order, base.c:227-306): every list is the
+    exact best over its probed leaves and those leaves, collected here from
+    the function's own output one row up; and the chain reaches points the
+    parallel form does not."""
+    rng = np.random.default_rng(9)
+    n_rows, n_cols, D, k = 10, 9, 12, 3
+    n = n_rows * n_cols
+    pa = rng.integers(0, 256, (n, D)).astype(np.uint8)
+    pb = rng.integers(0, 256, (n, D)).astype(np.uint8)
+    leaves = np.array([0, 12, 12, 23, 23, 34, 34, 45, 45, 56, 56, 67, 67, 78, 78, 90])
+    orig = rng.permutation(n)
+    pli = np.empty(n, np.int64)
+    for lf in range(8):
+        pli[orig[leaves[2 * lf]:leaves[2 * lf + 1]]] = lf
+    qleaf = rng.integers(0, 8, n)
+    ic, dc = O.leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=nprobe, prop_k=prop_k,
+                                   chain=True)
+    ip, _ = O.leaf_knn_propagated(pa, pb, qleaf, leaves, orig, pli, n_cols, k, nprobe=nprobe, prop_k=prop_k)
+    pk = k if prop_k is None else prop_k
+    for q in range(n):
+        allowed = {int(qleaf[q]) ^ r for r in range(nprobe)}
+        if q >= n_cols:
+            for o in ic[q - n_cols][:pk]:
+                o2 = o + n_cols if o + n_cols < n else o
+                allowed.add(int(pli[o2]))
+        pts = np.concatenate([orig[leaves[2 * lf]:leaves[2 * lf + 1]] for lf in sorted(allowed)])
+        d_all = sorted((int(((pa[q].astype(int) - pb[j].astype(int)) ** 2).sum()), int(j)) for j in pts)
+        assert [j for _, j in d_all[:k]] == list(ic[q]) and [x for x, _ in d_all[:k]] == list(dc[q])
+    assert (ic[:2 * n_cols] == ip[:2 * n_cols]).all()     # rows 0 and 1 agree by construction
+    assert (ic != ip).any()
+
+
 @pytest.mark.parametrize("shape,p,d,n_fit", [((256, 256), 7, 10, 10000), ((300, 310), 11, 16, 5000),
                                               ((64, 80), 5, 40, 2000)])
 def test_project_f64_is_sklearns_transform(shape, p, d, n_fit):
