@@ -9,6 +9,9 @@ namespace annf {
 constexpr int kWave = 64;
 constexpr float kFltMax = 3.402823466e+38f;  // list sentinel, float.h:24
 
+// XCDs of an MI355X (workgroups are handed to them round-robin by index)
+constexpr int kXcds = 8;
+
 // rank of this lane among the lanes set in m
 __device__ __forceinline__ int lane_rank(unsigned long long m) {
     return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

@@ -2105,7 +2105,6 @@ hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hi
 // neighbouring columns, whose propagated leaves overlap, share an L2.
 constexpr int kSweepCap = 64 + 4 * kWave;   // K <= 64 live + four wavefronts of appends
 constexpr int kSweepTab = 256;              // chunk table entries (a window of a row's chunks)
-constexpr int kXcds = 8;
 constexpr int kRowBatch = 16;             // rows whose results are stored together
 
 template <int DP, int UNR>

@@ -5,8 +5,9 @@
 // of x - 128 (k_center_bf16), which BF16 holds exactly, so staging a patch
 // is a copy.  For a query patch a' and a leaf point b' (centred), every
 // product is an integer of magnitude <= 16384 and every partial sum of a
-// dot product over D <= 432 elements stays below 2^23, so one FP32
-// accumulator holds a'.b' exactly whatever the MFMA's summation order, and
+// dot product over D <= 432 elements stays below 2^23, so an FP32
+// accumulator holds a'.b' exactly whatever the MFMA's summation order (two
+// accumulators, even and odd k-steps, sum below 2^24: also exact), and
 //     d2 = |a'|^2 + |b'|^2 - 2 a'.b' = |a - b|^2
 // is the exact integer squared distance.  The norms |x'|^2 are exact
 // integer sums per patch (k_patch_norms, a p x p box sum of per-pixel
@@ -285,7 +286,10 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_bf16(TreeView t, CImg ia, 
             fetch_tile(np);
         }
         if (wave_live) {
-            floatx16 acc = {};
+            // even and odd k-steps accumulate separately (two independent
+            // MFMA chains); each partial sum stays below 2^23 and their sum
+            // below 2^24, so both and the total are exact
+            floatx16 acc = {}, acc1 = {};
             const uint8_t *br = bt + h * kSlot + qc * 16;
             if constexpr (EXACT) {
                 // fragment reads run kAhead k-steps in front of the MFMAs
@@ -293,7 +297,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_bf16(TreeView t, CImg ia, 
 #pragma unroll
                 for (int s = 0; s < KSM; ++s) {
                     const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(br + s * (2 * kSlot));
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv, af[s], acc, 0, 0, 0);
+                    if (s & 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv, af[s], acc1, 0, 0, 0);
+                    else acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv, af[s], acc, 0, 0, 0);
                 }
                 __builtin_amdgcn_sched_group_barrier(0x100, kAhead, 0);   // LDS reads
 #pragma unroll
@@ -306,10 +311,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_bf16(TreeView t, CImg ia, 
                 for (int s = 0; s < KSM; ++s) {
                     if (s < pk.ks) {
                         const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(br + s * (2 * kSlot));
-                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv, af[s], acc, 0, 0, 0);
+                        if (s & 1) acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv, af[s], acc1, 0, 0, 0);
+                        else acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bv, af[s], acc, 0, 0, 0);
                     }
                 }
             }
+            acc += acc1;
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 // rows 8 j + 4 h + 0..3 of the tile
