@@ -27,7 +27,7 @@ EXPORTS = (
     "annf_exact_field_dev",
     "annf_field_mm_dev", "annf_mlp_forward_dev", "annf_mlp_triplet_grad_dev", "annf_mlp_adam_dev",
     "annf_set_encoder", "annf_encode_image_dev", "annf_pca_cov_dev", "annf_mlp_triplet_grad_noalloc_dev",
-    "annf_pca_fit_dev", "annf_project_dev", "annf_field_openclopt_dev", "annf_openclopt_tree_export",
+    "annf_pca_fit_dev", "annf_patch_fp8_dev", "annf_project_dev", "annf_field_openclopt_dev", "annf_openclopt_tree_export",
 )
 
 PROJ_PCA32 = 0
@@ -92,6 +92,7 @@ def _declare(L):
     L.annf_set_encoder.argtypes = [_vp, _vp, i, i, i, i]
     L.annf_encode_image_dev.argtypes = [_vp, _vp, i, i, i, _vp]
     L.annf_pca_cov_dev.argtypes = [_vp, _vp, i, i, _vp, i, _vp, i, i, _vp, i, i, _vp, _vp]
+    L.annf_patch_fp8_dev.argtypes = [_vp, _vp, i, i, i, _vp, _vp, _vp]
     L.annf_project_dev.argtypes = [_vp, _vp, i, i, i, _vp, _vp, i, i, _vp]
     L.annf_field_openclopt_dev.argtypes = [_vp, _vp, i, i, _vp, i, i, i, _vp, i, i, _vp, i, i, i, _vp, _vp, _vp]
     L.annf_openclopt_tree_export.argtypes = [_vp, _ip, _ip, _fp, _ip, _ip, _ip]

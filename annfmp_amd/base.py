@@ -320,13 +320,38 @@ def field_on_device(image_a, image_b, psize, components, mean, tree_depth, n_nei
     return out
 
 
+def fp8_patch_bytes(psize):
+    """Bytes per patch of the FP8 codes (rows of ceil(3p / 4) dwords, padded
+    to a multiple of 64)."""
+    rw = (3 * psize + 3) // 4
+    return (psize * rw * 4 + 63) // 64 * 64
+
+
+def patch_fp8_on_device(image, psize, device=0, handle=None):
+    """The FP8 leaf pass's operands of every patch of a host image
+    (annf_patch_fp8_dev): (codes n x DQ uint8, norms, inverse scales)."""
+    torch = dev.torch()
+    h = handle or _lib.Handle(device)
+    cuda = dev.cuda_device(device)
+    H, W = image.shape[:2]
+    n = (H - psize + 1) * (W - psize + 1)
+    ti = torch.from_numpy(numpy.ascontiguousarray(image, dtype=numpy.uint8)).to(cuda)
+    codes = torch.empty((n, fp8_patch_bytes(psize)), dtype=torch.uint8, device=cuda)
+    nrm = torch.empty(n, dtype=torch.float32, device=cuda)
+    inv = torch.empty(n, dtype=torch.float32, device=cuda)
+    _lib.check("annf_patch_fp8_dev", _lib.lib().annf_patch_fp8_dev(
+        h.h, dev.dptr(ti), H, W, int(psize), dev.dptr(codes), dev.dptr(nrm), dev.dptr(inv)))
+    return codes.cpu().numpy(), nrm.cpu().numpy(), inv.cpu().numpy()
+
+
 def field_mm_on_device(image_a, image_b, psize, components, mean, tree_depth, n_neighbors, nprobe=1,
                        precision="bf16", shortlist=20, projection="fp32", propagate=False, device=0, handle=None,
                        return_reduced=False, timing=True):
     """One call of annf_field_mm_dev (full-patch leaf k-NN on MFMA) on host
     images; returns numpy results like field_on_device.
 
-    precision: "bf16" (exact distances) or "fp8" (shortlist re-ranked
+    precision: "bf16" (exact distances) or "fp8" (per query, the shortlist
+    smallest approximate distances over its scanned leaves, re-ranked
     exactly); nprobe: leaves per query (a power of two); propagate: False,
     True (all K entries of the list above propagate) or a count, from the
     first-pass lists of the row above (one parallel pass), or "chain" (all K,

@@ -19,7 +19,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 SRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libannf_hip.so")
-SOURCES = ["annf_kernels.hip", "annf_eval.hip", "annf_leafmm.hip", "annf_leafbf16.hip", "annf_mlp.hip",
+SOURCES = ["annf_kernels.hip", "annf_eval.hip", "annf_propagate.hip", "annf_leafbf16.hip", "annf_leaffp8.hip", "annf_mlp.hip",
            "annf_sort.hip", "annf_ocl.hip", "annf_abi.hip"]
 ARCH = os.environ.get("ANNF_OFFLOAD_ARCH", "gfx950")
 

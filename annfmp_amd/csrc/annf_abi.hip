@@ -120,6 +120,9 @@ struct annf_ctx {
     DevBuf sort_keys, sort_tmp, mm_blkleaf, mm_ptorder, proj_pack;
     // exact BF16 leaf k-NN: centred BF16 images, per-patch norms
     DevBuf cimg_a, cimg_b, pnorm_a, pnorm_b, pnorm_tmp;
+    // FP8 leaf k-NN: per-patch codes, norms and inverse scales of both
+    // images, the per-(slot, leaf) approximate lists
+    DevBuf f8_qa, f8_qb, f8_na, f8_nb, f8_ia, f8_ib, f8_keys;
     bool has_encoder = false;
     int enc_p = 0, enc_d = 0;
     // this device's copy of the allocation-free entry point's workspace
@@ -166,7 +169,8 @@ struct annf_ctx {
                           &ocl_keys2, &ocl_dims, &ocl_clanc, &ocl_vals, &ocl_leafpts, &ocl_o2l, &ocl_sort, &ocl_nat,
                           &ocl_pos, &ocl_dst,
                           &sort_keys, &sort_tmp, &mm_blkleaf, &mm_ptorder, &proj_pack, &cimg_a, &cimg_b,
-                          &pnorm_a, &pnorm_b, &pnorm_tmp};
+                          &pnorm_a, &pnorm_b, &pnorm_tmp, &f8_qa, &f8_qb, &f8_na, &f8_nb, &f8_ia, &f8_ib,
+                          &f8_keys};
         for (DevBuf *b : list) b->release();
         bnd_key = -1;
         has_tree = false;
@@ -898,7 +902,8 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                       int n_neighbors, int nprobe, int precision, int shortlist, int propagate, int *nn_out,
                       float *dist_out, int *knn_out, float *red_a_out, float *red_b_out) {
     if (!nn_out) return fail(ANNF_EINVAL, "null required pointer");
-    if (!leafmm_supported(psize)) return fail(ANNF_EUNSUPPORTED, "patch size %d: padded patch above 512 bytes", psize);
+    if (!leafbf16_supported(psize) || !leaffp8_supported(psize))
+        return fail(ANNF_EUNSUPPORTED, "patch size %d: the full-patch leaf passes take 1..12", psize);
     if (precision != ANNF_PREC_BF16 && precision != ANNF_PREC_FP8)
         return fail(ANNF_EINVAL, "precision %d is not ANNF_PREC_BF16 or ANNF_PREC_FP8", precision);
     const int K = n_neighbors;
@@ -907,6 +912,8 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         return fail(ANNF_EUNSUPPORTED, "shortlist %d outside [n_neighbors, 32]", shortlist);
     if (tree_depth < 0 || tree_depth > 30 || nprobe < 1 || nprobe > (1 << tree_depth) || (nprobe & (nprobe - 1)))
         return fail(ANNF_EINVAL, "nprobe %d is not a power of two in [1, 2^tree_depth]", nprobe);
+    if (precision == ANNF_PREC_FP8 && nprobe > 32)
+        return fail(ANNF_EUNSUPPORTED, "nprobe %d: the FP8 shortlist merges at most 32 probed leaves", nprobe);
     if (propagate != 0 && wa != wb)
         return fail(ANNF_EINVAL, "propagation needs images of equal width (base.c:271)");
     // propagate < 0: the reference's row chain (rows in order, each from the
@@ -935,9 +942,36 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
         HIPCHK(patch_norms_centered(img_b, hb, wb, psize, c->pnorm_tmp.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), st));
         return ANNF_OK;
     };
+    // the FP8 pass reads both images as per-patch E4M3 codes: made on the
+    // side stream while the tree is built
+    auto prepare_fp8 = [&](hipStream_t st) -> int {
+        const long long na_ = (long long)(ha - psize + 1) * (wa - psize + 1);
+        const long long nb_ = (long long)(hb - psize + 1) * (wb - psize + 1);
+        const size_t dq = (size_t)leaffp8_dq(psize);
+        RCHK(c->f8_qa.ensure((size_t)na_ * dq));
+        RCHK(c->f8_qb.ensure((size_t)nb_ * dq));
+        RCHK(c->f8_na.ensure((size_t)na_ * sizeof(float)));
+        RCHK(c->f8_ia.ensure((size_t)na_ * sizeof(float)));
+        RCHK(c->f8_nb.ensure((size_t)nb_ * sizeof(float)));
+        RCHK(c->f8_ib.ensure((size_t)nb_ * sizeof(float)));
+        HIPCHK(patch_fp8(img_a, ha, wa, psize, c->f8_qa.as<uint8_t>(), c->f8_na.as<float>(), c->f8_ia.as<float>(),
+                         st));
+        HIPCHK(patch_fp8(img_b, hb, wb, psize, c->f8_qb.as<uint8_t>(), c->f8_nb.as<float>(), c->f8_ib.as<float>(),
+                         st));
+        // centred patch norms for the exact re-rank
+        RCHK(c->pnorm_a.ensure((size_t)na_ * sizeof(uint32_t)));
+        RCHK(c->pnorm_b.ensure((size_t)nb_ * sizeof(uint32_t)));
+        RCHK(c->pnorm_tmp.ensure((size_t)std::max((long long)ha * (wa - psize + 1), (long long)hb * (wb - psize + 1)) *
+                                 sizeof(uint32_t)));
+        HIPCHK(patch_norms_centered(img_a, ha, wa, psize, c->pnorm_tmp.as<uint32_t>(), c->pnorm_a.as<uint32_t>(), st));
+        HIPCHK(patch_norms_centered(img_b, hb, wb, psize, c->pnorm_tmp.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), st));
+        return ANNF_OK;
+    };
     FieldShape fs;
     RCHK(project_and_build(c, img_a, ha, wa, img_b, hb, wb, psize, basis, mean, d, projection, tree_depth, K,
-                           red_a_out, red_b_out, fs, fp8 ? std::function<int(hipStream_t)>() : prepare_bf16,
+                           red_a_out, red_b_out, fs,
+                           fp8 ? std::function<int(hipStream_t)>(prepare_fp8)
+                               : std::function<int(hipStream_t)>(prepare_bf16),
                            false));
     const TreeView &t = c->tree;
     const int nleaves = 1 << t.depth;
@@ -960,20 +994,38 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     const size_t sort_bytes = sort_scratch_bytes((int)n_sort, nleaves);
     RCHK(c->sort_tmp.ensure(sort_bytes));
     int *skeys = c->sort_keys.as<int>();
-    RCHK(c->mm_blkleaf.ensure(fp8 ? leafmm_scratch_bytes((int)n_sort, nleaves)
-                                  : leafbf16_scratch_bytes((int)n_sort, nleaves)));
+    RCHK(c->mm_blkleaf.ensure(leafbf16_scratch_bytes((int)n_sort, nleaves)));
     void *blk_info = c->mm_blkleaf.p;
-    const int chunk = fp8 ? leafmm_chunk() : leafbf16_chunk();
-    const int *pts_order = t.orig;
-    if (!fp8) {
-        RCHK(c->mm_ptorder.ensure((size_t)fs.n_b * sizeof(int)));
-        HIPCHK(leaf_point_order(t, c->mm_ptorder.as<int>(), c->stream));
-        pts_order = c->mm_ptorder.as<int>();
+    const int chunk = leafbf16_chunk();
+    RCHK(c->mm_ptorder.ensure((size_t)fs.n_b * sizeof(int)));
+    HIPCHK(leaf_point_order(t, c->mm_ptorder.as<int>(), c->stream));
+    const int *pts_order = c->mm_ptorder.as<int>();
+    // FP8: position bits of the approximate keys (the largest leaf) and the
+    // per-(slot, leaf) lists
+    int pb = 1;
+    while (pb < 24 && (1 << pb) < c->bnd_max_leaf) ++pb;
+    if (fp8 && c->bnd_max_leaf > (1 << pb))
+        return fail(ANNF_EUNSUPPORTED, "a leaf of %d points: the FP8 keys hold positions below 2^24", c->bnd_max_leaf);
+    const int sp8 = leaffp8_stride(shortlist);
+    if (fp8) {
+        RCHK(c->f8_keys.ensure(std::max((size_t)n_a * nprobe, n_pairs) * sp8 * sizeof(uint64_t)));
     }
+    // FP8 leaf pass of nq bucketed slots (slot s: A patch a0 + s / sdiv)
+    auto fp8_pass = [&](int nq, int np_, int sdiv) -> int {
+        HIPCHK(leaf_fp8(t, c->f8_qa.as<uint8_t>(), c->f8_na.as<float>(), c->f8_ia.as<float>(), c->f8_qb.as<uint8_t>(),
+                        c->f8_nb.as<float>(), c->f8_ib.as<float>(), psize, pts_order, c->order.as<int>(),
+                        c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, nq, np_, shortlist, sdiv, 0, pb,
+                        c->f8_keys.as<uint64_t>(), c->stream));
+        return ANNF_OK;
+    };
     auto leaf_pass = [&](int nq, int np_, int sdiv, uint64_t *keys, int *knn, int *nn, float *dist) -> int {
         if (fp8) {
-            HIPCHK(leaf_mm(t, fs.ga, fs.gb, pts_order, c->order.as<int>(), c->boffset.as<int>(), c->bchunk.as<int>(),
-                           blk_info, nq, np_, shortlist, K, fp8, sdiv, keys, knn, nn, dist, c->stream));
+            // approximate lists, each query's shortlist, its exact re-rank
+            RCHK(fp8_pass(nq, np_, sdiv));
+            // queries in bucket order: a leaf's queries share its points
+            HIPCHK(fp8_select(c->f8_keys.as<uint64_t>(), shortlist, 0, n_a, c->order.as<int>(), np_, nullptr,
+                              fs.ga, fs.gb, c->pnorm_a.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), nullptr, K, keys,
+                              knn, nn, dist, c->stream));
         } else {
             HIPCHK(leaf_bf16(t, c->cimg_a.as<uint16_t>(), wa, c->cimg_b.as<uint16_t>(), wb, psize,
                              c->pnorm_a.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), pts_order, c->order.as<int>(),
@@ -993,8 +1045,10 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
     if (prop) {
         // the propagated leaves, bucketed like the queries, scanned by the same kernel
         RCHK(c->mm_pairleaf.ensure(n_pairs * sizeof(int)));
-        RCHK(c->mm_keys2.ensure(n_pairs * K * sizeof(uint64_t)));
-        keys2 = c->mm_keys2.as<uint64_t>();
+        if (!fp8) {
+            RCHK(c->mm_keys2.ensure(n_pairs * K * sizeof(uint64_t)));
+            keys2 = c->mm_keys2.as<uint64_t>();
+        }
         // pairs of queries q0 .. q0 + nq - 1 from `above` (keys by query),
         // scanned into keys2
         auto pair_pass = [&](const uint64_t *above, int q0, int nq) -> int {
@@ -1003,15 +1057,26 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                               c->mm_pairleaf.as<int>(), nullptr, c->stream));
             HIPCHK(bucket_sorted(c->mm_pairleaf.as<int>(), (int)np, nleaves, chunk, skeys, c->order.as<int>(),
                                  c->boffset.as<int>(), c->bchunk.as<int>(), c->sort_tmp.p, sort_bytes, c->stream));
+            if (fp8) {
+                // the pairs' approximate lists, then one shortlist and exact
+                // re-rank per query, merged with its first-pass keys
+                RCHK(fp8_pass((int)np, 1, prop_k));
+                HIPCHK(fp8_select(c->f8_keys.as<uint64_t>(), shortlist, q0, nq, nullptr, prop_k,
+                                  c->mm_pairleaf.as<int>(), fs.ga, fs.gb, c->pnorm_a.as<uint32_t>(),
+                                  c->pnorm_b.as<uint32_t>(), c->mm_keys1.as<uint64_t>(), K, nullptr, knn_out,
+                                  nn_out, dist_out, c->stream));
+                return ANNF_OK;
+            }
             HIPCHK(fill_keys(keys2, (long long)np * K, c->stream));
-            if (fp8) return leaf_pass((int)np, 1, prop_k, keys2, nullptr, nullptr, nullptr);
             HIPCHK(leaf_bf16(t, c->cimg_a.as<uint16_t>(), wa, c->cimg_b.as<uint16_t>(), wb, psize,
                              c->pnorm_a.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), pts_order, c->order.as<int>(),
                              c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, (int)np, 1, K, prop_k, keys2,
                              nullptr, nullptr, nullptr, c->stream, ha, hb, q0));
             return ANNF_OK;
         };
-        if (!chain) {
+        if (!chain && fp8) {
+            RCHK(pair_pass(c->mm_keys1.as<uint64_t>(), 0, n_a));
+        } else if (!chain) {
             RCHK(pair_pass(c->mm_keys1.as<uint64_t>(), 0, n_a));
             HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, 0, n_a, K, prop_k, knn_out, nn_out, dist_out,
                                 nullptr, c->stream));
@@ -1211,6 +1276,26 @@ int annf_pca_fit_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const in
     RCHK(sync_stream(c));
     if (hinfo != 0) return fail(ANNF_EHIP, "rocsolver_dsyevd did not converge (info %d)", (int)hinfo);
     return ANNF_OK;
+}
+
+int annf_patch_fp8_dev(annf_ctx *c, const uint8_t *img, int h, int w, int psize, uint8_t *codes, float *norm,
+                       float *inv) {
+    if (!c) return fail(ANNF_EINVAL, "null handle");
+    if (!img) return fail(ANNF_EINVAL, "null image");
+    const int p = psize;
+    if (p < 1 || h < p || w < p) return fail(ANNF_EINVAL, "patch size %d larger than the image", p);
+    if (!leaffp8_supported(p)) return fail(ANNF_EUNSUPPORTED, "patch size %d: the FP8 codes take 1..12", p);
+    RCHK(set_device(c));
+    const long long n = (long long)(h - p + 1) * (w - p + 1);
+    const size_t dq = (size_t)leaffp8_dq(p);
+    RCHK(c->f8_qa.ensure((size_t)n * dq));
+    RCHK(c->f8_na.ensure((size_t)n * sizeof(float)));
+    RCHK(c->f8_ia.ensure((size_t)n * sizeof(float)));
+    HIPCHK(patch_fp8(img, h, w, p, c->f8_qa.as<uint8_t>(), c->f8_na.as<float>(), c->f8_ia.as<float>(), c->stream));
+    if (codes) HIPCHK(hipMemcpyAsync(codes, c->f8_qa.p, (size_t)n * dq, hipMemcpyDeviceToDevice, c->stream));
+    if (norm) HIPCHK(hipMemcpyAsync(norm, c->f8_na.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    if (inv) HIPCHK(hipMemcpyAsync(inv, c->f8_ia.p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+    return sync_stream(c);
 }
 
 int annf_project_dev(annf_ctx *c, const uint8_t *img, int h, int w, int psize, const void *basis,

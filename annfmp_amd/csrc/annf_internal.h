@@ -136,29 +136,21 @@ hipError_t project_patches(const uint8_t *img, int h, int w, int p, const float 
 size_t projection_pack_floats(int p, int dpad);
 hipError_t pack_projection(const float *basis_t, const float *mean_f, int p, int dpad, float *pack, hipStream_t st);
 
-// full-patch leaf k-NN on MFMA (annf_leafmm.hip): queries bucketed with
-// chunks of leafmm_chunk(); BF16 (exact distances) or FP8 (shortlist of
-// `shortlist` re-ranked exactly); nprobe leaves leaf ^ 0 .. leaf ^ (nprobe-1)
-int leafmm_chunk();
-int leafmm_dk(int p);
-bool leafmm_supported(int p);
-// order holds output slots; slot s is A patch s / slot_div and receives its
-// K exact keys ((d2 << 32) | B index, ascending) in keys_out[s * K ...], or,
-// with nn given, the field itself (knn, nn, dist as finish_field writes them).
-// scratch: leafmm_scratch_bytes(nq, nleaves) bytes (each block's work item).
-// pts_order: every leaf's points in ascending B index (leaf_point_order);
-// the BF16 form needs it, the FP8 form uses the tree order.
-size_t leafmm_scratch_bytes(int nq, int nleaves);
+// every leaf's points in ascending B index (the position order of the
+// full-patch leaf passes)
 hipError_t leaf_point_order(const TreeView &t, int *out, hipStream_t st);
-hipError_t leaf_mm(const TreeView &t, const PatchGeom &ga, const PatchGeom &gb, const int *pts_order, const int *order,
-                   const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int shortlist, int K,
-                   int fp8, int slot_div, uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st);
 // exact BF16 full-patch leaf k-NN (annf_leafbf16.hip): queries bucketed with
 // chunks of leafbf16_chunk(); images as two centred BF16 copies
 // (center_image, buffers of 2 * cimg_elems elements) with per-patch norms of
-// x - 128 (patch_norms_centered, tmp: h * (w - p + 1) words); pts_order as
-// for leaf_mm; scratch of leafbf16_scratch_bytes; output slots as leaf_mm.
+// x - 128 (patch_norms_centered, tmp: h * (w - p + 1) words); nprobe leaves
+// leaf ^ 0 .. leaf ^ (nprobe - 1); pts_order from leaf_point_order; scratch
+// of leafbf16_scratch_bytes (each block's work item).  order holds output
+// slots; slot s is A patch a0 + s / slot_div and receives its K exact keys
+// ((d2 << 32) | B index, ascending) in keys_out[s * K ...], or, with nn
+// given, the field itself (knn, nn, dist as finish_field writes them).
 int leafbf16_chunk();
+hipError_t leaf_blocks_q(const int *chunk_off, const int *offset, const int *leaves, int nleaves, int nblk,
+                         int4 *blk_info, hipStream_t st);
 bool leafbf16_supported(int p);
 size_t leafbf16_scratch_bytes(int nq, int nleaves);
 long long cimg_elems(int h, int w);
@@ -170,6 +162,30 @@ hipError_t leaf_bf16(const TreeView &t, const uint16_t *ca, int wa, const uint16
                      const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int K, int slot_div,
                      uint64_t *keys_out, int *knn, int *nn, float *dist, hipStream_t st, int ha, int hb,
                      int a0 = 0);
+// FP8 full-patch leaf k-NN (annf_leaffp8.hip): patch_fp8 writes an image's
+// per-patch E4M3 codes (leaffp8_dq(p) bytes per patch), norms and inverse
+// scales; leaf_fp8 scans the probed leaves of the bucketed slots (chunks of
+// leafbf16_chunk(), scratch of leafbf16_scratch_bytes; pb bits hold a
+// position in the largest leaf) and writes, for slot s and probe r, the S
+// smallest approximate keys (d~ bits rounded down << 32 | B index,
+// ascending) at keys[(s * nprobe + r) * leaffp8_stride(S)]; fp8_select
+// (queries in the order qorder, a permutation of q0 .. q0 + nq - 1, or
+// ascending) merges each query's nsrc lists (slots (q - q0) * nsrc + r;
+// with pairleaf, slots of leaf -1 are skipped) into its S smallest, scores
+// them exactly and keeps the K smallest with the query's base keys (may be
+// null); norm_a / norm_b: centred patch norms (patch_norms_centered).
+int leaffp8_dq(int p);
+bool leaffp8_supported(int p);
+int leaffp8_stride(int S);
+hipError_t patch_fp8(const uint8_t *img, int h, int w, int p, uint8_t *q, float *nrm, float *inv, hipStream_t st);
+hipError_t leaf_fp8(const TreeView &t, const uint8_t *qa, const float *na, const float *ia, const uint8_t *qb,
+                    const float *nb, const float *ib, int p, const int *pts_order, const int *order,
+                    const int *offset, const int *chunk_off, void *scratch, int nq, int nprobe, int S, int slot_div,
+                    int a0, int pb, uint64_t *keys, hipStream_t st);
+hipError_t fp8_select(const uint64_t *keys, int S, int q0, int nq, const int *qorder, int nsrc, const int *pairleaf,
+                      const PatchGeom &ga, const PatchGeom &gb, const uint32_t *norm_a, const uint32_t *norm_b,
+                      const uint64_t *base, int K, uint64_t *keys_out, int *knn, int *nn, float *dist,
+                      hipStream_t st);
 // propagated (query, leaf) pairs of the queries q0 .. q0 + nq - 1 from the
 // keys of the row above (keys1, by query): pairleaf[(q - q0) * prop_k + k]
 // (-1: none, or a leaf the first pass scanned: natleaf ^ r, r < nprobe),

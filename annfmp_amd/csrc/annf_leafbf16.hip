@@ -510,6 +510,13 @@ PatchPack make_pack(int p) {
 
 int leafbf16_chunk() { return kQ; }
 
+hipError_t leaf_blocks_q(const int *chunk_off, const int *offset, const int *leaves, int nleaves, int nblk,
+                         int4 *blk_info, hipStream_t st) {
+    hipLaunchKernelGGL(k_block_info_q, dim3((nblk + 255) / 256), dim3(256), 0, st, chunk_off, offset, leaves, nleaves,
+                       nblk, blk_info);
+    return hipGetLastError();
+}
+
 bool leafbf16_supported(int p) {
     if (p < 1 || p > 12) return false;   // images: fewer than 2^30 elements (checked by the caller)
     const PatchPack pk = make_pack(p);
@@ -554,8 +561,8 @@ hipError_t leaf_bf16(const TreeView &t, const uint16_t *ca, int wa, const uint16
     const PatchPack pk = make_pack(p);
     const unsigned nblk = (unsigned)(nleaves + (nq + kQ - 1) / kQ);
     int4 *blk_info = static_cast<int4 *>(scratch);
-    hipLaunchKernelGGL(k_block_info_q, dim3((nblk + 255) / 256), dim3(256), 0, st, chunk_off, offset, t.leaves,
-                       nleaves, (int)nblk, blk_info);
+    hipError_t e = leaf_blocks_q(chunk_off, offset, t.leaves, nleaves, (int)nblk, blk_info, st);
+    if (e != hipSuccess) return e;
     const CImg ia{ca, (uint32_t)cimg_elems(ha, wa), wa, wa - p + 1}, ib{cb, (uint32_t)cimg_elems(hb, wb), wb, wb - p + 1};
     const Out out{knn, nn, dist, keys_out};
     return with_ks_lm(pk.ks, K, [&](auto ks, auto lm) -> hipError_t {

@@ -47,6 +47,7 @@ MM_STAGES = ["project_b", "launch_project_a", "tree_build_with_project_a", "natu
              "propagated_pass"]
 DEFAULT_ENCODER = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p11_d16.npz")
 DEFAULT_ENCODER_4K = os.path.join(ROOT, "annfmp_amd", "data", "encoder_p9_d16.npz")
+PATHS_DEFAULT = "plain,pca,learned"
 FP8_PEAK_TFS = 5000.0          # MI355X dense FP8 MFMA, 5.0 PFLOP/s
 FP32_PEAK_TFS = 157.3          # MI355X FP32, 157.3 TFLOP/s
 
@@ -232,23 +233,32 @@ def run_4k(args):
             h.h, dev.dptr(ia), H, W, dev.dptr(ib), H, W, p, dev.dptr(tc), dev.dptr(tm), d, proj, depth, K, 1,
             prec, shortlist, propagate, dev.dptr(nn), None, dev.dptr(knn_out), None, None))
 
-    q = metrics.sample_queries(n_a, 8192)
-    exact = metrics.exact_knn(a, b, p, q, K, handle=h)
+    if not args.no_quality:
+        q = metrics.sample_queries(n_a, 8192)
+        exact = metrics.exact_knn(a, b, p, q, K, handle=h)
 
     def quality():
         torch.cuda.synchronize()
+        if args.no_quality:
+            return {"recall@8": float("nan"), "distance_ratio": float("nan")}
         return metrics.field_quality(a, b, p, nn.cpu().numpy(), knn.cpu().numpy(), k=K, handle=h, exact=exact)
 
+    # --paths other than the 1080p default selects runs by name (plain, learned_fp8_prop, ...)
+    select = None if args.paths == PATHS_DEFAULT else set(args.paths.split(","))
     paths = {}
-    ms, stage = timed(plain, args.steps, args.warmup, torch, _lib, h, 7)
-    plain(knn)
-    paths["plain_fp32"] = {"ms_per_step": round(ms, 3), "query_patches_per_s": round(n_a / ms * 1e3, 1),
-                           "stage_ms": {s_: round(float(v), 3) for s_, v in zip(PLAIN_STAGES, stage)}, **quality()}
+    if select is None or "plain" in select:
+        ms, stage = timed(plain, args.steps, args.warmup, torch, _lib, h, 7)
+        plain(knn)
+        paths["plain_fp32"] = {"ms_per_step": round(ms, 3), "query_patches_per_s": round(n_a / ms * 1e3, 1),
+                               "stage_ms": {s_: round(float(v), 3) for s_, v in zip(PLAIN_STAGES, stage)},
+                               **quality()}
     flops_pass1 = 2.0 * (p * p * 3) * n_a * (n_b / (1 << depth))
     runs = (("learned_fp8_prop", _lib.PREC_FP8, S, prop), ("learned_bf16_prop", _lib.PREC_BF16, K, prop),
             ("learned_bf16_prop2", _lib.PREC_BF16, K, 2), ("learned_fp8", _lib.PREC_FP8, S, 0),
             ("learned_bf16", _lib.PREC_BF16, K, 0))
     for name, prec, sl, pr in runs:
+        if select is not None and name not in select:
+            continue
         ms, stage = timed(lambda: mm(prec, sl, pr), args.steps, args.warmup, torch, _lib, h, 6)
         mm(prec, sl, pr, knn)
         peak = FP8_PEAK_TFS if prec == _lib.PREC_FP8 else BF16_PEAK_TFS
@@ -260,6 +270,9 @@ def run_4k(args):
                        "leaf_pass_mfma_tflops": round(flops_pass1 / (leaf_ms * 1e-3) / 1e12, 2),
                        "leaf_pass_mfma_util": round(flops_pass1 / (leaf_ms * 1e-3) / 1e12 / peak, 5),
                        **quality()}
+    if "learned_fp8_prop" not in paths or "plain_fp32" not in paths:
+        print(json.dumps({"paths": paths}), flush=True)
+        return
     main_p = paths["learned_fp8_prop"]
     out = {
         "metric": "query patches/sec, 4K pair: learned+FP8 (top-32 re-rank, propagation) vs plain FP32, recall@8",
@@ -286,7 +299,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--nprobe", type=int, default=1, help="leaves probed per query on the BF16 path")
     ap.add_argument("--encoder", default=DEFAULT_ENCODER, help="trained encoder (.npz); 'none' forces PCA")
-    ap.add_argument("--paths", default="plain,pca,learned",
+    ap.add_argument("--paths", default=PATHS_DEFAULT,
                     help="paths to run (profiling runs select one); the value needs pca and learned; "
                          "learned_prop / learned_chain add the learned BF16 form with the parallel propagated "
                          "pass / the row chain")

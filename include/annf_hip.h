@@ -156,10 +156,12 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
  * |a|^2 + |b|^2 - 2 a.b on MFMA with FP32 accumulation:
  *   ANNF_PREC_BF16: BF16 operands; the distances are exact integers, the
  *     K list is the K smallest (distance, B index);
- *   ANNF_PREC_FP8:  FP8-E4M3 operands, patches centred at 128 and scaled
- *     per patch by 448 / max|x - 128|; the `shortlist` smallest approximate
- *     distances (n_neighbors <= shortlist <= 32) are re-ranked with exact
- *     distances into the K list.
+ *   ANNF_PREC_FP8:  FP8-E4M3 operands (annf_patch_fp8_dev's codes, on
+ *     v_mfma_f32_32x32x64_f8f6f4); each scanned (patch, leaf) keeps its
+ *     `shortlist` smallest approximate distances (n_neighbors <= shortlist
+ *     <= 32, nprobe <= 32), each A patch's `shortlist` smallest over its
+ *     scanned leaves are re-ranked with exact integer distances into the K
+ *     list.
  * propagate = m > 0 adds the reference's propagated leaves (base.c:255-282)
  * as a second, parallel pass: for a patch below the first row, the leaves of
  * the B patches one row below the first min(m, K) first-pass matches of the
@@ -169,7 +171,9 @@ int annf_field_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
  * first pass of the row above, not from its propagated result (one parallel
  * pass); propagate = -m < 0 (ANNF_PREC_BF16 only) is the reference's row
  * chain instead: rows in order, each taking its candidates from the final
- * lists of the row above.  Either needs wa == wb.  A propagated leaf
+ * lists of the row above.  With ANNF_PREC_FP8 the propagated leaves' lists
+ * form one shortlist per A patch, re-ranked exactly and merged with its
+ * first-pass list.  Either needs wa == wb.  A propagated leaf
  * the first pass already scanned (natural leaf ^ r, r < nprobe) is left
  * out, so no B index is listed twice.
  * Outputs as annf_field_dev: nn_out is the first list entry, dist_out its
@@ -191,6 +195,19 @@ int annf_field_mm_dev(annf_ctx *ctx, const uint8_t *img_a, int ha, int wa,
                       int nprobe, int precision, int shortlist, int propagate,
                       int *nn_out, float *dist_out, int *knn_out,
                       float *red_a_out, float *red_b_out);
+
+/* The FP8 operands of annf_field_mm_dev's ANNF_PREC_FP8 form (no
+ * counterpart in the reference; the codes its leaf pass reads): for every p x p
+ * patch i of a device image (h x w x 3 uint8, psize <= 12), m = max |x - 128|
+ * over the patch, scale s = 448 / m (1 when m = 0), and codes[i * DQ + r * RW
+ * + k] the E4M3 codes (round to nearest even) of (x - 128) * s for bytes
+ * 4k .. 4k + 3 of patch row r (RW = ceil(3p / 4) dwords per row, zero codes
+ * past the row and up to DQ = 64 * ceil(p * RW * 4 / 64) bytes per patch);
+ * inv[i] = m / 448 (1 when m = 0); norm[i] = (sum of the squared code values)
+ * * inv[i]^2 (the sum exact, the product in FP64, rounded to FP32).  Any
+ * output may be null. */
+int annf_patch_fp8_dev(annf_ctx *ctx, const uint8_t *img, int h, int w,
+                       int psize, uint8_t *codes, float *norm, float *inv);
 
 /* Projection alone (openmp/base.py:193-200's transform, or the handle's
  * encoder): every p x p patch of a device image (h x w x 3 uint8) into out

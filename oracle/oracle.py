@@ -26,6 +26,12 @@ Plus numpy restatements of the Python-side stages:
 * ``leaf_knn_propagated`` (and ``_rows`` for a sample of the queries) -- the
                           same over the probed leaves plus the leaves the row
                           above's matches propagate (base.c:255-282)
+* ``e4m3_rne`` / ``patch_fp8`` -- the FP8 operands of the FP8 full-patch
+                          variant (no reference counterpart): E4M3 (the OCP
+                          8-bit float format, 4 exponent bits of bias 7, 3
+                          mantissa bits, largest 448) by round to nearest
+                          even over the enumerated code table; pinned by
+                          known codes, ties and round trips in the tests
 
 and the openclopt backend's semantics (``ocl_oracle.c``, restating
 annfmp/openclopt/src/futhark/*.fut): ``ocl_tree_shape``, ``OclTree``,
@@ -97,6 +103,61 @@ def project_f64(patches, components, mean):
         x = patches[s:s + step].astype(np.float64)
         out[s:s + step] = (x @ comp.T - off).astype(np.float32)
     return out
+
+
+def _e4m3_table():
+    """Values of the non-negative E4M3 codes 0..126 (127 is NaN), ascending."""
+    c = np.arange(127)
+    e, m = c >> 3, c & 7
+    return np.where(e == 0, m * 2.0 ** -9, (1.0 + m / 8.0) * 2.0 ** (e.astype(np.float64) - 7))
+
+
+def e4m3_rne(v):
+    """E4M3 codes of float32 values |v| <= 448 (and a rounding's worth above),
+    round to nearest, ties to the even code; +0 for zero."""
+    v = np.asarray(v, np.float32)
+    tab = _e4m3_table()
+    a = np.abs(v).astype(np.float64)
+    hi = np.clip(np.searchsorted(tab, a), 0, 126)
+    lo = np.clip(hi - 1, 0, 126)
+    dlo, dhi = a - tab[lo], tab[hi] - a
+    pick = np.where(dlo < dhi, lo, np.where(dhi < dlo, hi, np.where(lo % 2 == 0, lo, hi)))
+    pick = np.where(a >= tab[126], 126, pick)
+    return (pick | np.where((v < 0) & (pick > 0), 0x80, 0)).astype(np.uint8)
+
+
+def e4m3_value(code):
+    """float64 value of E4M3 codes (no NaN codes expected)."""
+    code = np.asarray(code, np.int64)
+    return np.where(code & 0x80, -1.0, 1.0) * _e4m3_table()[np.minimum(code & 0x7F, 126)]
+
+
+def patch_fp8(img, p):
+    """Per-patch FP8 codes, norms and inverse scales of every p x p patch:
+    codes (n x DQ uint8; row r of the patch at bytes 4 RW r .. with RW =
+    ceil(3p / 4) dwords, zero past the row and up to DQ = 64 ceil(4 p RW /
+    64)), inv = m / 448 and norm = (sum of squared code values) * inv^2
+    (the sum exact in float64, the product in float64, then float32), with
+    m = max |x - 128| over the patch (inv = 1 and scale 1 when m = 0)."""
+    pa = extract_patches(img, p).astype(np.int64)           # n x 3 p^2, feature (r * p + c) * 3 + ch
+    n = pa.shape[0]
+    rl = 3 * p
+    rw = (rl + 3) // 4
+    dq = (p * rw * 4 + 63) // 64 * 64
+    x = pa.reshape(n, p, rl) - 128
+    m = np.abs(x).reshape(n, -1).max(axis=1)
+    m32 = m.astype(np.float32)
+    scale = np.where(m > 0, np.float32(448.0) / np.maximum(m32, np.float32(1)), np.float32(1)).astype(np.float32)
+    inv = np.where(m > 0, m32 / np.float32(448.0), np.float32(1)).astype(np.float32)
+    v = (x.astype(np.float32) * scale[:, None, None]).astype(np.float32)
+    codes_rows = e4m3_rne(v)                                  # n x p x 3p
+    codes = np.zeros((n, dq), np.uint8)
+    for r in range(p):
+        codes[:, 4 * rw * r:4 * rw * r + rl] = codes_rows[:, r, :]
+    vals = e4m3_value(codes_rows).reshape(n, -1)
+    ssum = (vals * vals).sum(axis=1)
+    norm = (ssum * (inv.astype(np.float64) * inv.astype(np.float64))).astype(np.float32)
+    return codes, norm, inv
 
 
 def exact_field(pa, pb, k, chunk=2048):

@@ -14,7 +14,7 @@ from oracle import oracle as O
 def test_library_exports_every_declared_symbol():
     from annfmp_amd import _lib
     header = open(os.path.join(ROOT, "include", "annf_hip.h")).read()
-    declared = sorted(set(re.findall(r"\b(annf_[a-z_]+)\s*\(", header)))
+    declared = sorted(set(re.findall(r"\b(annf_[a-z0-9_]+)\s*\(", header)))
     assert set(declared) == set(_lib.EXPORTS)
     L = _lib.lib()
     for name in declared:

@@ -336,3 +336,47 @@ def test_leaf_knn_propagated_rows_equals_the_full_pass(nprobe, prop_k):
     si, sd = O.leaf_knn_propagated_rows(pa, pb, qleaf, leaves, orig, pli, n_cols, k, rows, nprobe=nprobe,
                                         prop_k=prop_k)
     assert np.array_equal(si, fi[rows]) and np.array_equal(sd, fd[rows])
+
+
+def test_e4m3_rounding_known_codes():
+    """The FP8 operands' E4M3 codes (OCP 8-bit float, bias 7, 3 mantissa
+    bits, largest 448): known codes, ties to the even code, subnormals, and
+    every code's value rounds back to itself."""
+    v = np.float32([0.0, 1.0, 448.0, 1.0625, 1.1875, -3.5, 2.0 ** -9, 2.0 ** -10, 3 * 2.0 ** -11, 240.0,
+                    447.99, 0.875, 17.0, 18.0])
+    want = [0x00, 0x38, 0x7E, 0x38, 0x3A, 0x80 | 0x46, 0x01, 0x00, 0x01, 0x77, 0x7E, 0x36, 0x58, 0x59]
+    assert O.e4m3_rne(v).tolist() == want
+    codes = np.arange(127)
+    vals = O.e4m3_value(codes).astype(np.float32)
+    assert np.array_equal(O.e4m3_rne(vals), codes.astype(np.uint8))
+    assert np.array_equal(O.e4m3_rne(-vals[1:]), (codes[1:] | 0x80).astype(np.uint8))
+    # midpoints go to the even neighbour, points just off them to the nearer one
+    mid = ((vals[1:-1].astype(np.float64) + vals[2:]) / 2).astype(np.float32)
+    got = O.e4m3_rne(mid).astype(np.int64)
+    lo = codes[1:-1]
+    assert np.array_equal(got, np.where(lo % 2 == 0, lo, lo + 1))
+    assert np.array_equal(O.e4m3_rne(np.nextafter(mid, np.float32(0))), lo.astype(np.uint8))
+
+
+def test_patch_fp8_definition():
+    """oracle.patch_fp8 on a hand-made image: scale from the patch's largest
+    |x - 128|, rows padded to whole dwords, flat patches coded as zeros with
+    inverse scale 1, the norm of the de-scaled codes."""
+    img = np.full((4, 5, 3), 128, np.uint8)
+    img[0, 0] = (128 + 64, 128 - 32, 128 + 1)
+    img[3, 4] = (0, 255, 128)
+    codes, norm, inv = O.patch_fp8(img, 2)
+    assert codes.shape == (12, 64)              # 6-byte rows -> 2 dwords, 2 rows -> 16 bytes -> 64
+    # patch 0 holds img[0:2, 0:2]: m = 64, scale 7, values 448, -224, 7
+    assert inv[0] == np.float32(64 / 448)
+    assert codes[0, :3].tolist() == [0x7E, O.e4m3_rne(np.float32(-224.0)).item(), O.e4m3_rne(np.float32(7.0)).item()]
+    assert (codes[0, 3:].astype(int) == 0).all()
+    want = (448.0 ** 2 + 224.0 ** 2 + 7.0 ** 2) * (np.float64(inv[0]) ** 2)
+    assert norm[0] == np.float32(want)
+    # a flat patch
+    assert inv[5] == 1.0 and norm[5] == 0.0 and (codes[5] == 0).all()
+    # the last patch holds img[2:4, 3:5]: m = 128 (the 0 byte), scale 3.5
+    last = codes[-1]
+    assert inv[-1] == np.float32(128 / 448)
+    assert last[8 + 3:8 + 6].tolist() == [O.e4m3_rne(np.float32(-448.0)).item(),
+                                         O.e4m3_rne(np.float32(127 * 3.5)).item(), 0]
