@@ -1108,23 +1108,27 @@ int annf_exact_field_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, cons
     const int p = psize;
     if (p < 1 || ha < p || wa < p || hb < p || wb < p) return fail(ANNF_EINVAL, "patch size %d larger than an image", p);
     if (k < 1 || k > 32) return fail(ANNF_EUNSUPPORTED, "k %d outside [1, 32]", k);
-    const int D = p * p * 3, dpad_b = (D + 3) & ~3;
-    if (dpad_b / 4 > 128) return fail(ANNF_EUNSUPPORTED, "patch of %d bytes: at most 512 supported", D);
+    const int D = p * p * 3, dq = exact_dq(p);
+    if (dq == 0) return fail(ANNF_EUNSUPPORTED, "patch of %d bytes: at most 512 supported", D);
     const long long n_a = (long long)(ha - p + 1) * (wa - p + 1), n_b = (long long)(hb - p + 1) * (wb - p + 1);
     if (n_b > 0x7FFFFFFFLL || n_a > 0x7FFFFFFFLL) return fail(ANNF_EINVAL, "too many patches");
     if ((long long)k > n_b) return fail(ANNF_EINVAL, "k %d exceeds the %lld B patches", k, n_b);
-    RCHK(c->ev_am.ensure((size_t)n_a * dpad_b));
-    RCHK(c->ev_an.ensure((size_t)n_a * sizeof(uint32_t)));
-    RCHK(c->ev_bm.ensure((size_t)n_b * dpad_b));
+    // the sampled A patches' rows, every B patch's row, their norms
+    const size_t nqa = (size_t)(nq > 0 ? nq : 1);
+    RCHK(c->ev_am.ensure(nqa * dq));
+    RCHK(c->ev_an.ensure(nqa * sizeof(uint32_t)));
+    RCHK(c->ev_bm.ensure((size_t)n_b * dq));
     RCHK(c->ev_bn.ensure((size_t)n_b * sizeof(uint32_t)));
     const int splits = exact_splits(n_b);
-    RCHK(c->ev_part.ensure((size_t)(nq > 0 ? nq : 1) * splits * k * sizeof(uint64_t)));
-    HIPCHK(patch_matrix(img_a, ha, wa, p, c->ev_am.as<uint8_t>(), dpad_b, c->stream));
-    HIPCHK(patch_norms(c->ev_am.as<uint8_t>(), n_a, dpad_b, c->ev_an.as<uint32_t>(), c->stream));
-    HIPCHK(patch_matrix(img_b, hb, wb, p, c->ev_bm.as<uint8_t>(), dpad_b, c->stream));
-    HIPCHK(patch_norms(c->ev_bm.as<uint8_t>(), n_b, dpad_b, c->ev_bn.as<uint32_t>(), c->stream));
-    HIPCHK(exact_sample(c->ev_am.as<uint8_t>(), c->ev_an.as<uint32_t>(), query_idx, nq, c->ev_bm.as<uint8_t>(),
-                        c->ev_bn.as<uint32_t>(), n_b, dpad_b, k, c->ev_part.as<uint64_t>(), out_idx, out_d2,
+    RCHK(c->ev_part.ensure(nqa * splits * k * sizeof(uint64_t)));
+    HIPCHK(patch_matrix(img_a, ha, wa, p, query_idx, nq, c->ev_am.as<uint8_t>(), dq, c->stream));
+    HIPCHK(patch_norms(c->ev_am.as<uint8_t>(), nq, dq, c->ev_an.as<uint32_t>(), c->stream));
+    HIPCHK(patch_matrix(img_b, hb, wb, p, nullptr, n_b, c->ev_bm.as<uint8_t>(), dq, c->stream));
+    // every B patch's centred norm from box sums over the image
+    RCHK(c->pnorm_tmp.ensure((size_t)hb * (wb - p + 1) * sizeof(uint32_t)));
+    HIPCHK(patch_norms_centered(img_b, hb, wb, p, c->pnorm_tmp.as<uint32_t>(), c->ev_bn.as<uint32_t>(), c->stream));
+    HIPCHK(exact_sample(c->ev_am.as<uint8_t>(), c->ev_an.as<uint32_t>(), nq, c->ev_bm.as<uint8_t>(),
+                        c->ev_bn.as<uint32_t>(), n_b, p, dq, k, c->ev_part.as<uint64_t>(), out_idx, out_d2,
                         c->stream));
     return sync_stream(c);
 }

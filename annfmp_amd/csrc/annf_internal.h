@@ -220,12 +220,17 @@ hipError_t encode_patches(const PatchGeom &g, long long npatch, const __bf16 *W1
                           float *out, hipStream_t st, int per_cu = 2);
 
 // exact field (annf_eval.hip)
-hipError_t patch_matrix(const uint8_t *img, int h, int w, int p, uint8_t *out, int dpad_b, hipStream_t st);
-hipError_t patch_norms(const uint8_t *m, long long n, int dpad_b, uint32_t *nrm, hipStream_t st);
-int exact_splits(long long nb);
-hipError_t exact_sample(const uint8_t *am, const uint32_t *an, const int *qidx, int nq, const uint8_t *bm,
-                        const uint32_t *bn, long long nb, int dpad_b, int k, uint64_t *part, int *idx, int *d2,
+// centred patch rows (x ^ 0x80, zero padded to dq = exact_dq(p) bytes; 0:
+// unsupported) of the patches idx[0 .. n) (or 0 .. n), their norms, and the
+// exact k nearest B patches of the nq query rows (part: nq * exact_splits(nb)
+// * k keys of scratch)
+int exact_dq(int p);
+hipError_t patch_matrix(const uint8_t *img, int h, int w, int p, const int *idx, long long n, uint8_t *out, int dq,
                         hipStream_t st);
+hipError_t patch_norms(const uint8_t *m, long long n, int dq, uint32_t *nrm, hipStream_t st);
+int exact_splits(long long nb);
+hipError_t exact_sample(const uint8_t *am, const uint32_t *an, int nq, const uint8_t *bm, const uint32_t *bn,
+                        long long nb, int p, int dq, int k, uint64_t *part, int *idx, int *d2, hipStream_t st);
 
 // FP64 PCA covariance of sampled patches (annf_eval.hip): scratch of
 // pca_cov_scratch(D, na + nb) bytes (256-aligned)

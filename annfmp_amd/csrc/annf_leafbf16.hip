@@ -50,6 +50,7 @@
 // selection runs in registers (a v_med3 network per key) with no LDS round
 // trip; the two lanes of a query merge their lists at the end of a segment.
 #include "annf_internal.h"
+#include "annf_lists.h"
 
 #include <type_traits>
 
@@ -104,39 +105,6 @@ __device__ __forceinline__ u32x4 load_group(const CImg &im, uint32_t e) {
     const uint32_t odd = e & 1u;
     const uint32_t idx = (e - odd) + odd * im.npad;
     return *reinterpret_cast<const u32x4a *>(im.px0 + idx);
-}
-
-__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t r;
-    asm volatile("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-
-// Sorted list of LM 32-bit keys whose live part is the top K slots; the
-// front slots hold 0, which never moves.  Slot s of the list with `key`
-// added is the median of (old l[s-1], key, old l[s]); a key above every
-// entry changes nothing.
-template <int LM>
-__device__ __forceinline__ void list_insert(uint32_t (&l)[LM], uint32_t key) {
-    if constexpr (LM <= 8) {
-#pragma unroll
-        for (int s = LM - 1; s >= 1; --s) l[s] = med3_u32(l[s - 1], key, l[s]);
-        l[0] = min(l[0], key);
-    } else if (key < l[LM - 1]) {
-#pragma unroll
-        for (int s = LM - 1; s >= 1; --s) l[s] = med3_u32(l[s - 1], key, l[s]);
-        l[0] = min(l[0], key);
-    }
-}
-
-template <int LM>
-__device__ __forceinline__ void list_insert64(uint64_t (&l)[LM], uint64_t key) {
-    bool le[LM];
-#pragma unroll
-    for (int s = 0; s < LM; ++s) le[s] = l[s] <= key;
-#pragma unroll
-    for (int s = LM - 1; s >= 1; --s) l[s] = le[s] ? l[s] : (le[s - 1] ? key : l[s - 1]);
-    l[0] = le[0] ? l[0] : key;
 }
 
 struct Out {

@@ -42,6 +42,7 @@
 // the K smallest are the answer, the order of the exact field.
 #include "annf_internal.h"
 #include "annf_patch.h"
+#include "annf_lists.h"
 
 #include <type_traits>
 
@@ -60,23 +61,6 @@ constexpr int kThreads = 256;
 constexpr int kStageSets = kThreads / kTP;   // staging threads per point
 constexpr int kMaxDQ = 448;      // p <= 12: 12 rows of 9 dwords, rounded up to 64
 constexpr uint64_t kNoKey = ~0ull;
-
-__device__ __forceinline__ uint32_t med3_u32(uint32_t a, uint32_t b, uint32_t c) {
-    uint32_t r;
-    asm volatile("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-}
-
-// Sorted list of LM 32-bit keys (every slot live, 0xFFFFFFFF empty): slot s
-// of the list with `key` added is the median of (old l[s-1], key, old l[s]).
-template <int LM>
-__device__ __forceinline__ void list_insert(uint32_t (&l)[LM], uint32_t key) {
-    if (key < l[LM - 1]) {
-#pragma unroll
-        for (int s = LM - 1; s >= 1; --s) l[s] = med3_u32(l[s - 1], key, l[s]);
-        l[0] = min(l[0], key);
-    }
-}
 
 // (x - 128) * scale rounded to E4M3 (round to nearest even), byte by byte;
 // bytes outside `mask` become code 0.

@@ -301,6 +301,27 @@ def test_exact_field_p11_matches_brute_force(gpu_available):
     assert np.array_equal(idx, ridx) and np.array_equal(d2, rd2)
 
 
+@pytest.mark.parametrize("p,k,shape", [(7, 32, (200, 210)), (12, 9, (60, 80)), (3, 5, (150, 260))])
+def test_exact_field_splits_ties_and_patch_sizes(p, k, shape, gpu_available):
+    """The int8-MFMA exact field over several B ranges (more than 32,768 B
+    patches), with flat regions (many equal distances: ties go to the lower
+    index), k up to 32 and patch sizes of 1 to 14 k-steps, against the
+    integer brute force; 333 queries (not a multiple of the block's 128)."""
+    from annfmp_amd import synth, metrics
+    a, b = synth.make_pair(synth.small_spec(*shape), seed=p + k)
+    a[10:40, 20:70] = 90
+    b[30:60, 100:160] = 90
+    b[70:90, 5:45] = 91
+    pa = O.extract_patches(a, p)
+    pb = O.extract_patches(b, p)
+    q = metrics.sample_queries(pa.shape[0], 333, seed=k)
+    q[:20] = np.arange(20) + (12 * (shape[1] - p + 1) + 25)    # queries inside the flat block
+    idx, d2 = metrics.exact_knn(a, b, p, q, k)
+    ridx, rd2 = O.exact_field(pa[q], pb, k)
+    assert np.array_equal(d2, rd2)
+    assert np.array_equal(idx, ridx)
+
+
 def test_field_high_dimension_many_neighbours_match_oracle(gpu_available):
     """d = 48 and K = 64 together (the widest lists and points of the build):
     tree, row 0, propagation sweep and best-of-K against the oracle."""
