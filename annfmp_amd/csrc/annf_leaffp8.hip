@@ -25,10 +25,13 @@
 // of a candidate is the FP32 bits of max(d~, 0) with the low PB bits
 // replaced by its position in the leaf (PB bits hold the largest leaf);
 // positions follow ascending B index, so keys order like (d~ rounded down,
-// B index).  Each lane keeps the LM smallest keys of the 16 points per tile
-// that it sees (a v_med3 network); at a leaf's end the two lanes of a query
-// merge their lists (bitonic) and the S smallest of the (query, leaf) are
-// written as 64-bit keys (d~ rounded down << 32 | B index), empty slots ~0.
+// B index).  The 32x32 MFMA block leaves lane h of a query the points
+// whose position bit 2 is h (16 per tile); each lane keeps the LM / 2
+// smallest keys of its half (a v_med3 network; LM = S rounded up to 8, 16
+// or 32), and at a leaf's end the two lanes merge their lists (bitonic) and
+// the S smallest are written as 64-bit keys (d~ rounded down << 32 | B
+// index), empty slots ~0: the S smallest of the leaf whenever neither half
+// holds more than LM / 2 of them (always so for its S / 2 smallest).
 //
 // Shortlist and re-rank (k_fp8_select), 32 lanes per query: lane i holds
 // entry i of each of the query's per-leaf lists; pairwise bitonic merges
@@ -259,9 +262,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_fp8(TreeView t, Fp8Patches
         }
     };
 
-    uint32_t l32[LM];
+    // each lane keeps the LH smallest keys of its half of the points (the
+    // tile rows with position bit 2 equal to h); the two halves' lists make
+    // the (query, leaf) list of LM
+    constexpr int LH = LM / 2;
+    uint32_t l32[LH];
 #pragma unroll
-    for (int s = 0; s < LM; ++s) l32[s] = 0xFFFFFFFFu;
+    for (int s = 0; s < LH; ++s) l32[s] = 0xFFFFFFFFu;
 
     bool more = seek();
     if (more) fetch();
@@ -302,18 +309,18 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_fp8(TreeView t, Fp8Patches
                 for (int i = 0; i < 4; ++i) {
                     const float d = __builtin_fmaf(acc[4 * j + i], sa * m2v[i], na + nbv[i]);
                     const uint32_t bits = __float_as_uint(fmaxf(d, 0.0f));
-                    list_insert<LM>(l32, (bits & ~pmask) | ps4[i]);
+                    list_insert<LH>(l32, (bits & ~pmask) | ps4[i]);
                 }
             }
             if (leaf_end) {
-                // the two lanes' lists: the LM smallest of both, sorted
-                // (a bitonic sequence from the two sorted lists, then its
-                // merging network), and the S smallest written
+                // the two lanes' lists sorted together (this lane's list
+                // followed by the other's reversed is bitonic; its merging
+                // network sorts it), and the S smallest written
                 uint32_t c[LM];
 #pragma unroll
-                for (int s = 0; s < LM; ++s) {
-                    const uint32_t o = (uint32_t)__shfl_xor((int)l32[LM - 1 - s], 32, 64);
-                    c[s] = min(l32[s], o);
+                for (int s = 0; s < LH; ++s) {
+                    c[s] = l32[s];
+                    c[LM - 1 - s] = (uint32_t)__shfl_xor((int)l32[s], 32, 64);
                 }
 #pragma unroll
                 for (int k = LM / 2; k >= 1; k >>= 1) {
@@ -345,7 +352,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_leaf_fp8(TreeView t, Fp8Patches
                     }
                 }
 #pragma unroll
-                for (int s = 0; s < LM; ++s) l32[s] = 0xFFFFFFFFu;
+                for (int s = 0; s < LH; ++s) l32[s] = 0xFFFFFFFFu;
             }
         }
         if (more) store(cur ^ 1);
@@ -549,6 +556,138 @@ __global__ __launch_bounds__(256) void k_fp8_select(const uint64_t *__restrict__
     }
 }
 
+// The two-lanes-per-entry form of k_fp8_select for rows of at most 28
+// bytes (p <= 9): a wavefront per query, lanes 2c and 2c + 1 hold entry c.
+// Each reads one 16-byte half of the candidate's 32-byte row window (the
+// dword past the first half comes from the partner lane), so a load
+// instruction touches one cache line per entry instead of one per lane and
+// half; the two partial dot products are summed across the pair.
+__device__ __forceinline__ uint64_t merge32x2(uint64_t a, uint64_t b, int lane) {
+    const int c = lane >> 1;
+    uint64_t o = shfl64(b, ((31 - c) << 1) | (lane & 1));
+    o = a < o ? a : o;
+#pragma unroll
+    for (int k = 16; k >= 1; k >>= 1) {
+        const uint64_t x = shfl_xor64(o, k << 1);
+        o = (c & k) ? (x > o ? x : o) : (x < o ? x : o);
+    }
+    return o;
+}
+
+template <int NS>
+__global__ __launch_bounds__(256) void k_fp8_select2(const uint64_t *__restrict__ keys, int SP, int S, int q0, int nq,
+                                                     const int *__restrict__ qorder, int nsrc,
+                                                     const int *__restrict__ pairleaf, PatchGeom ga, PatchGeom gb,
+                                                     const uint32_t *__restrict__ norm_a,
+                                                     const uint32_t *__restrict__ norm_b,
+                                                     const uint64_t *__restrict__ base, int K,
+                                                     uint64_t *__restrict__ keys_out, int *__restrict__ knn,
+                                                     int *__restrict__ nn, float *__restrict__ dist) {
+    constexpr int kQB = 4;                          // queries per block (a wavefront each)
+    __shared__ uint32_t sa[kQB][kMaxRowWords];
+    const int tid = threadIdx.x, lane = tid & 63, g = tid >> 6;
+    const int c = lane >> 1, part = lane & 1;
+    const int per = (int)gridDim.x / kXcds;
+    const int lb = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+    if ((long long)lb * kQB >= nq) return;
+    const int pos = lb * kQB + g;
+    const bool live = pos < nq;          // uniform over the wavefront
+    const int ql = live ? (qorder ? qorder[pos] - q0 : pos) : 0;
+    const int q = q0 + ql;
+    const int rw = (ga.rowlen + 3) >> 2;
+    const int nw = ga.rows * rw;
+    const int tail = ga.rowlen & 3;
+    const uint32_t tmask = tail ? ((1u << (8 * tail)) - 1u) : ~0u;
+
+    const long long oa = patch_origin(ga, q);
+    for (int e = lane; e < nw; e += 64) {
+        const int r = e / rw, k = e - r * rw;
+        sa[g][e] = (row_word(ga, oa + r * ga.rstride, k) ^ 0x80808080u) & (k == rw - 1 ? tmask : ~0u);
+    }
+
+    // the shortlist: entry c of every source list (both lanes of a pair)
+    uint64_t m = kNoKey;
+#pragma unroll
+    for (int r = 0; r < NS; ++r) {
+        uint64_t v = kNoKey;
+        if (live && r < nsrc && c < S) {
+            const size_t slot = (size_t)ql * nsrc + r;
+            const uint64_t kk = keys[slot * SP + c];
+            if (!pairleaf || pairleaf[slot] >= 0) v = kk;
+        }
+        m = r == 0 ? v : merge32x2(m, v, lane);
+    }
+    if (c >= S) m = kNoKey;
+    __syncthreads();
+
+    uint64_t key = kNoKey;
+    {
+        const bool have = m != kNoKey;
+        const int j = have ? (int)(uint32_t)m : 0;
+        const long long ob = patch_origin(gb, j);
+        int sab = 0;
+        for (int r = 0; r < ga.rows; ++r) {
+            const long long off = ob + r * gb.rstride;
+            const long long al = (off & ~3LL) + 16 * part;
+            uint32_t w[4];
+            if (al + 16 <= gb.nbytes) {
+                const u32x4 v = *reinterpret_cast<const u32x4a *>(gb.base + al);
+                w[0] = v[0];
+                w[1] = v[1];
+                w[2] = v[2];
+                w[3] = v[3];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    uint32_t v = 0;
+                    for (int b = 0; b < 4; ++b) {
+                        const long long o = al + 4 * k + b;
+                        if (o < gb.nbytes) v |= (uint32_t)gb.base[o] << (8 * b);
+                    }
+                    w[k] = v;
+                }
+            }
+            const uint32_t w4 = (uint32_t)__shfl_xor((int)w[0], 1, 64);   // the partner's first dword
+            const int sh = (int)(off & 3);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int kw = 4 * part + k;                // word of the row
+                if (have && kw < rw) {
+                    const uint32_t hi = k < 3 ? w[k + 1] : w4;
+                    const int y = (int)((__builtin_amdgcn_alignbyte(hi, w[k], sh) ^ 0x80808080u) &
+                                        (kw == rw - 1 ? tmask : ~0u));
+                    sab = __builtin_amdgcn_sdot4((int)sa[g][r * rw + kw], y, sab, false);
+                }
+            }
+        }
+        sab += __shfl_xor(sab, 1, 64);
+        if (have) key = ((uint64_t)(norm_a[q] + norm_b[j] - 2u * (uint32_t)sab) << 32) | (uint32_t)j;
+    }
+    // sorted across the 32 entries, merged with the query's given keys
+#pragma unroll
+    for (int k = 2; k <= 32; k <<= 1) {
+#pragma unroll
+        for (int jj = k >> 1; jj >= 1; jj >>= 1) {
+            const uint64_t x = shfl_xor64(key, jj << 1);
+            const bool keep_min = ((c & jj) == 0) == ((c & k) == 0);
+            key = keep_min ? (x < key ? x : key) : (x > key ? x : key);
+        }
+    }
+    if (base) {
+        const uint64_t bk = (live && c < K) ? base[(size_t)q * K + c] : kNoKey;
+        key = merge32x2(key, bk, lane);
+    }
+    if (live && part == 0 && c < K) {
+        const int jj = key == kNoKey ? -1 : (int)(uint32_t)key;
+        if (keys_out) keys_out[(size_t)q * K + c] = key;
+        if (knn) knn[(size_t)q * K + c] = jj;
+        if (c == 0) {
+            if (nn) nn[q] = jj;
+            if (dist) dist[q] = key == kNoKey ? kFltMax : (float)__builtin_sqrt((double)(uint32_t)(key >> 32));
+        }
+    }
+}
+
 template <typename Fn>
 hipError_t with_ks_lm(int ks, int lm_need, Fn &&fn) {
     auto pick_lm = [&](auto k) -> hipError_t {
@@ -641,9 +780,15 @@ hipError_t fp8_select(const uint64_t *keys, int S, int q0, int nq, const int *qo
     const int rw = (ga.rowlen + 3) / 4;
     const long long nblk = ((long long)nq + kSelQ - 1) / kSelQ;
     const dim3 g((unsigned)((nblk + kXcds - 1) / kXcds * kXcds)), b(256);
+    // rows of at most 28 bytes: two lanes per entry, a wavefront per query
+    const long long nblk2 = ((long long)nq + 3) / 4;
+    const dim3 g2((unsigned)((nblk2 + kXcds - 1) / kXcds * kXcds));
     auto launch = [&](auto ns) -> hipError_t {
         constexpr int NS = decltype(ns)::value;
-        if (rw + 1 <= 8)
+        if (rw <= 7)
+            hipLaunchKernelGGL((k_fp8_select2<NS>), g2, b, 0, st, keys, SP, S, q0, nq, qorder, nsrc, pairleaf, ga, gb,
+                               norm_a, norm_b, base, K, keys_out, knn, nn, dist);
+        else if (rw + 1 <= 8)
             hipLaunchKernelGGL((k_fp8_select<NS, 8, 9>), g, b, 0, st, keys, SP, S, q0, nq, qorder, nsrc,
                                pairleaf, ga, gb, norm_a, norm_b, base, K, keys_out, knn, nn, dist);
         else
