@@ -1067,7 +1067,6 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
                                   nn_out, dist_out, c->stream));
                 return ANNF_OK;
             }
-            HIPCHK(fill_keys(keys2, (long long)np * K, c->stream));
             HIPCHK(leaf_bf16(t, c->cimg_a.as<uint16_t>(), wa, c->cimg_b.as<uint16_t>(), wb, psize,
                              c->pnorm_a.as<uint32_t>(), c->pnorm_b.as<uint32_t>(), pts_order, c->order.as<int>(),
                              c->boffset.as<int>(), c->bchunk.as<int>(), blk_info, (int)np, 1, K, prop_k, keys2,
@@ -1078,19 +1077,19 @@ int annf_field_mm_dev(annf_ctx *c, const uint8_t *img_a, int ha, int wa, const u
             RCHK(pair_pass(c->mm_keys1.as<uint64_t>(), 0, n_a));
         } else if (!chain) {
             RCHK(pair_pass(c->mm_keys1.as<uint64_t>(), 0, n_a));
-            HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, 0, n_a, K, prop_k, knn_out, nn_out, dist_out,
-                                nullptr, c->stream));
+            HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, c->mm_pairleaf.as<int>(), 0, n_a, K, prop_k,
+                                knn_out, nn_out, dist_out, nullptr, c->stream));
         } else {
             // row by row: row y's pairs come from row y - 1's final keys
             // (row 0's are its first-pass keys), the reference's dependence
             RCHK(c->mm_fkeys.ensure((size_t)n_a * K * sizeof(uint64_t)));
             uint64_t *fk = c->mm_fkeys.as<uint64_t>();
-            HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), nullptr, 0, nc_a, K, 1, knn_out, nn_out, dist_out, fk,
-                                c->stream));
+            HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), nullptr, nullptr, 0, nc_a, K, 1, knn_out, nn_out,
+                                dist_out, fk, c->stream));
             for (int y = 1; y < nr_a; ++y) {
                 RCHK(pair_pass(fk, y * nc_a, nc_a));
-                HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, y * nc_a, nc_a, K, prop_k, knn_out, nn_out,
-                                    dist_out, fk, c->stream));
+                HIPCHK(finish_field(c->mm_keys1.as<uint64_t>(), keys2, c->mm_pairleaf.as<int>(), y * nc_a, nc_a, K,
+                                    prop_k, knn_out, nn_out, dist_out, fk, c->stream));
             }
         }
     }

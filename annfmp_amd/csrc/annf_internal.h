@@ -192,12 +192,13 @@ hipError_t fp8_select(const uint64_t *keys, int S, int q0, int nq, const int *qo
 // per-leaf counts
 hipError_t prop_pairs(const uint64_t *keys1, int q0, int nq, int K, int prop_k, int nprobe, const int *natleaf,
                       const int *pli, int n_cols, int n_b, int *pairleaf, int *count, hipStream_t st);
-hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st);
-// the K smallest of each query's first-pass keys (keys1, by query) and its
-// pairs' keys (keys2, by query - q0) for queries q0 .. q0 + nq - 1; any
-// output may be null; keys_out: the final keys, by query
-hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int q0, int nq, int K, int prop_k, int *knn,
-                        int *nn, float *dist, uint64_t *keys_out, hipStream_t st);
+// the K smallest of each query's first-pass keys (keys1, by query, each
+// list ascending) and its scanned pairs' keys (keys2, by query - q0; pairs
+// with pairleaf -1 are skipped, so keys2 need not be filled for them) for
+// queries q0 .. q0 + nq - 1; any output may be null; keys_out: the final
+// keys, by query
+hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, const int *pairleaf, int q0, int nq, int K,
+                        int prop_k, int *knn, int *nn, float *dist, uint64_t *keys_out, hipStream_t st);
 
 // learned encoder (annf_mlp.hip): parameters W1 (h1 x D), b1, W2 (h2 x h1),
 // b2, W3 (d x h2), b3 in one FP32 buffer; inputs (x - 128) / 128

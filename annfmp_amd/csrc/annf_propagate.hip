@@ -14,15 +14,9 @@ namespace {
 
 constexpr uint64_t kNoKey = ~0ull;
 
-// Sorted key list of LM slots whose live part is the top L slots; the front
+// Sorted key list of LM slots whose live part is the top K slots; the front
 // slots hold 0, which never moves (an insert lands after every key <= it),
 // so every index is static.
-template <int LM>
-__device__ __forceinline__ void klist_init(uint64_t (&l)[LM], int L) {
-#pragma unroll
-    for (int s = 0; s < LM; ++s) l[s] = (s < LM - L) ? 0ull : kNoKey;
-}
-
 // Branch-free: the list is sorted, so l[s] <= key holds for a prefix of the
 // slots; slot s keeps its key inside that prefix, takes `key` right after it
 // and its left neighbour's key further on (a key above every entry changes
@@ -99,35 +93,66 @@ __global__ __launch_bounds__(kLeafSortTPB) void k_leaf_order(const int *__restri
     }
 }
 
-__global__ void k_fill_keys(uint64_t *__restrict__ keys, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = kNoKey;
-}
-
 // The K smallest of a query's first-pass keys and its propagated pairs'
 // keys (k_prop_pairs leaves out every leaf the first pass scanned and
 // repeats, so the leaves are disjoint and no B index comes twice) -> knn,
-// nn, dist.
+// nn, dist.  One thread per query; a wavefront's 64 queries own one
+// contiguous run of pair keys, which the wavefront reads 16 keys per query
+// at a time, coalesced, through LDS (each thread then takes its own row).
+constexpr int kFinChunk = 16;
+
+// orders the LDS accesses of the lanes of one wavefront
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <int KM>
-__global__ void k_finish(const uint64_t *__restrict__ keys1, const uint64_t *__restrict__ keys2, int q0, int nq,
-                         int K, int prop_k, int *__restrict__ knn, int *__restrict__ nn, float *__restrict__ dist,
-                         uint64_t *__restrict__ keys_out) {
-    const int ql = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ql >= nq) return;
-    const int q = q0 + ql;
+__global__ __launch_bounds__(256) void k_finish(const uint64_t *__restrict__ keys1,
+                                                const uint64_t *__restrict__ keys2,
+                                                const int *__restrict__ pairleaf, int q0, int nq, int K,
+                                                int prop_k, int *__restrict__ knn, int *__restrict__ nn,
+                                                float *__restrict__ dist, uint64_t *__restrict__ keys_out) {
+    __shared__ uint64_t tile[4][kWave][kFinChunk + 1];
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int wq0 = blockIdx.x * blockDim.x + w * kWave;    // the wavefront's first query (local)
+    if (wq0 >= nq) return;                                   // whole wavefronts leave together
+    const int ql = wq0 + lane;
+    const bool live = ql < nq;
+    const int q = q0 + (live ? ql : wq0);
+    // the first-pass keys are ascending already: they are the list
     uint64_t l[KM];
-    klist_init(l, K);
-    for (int s = 0; s < K; ++s) {
-        const uint64_t key = keys1[(size_t)q * K + s];
-        if (key < l[KM - 1]) klist_insert(l, key);
-    }
+#pragma unroll
+    for (int s = 0; s < KM; ++s) l[s] = s < KM - K ? 0ull : keys1[(size_t)q * K + (s - (KM - K))];
     if (keys2) {
-        const uint64_t *p2 = keys2 + (size_t)ql * prop_k * K;
-        for (int s = 0; s < prop_k * K; ++s) {
-            const uint64_t key = p2[s];
-            if (key < l[KM - 1]) klist_insert(l, key);
+        // pairs of leaf -1 were not scanned: their keys are skipped
+        uint32_t dead = 0;
+        for (int k = 0; k < prop_k; ++k)
+            if (live && pairleaf[(size_t)ql * prop_k + k] < 0) dead |= 1u << k;
+        const int n2 = prop_k * K;
+        const int nw = min(kWave, nq - wq0);                 // queries of this wavefront
+        const uint64_t *src = keys2 + (size_t)wq0 * n2;
+        for (int c0 = 0; c0 < n2; c0 += kFinChunk) {
+            // keys c0 .. c0 + 15 of every query of the wavefront: eight lanes
+            // per query row, two keys each, eight rows per pass
+            for (int e = lane; e < nw * (kFinChunk / 2); e += kWave) {
+                const int row = e / (kFinChunk / 2), u = 2 * (e - row * (kFinChunk / 2));
+                const size_t o = (size_t)row * n2 + c0 + u;
+                tile[w][row][u] = c0 + u < n2 ? src[o] : kNoKey;
+                tile[w][row][u + 1] = c0 + u + 1 < n2 ? src[o + 1] : kNoKey;
+            }
+            wave_lds_sync();
+#pragma unroll
+            for (int u = 0; u < kFinChunk; ++u) {
+                const int s = c0 + u;
+                const uint64_t key = (s < n2 && !((dead >> (s / K)) & 1u)) ? tile[w][lane][u] : kNoKey;
+                if (key < l[KM - 1]) klist_insert(l, key);
+            }
+            wave_lds_sync();
         }
     }
+    if (!live) return;
 #pragma unroll
     for (int s = 0; s < KM; ++s) {
         if (s < KM - K) continue;
@@ -154,20 +179,17 @@ hipError_t prop_pairs(const uint64_t *keys1, int q0, int nq, int K, int prop_k, 
     return hipGetLastError();
 }
 
-hipError_t fill_keys(uint64_t *keys, long long n, hipStream_t st) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, keys, n);
-    return hipGetLastError();
-}
-
-hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, int q0, int nq, int K, int prop_k, int *knn,
-                        int *nn, float *dist, uint64_t *keys_out, hipStream_t st) {
+hipError_t finish_field(const uint64_t *keys1, const uint64_t *keys2, const int *pairleaf, int q0, int nq, int K,
+                        int prop_k, int *knn, int *nn, float *dist, uint64_t *keys_out, hipStream_t st) {
+    if (keys2 && !pairleaf) return hipErrorInvalidValue;
     if (nq == 0) return hipSuccess;
     const dim3 g((unsigned)((nq + 255) / 256)), b(256);
     if (K <= 8)
-        hipLaunchKernelGGL(k_finish<8>, g, b, 0, st, keys1, keys2, q0, nq, K, prop_k, knn, nn, dist, keys_out);
+        hipLaunchKernelGGL(k_finish<8>, g, b, 0, st, keys1, keys2, pairleaf, q0, nq, K, prop_k, knn, nn, dist,
+                           keys_out);
     else
-        hipLaunchKernelGGL(k_finish<32>, g, b, 0, st, keys1, keys2, q0, nq, K, prop_k, knn, nn, dist, keys_out);
+        hipLaunchKernelGGL(k_finish<32>, g, b, 0, st, keys1, keys2, pairleaf, q0, nq, K, prop_k, knn, nn, dist,
+                           keys_out);
     return hipGetLastError();
 }
 
