@@ -2198,9 +2198,11 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                 // below the bound; every read is issued unconditionally (lanes
                 // without a point read point 0), so each group is in flight at once
                 constexpr int TL = DP - 4;
+                // chunks past the row's last (c0 + u >= wn, a wave-uniform
+                // test) issue nothing: most rows have one or two chunks
                 int4 e[UNR];
 #pragma unroll
-                for (int u = 0; u < UNR; ++u) e[u] = tab[min(c0 + u, kSweepTab - 1)];
+                for (int u = 0; u < UNR; ++u) e[u] = c0 + u < wn ? tab[c0 + u] : make_int4(0, 0, 0, 0);
                 float4 hd[UNR];
                 int pos[UNR], meta[UNR];
                 bool val[UNR];
@@ -2209,7 +2211,8 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                     val[u] = c0 + u < wn && lane < e[u].y;
                     pos[u] = val[u] ? e[u].x + lane : 0;
                     meta[u] = (e[u].z << 24) | e[u].w;   // owner entry, offset of the chunk in its leaf
-                    hd[u] = head[pos[u]];
+                    hd[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                    if (c0 + u < wn) hd[u] = head[pos[u]];
                 }
                 if (!fetched && more) {
                     const size_t nqi = qi + n_cols;
@@ -2226,6 +2229,9 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                 int plc[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
+                    s2[u] = 0.0f;
+                    plc[u] = 0;
+                    if (c0 + u >= wn) continue;
                     float dd, a = 0.0f;
                     dd = hd[u].x - qv[0]; a = a + dd * dd;
                     dd = hd[u].y - qv[1]; a = a + dd * dd;
@@ -2250,6 +2256,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                 }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
+                    if (c0 + u >= wn) continue;
                     float a = s2[u];
                     if constexpr (TL > 0) {
 #pragma unroll
