@@ -246,7 +246,7 @@ void level_bounds(int n, int depth, std::vector<int> &bnd, std::vector<int> &off
         bnd.insert(bnd.end(), cur.begin(), cur.end());
         int maxc = 0;
         for (size_t j = 0; j + 1 < cur.size(); ++j) maxc = std::max(maxc, cur[j + 1] - cur[j]);
-        cpn[l] = std::max(1, (maxc + 2047) / 2048);
+        cpn[l] = std::max(1, (maxc + kBuildChunk - 1) / kBuildChunk);
         maxc_out[l] = maxc;
         if (l == depth) {
             max_leaf = maxc;

@@ -11,6 +11,8 @@ constexpr float kFltMax = 3.402823466e+38f;  // list sentinel, float.h:24
 
 // XCDs of an MI355X (workgroups are handed to them round-robin by index)
 constexpr int kXcds = 8;
+// points per block of the tree-level kernels (one chunk of a node)
+constexpr int kBuildChunk = 2048;
 
 // rank of this lane among the lanes set in m
 __device__ __forceinline__ int lane_rank(unsigned long long m) {

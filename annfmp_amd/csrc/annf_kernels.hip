@@ -240,8 +240,9 @@ __device__ __forceinline__ int2 block_scan_int2_tot(int2 v, int2 *wsum /* __shar
 // ---------------------------------------------------------------------------
 
 constexpr int kBuildTPB = 256;
-constexpr int kBuildEPT = 8;
-constexpr int kChunk = kBuildTPB * kBuildEPT;
+constexpr int kChunk = kBuildChunk;
+constexpr int kBuildEPT = kChunk / kBuildTPB;
+static_assert(kBuildEPT * kBuildTPB == kChunk, "a chunk is whole elements per thread");
 
 struct LevelArgs {
     int nodes;        // 2^lvl
