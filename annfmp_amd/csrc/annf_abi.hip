@@ -101,7 +101,7 @@ struct annf_ctx {
     // build scratch
     DevBuf keys, eqoff, box, box2, nax, prefix, rank, hist, cnt, cnt_off, tot, bnd, errflag;
     // query workspaces
-    DevBuf qbuf, natleaf, natdesc, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf, ptshead;
+    DevBuf qbuf, natleaf, natdesc, bcount, boffset, bcursor, bchunk, order, p1d, p1p, p1l, propleaf;
     DevBuf fl_pos, fl_d, fl_l, final_i, final_d, stage_in, stage_in2, stage_in3;
     DevBuf basis32, mean32, basis64, mean64;
     DevBuf ev_am, ev_an, ev_bm, ev_bn, ev_part;
@@ -162,7 +162,7 @@ struct annf_ctx {
     void release_all() {
         DevBuf *list[] = {&pts_a, &pts_b, &orig_a, &orig_b, &axis, &split, &leaves, &pli, &keys, &eqoff, &box, &box2,
                           &nax, &prefix, &rank, &hist, &cnt, &cnt_off, &tot, &bnd, &errflag, &qbuf, &natleaf, &natdesc, &bcount, &boffset,
-                          &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &ptshead, &fl_pos, &fl_d, &fl_l, &final_i,
+                          &bcursor, &bchunk, &order, &p1d, &p1p, &p1l, &propleaf, &fl_pos, &fl_d, &fl_l, &final_i,
                           &final_d, &stage_in, &stage_in2, &stage_in3, &basis32, &mean32, &basis64, &mean64,
                           &ev_am, &ev_an, &ev_bm, &ev_bn, &ev_part, &mlp_ws, &enc_w1, &enc_b1, &enc_w2,
                           &enc_b2, &enc_w3, &enc_b3, &mm_keys1, &mm_keys2, &mm_pairleaf, &mm_fkeys, &pca_scratch, &pca_eig, &ocl_lub, &ocl_indir, &ocl_indir2, &ocl_keys,
@@ -408,10 +408,9 @@ int run_rows(annf_ctx *c, int n_a, int n_rows, int n_cols, int K, int propagate,
                         c->fl_pos.as<int>(), c->fl_l.as<int>(), c->stream));
     c->mark(5);
     if (propagate && n_rows > 1) {
-        RCHK(c->ptshead.ensure((size_t)t.n * 4 * sizeof(float)));
         HIPCHK(row_sweep(t, c->qbuf.as<float>(), n_rows, n_cols, K, propleaf, c->p1d.as<float>(),
                          c->p1p.as<int>(), c->p1l.as<int>(), c->natdesc.as<int>(), c->fl_pos.as<int>(),
-                         c->fl_l.as<int>(), c->ptshead.as<float>(), c->stream));
+                         c->fl_l.as<int>(), c->stream));
     }
     c->mark(6);
     HIPCHK(best_of_k(ga, gb, c->fl_pos.as<int>(), t.orig, n_a, K, select_best, final_dev, dist_dev, knn_dev,

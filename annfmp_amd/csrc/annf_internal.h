@@ -123,7 +123,7 @@ hipError_t exact_search(const TreeView &t, const float *q, int nq, int K, long l
 hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hipStream_t st);
 hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K,
                      const int *propleaf, const float *p1d, const int *p1p, const int *p1l,
-                     const int *natleaf, int *fl_pos, const int *row0_l, float *head, hipStream_t st);
+                     const int *natleaf, int *fl_pos, const int *row0_l, hipStream_t st);
 hipError_t best_of_k(const PatchGeom &ga, const PatchGeom &gb, const int *fl_pos,
                      const int *orig, int nq, int K, int select_best, int *final_out,
                      float *dist_out, int *knn_out, hipStream_t st);

@@ -2097,9 +2097,12 @@ hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hi
 //     arrivals.
 // A row's unique leaves are cut into chunks of 64 points, listed in an LDS
 // table (point range, owner entry, offset in the leaf); the loads of UNR
-// chunks are issued together, so a row whose leaves fit UNR chunks (K = 5
-// with leaves of up to 128 points: 10) costs one dependent memory round
-// trip.  Leaf ranges come from the leaf descriptors (propleaf), and the next
+// chunks (every valid point's whole row and leaf descriptor) are issued
+// together, so a row whose leaves fit UNR chunks (K = 5 with leaves of up to
+// 64 points: 6) costs one dependent memory round trip after its table.  (A
+// first round of dimensions 0-3 that filtered which rows load the rest cut
+// the bytes but added a round trip per row: about 5 % slower.)  Leaf ranges
+// come from the leaf descriptors (propleaf), and the next
 // row's query, natural leaf and natural list are requested while the current
 // row is scored.  Consecutive workgroups go to different XCDs, so the
 // columns are dealt out in bands: XCD i walks columns [i B, (i + 1) B), and
@@ -2114,7 +2117,7 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                                                  const float *__restrict__ p1d, const int *__restrict__ p1p,
                                                  const int *__restrict__ p1l, const int *__restrict__ natleaf,
                                                  int *__restrict__ fl_pos, const int *__restrict__ row0_l, int stride,
-                                                 int band, const float4 *__restrict__ head) {
+                                                 int band) {
     extern __shared__ uint32_t smem_w[];
     const int lane = threadIdx.x;
     const BlockSet S = bset_at(smem_w, kSweepCap);
@@ -2194,26 +2197,27 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                 }
             wave_sync();
             for (int c0 = 0; c0 < wn; c0 += UNR) {
-                // the chunks' table entries, then the head (dimensions 0-3) of
-                // every point, then the rest of the points whose partial sum is
-                // below the bound; every read is issued unconditionally (lanes
-                // without a point read point 0), so each group is in flight at once
-                constexpr int TL = DP - 4;
-                // chunks past the row's last (c0 + u >= wn, a wave-uniform
-                // test) issue nothing: most rows have one or two chunks
+                // the chunks' table entries, then every valid point's row and
+                // leaf descriptor, all issued before the first use: one
+                // dependent memory round trip per window of UNR chunks
                 int4 e[UNR];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) e[u] = c0 + u < wn ? tab[c0 + u] : make_int4(0, 0, 0, 0);
-                float4 hd[UNR];
-                int pos[UNR], meta[UNR];
+                int pos[UNR], meta[UNR], plc[UNR];
                 bool val[UNR];
+                float4 rw[UNR][DP / 4];
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     val[u] = c0 + u < wn && lane < e[u].y;
                     pos[u] = val[u] ? e[u].x + lane : 0;
                     meta[u] = (e[u].z << 24) | e[u].w;   // owner entry, offset of the chunk in its leaf
-                    hd[u] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                    if (c0 + u < wn) hd[u] = head[pos[u]];
+                    plc[u] = 0;
+                    if (c0 + u < wn) {
+                        const float4 *src = reinterpret_cast<const float4 *>(t.pts + (size_t)pos[u] * DP);
+#pragma unroll
+                        for (int c4 = 0; c4 < DP / 4; ++c4) rw[u][c4] = src[c4];
+                        plc[u] = propleaf[pos[u]];
+                    }
                 }
                 if (!fetched && more) {
                     const size_t nqi = qi + n_cols;
@@ -2225,46 +2229,17 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
                     nl0 = p1l[nqi * K + kl];
                     fetched = true;
                 }
-                float s2[UNR];
-                float tl[UNR][TL > 0 ? TL : 1];
-                int plc[UNR];
-#pragma unroll
-                for (int u = 0; u < UNR; ++u) {
-                    s2[u] = 0.0f;
-                    plc[u] = 0;
-                    if (c0 + u >= wn) continue;
-                    float dd, a = 0.0f;
-                    dd = hd[u].x - qv[0]; a = a + dd * dd;
-                    dd = hd[u].y - qv[1]; a = a + dd * dd;
-                    dd = hd[u].z - qv[2]; a = a + dd * dd;
-                    dd = hd[u].w - qv[3]; a = a + dd * dd;
-                    s2[u] = a;
-                    // the sum only grows: a partial sum at or above the bound rejects
-                    val[u] = val[u] && a < bound;
-                    const int tp = val[u] ? pos[u] : 0;
-                    if constexpr (TL > 0) {
-                        const float4 *src = reinterpret_cast<const float4 *>(t.pts + (size_t)tp * DP + 4);
-#pragma unroll
-                        for (int c4 = 0; c4 < TL / 4; ++c4) {
-                            const float4 v = src[c4];
-                            tl[u][4 * c4 + 0] = v.x;
-                            tl[u][4 * c4 + 1] = v.y;
-                            tl[u][4 * c4 + 2] = v.z;
-                            tl[u][4 * c4 + 3] = v.w;
-                        }
-                    }
-                    plc[u] = propleaf[tp];
-                }
 #pragma unroll
                 for (int u = 0; u < UNR; ++u) {
                     if (c0 + u >= wn) continue;
-                    float a = s2[u];
-                    if constexpr (TL > 0) {
+                    float a = 0.0f;
 #pragma unroll
-                        for (int jj = 0; jj < TL; ++jj) {
-                            const float dd = tl[u][jj] - qv[4 + jj];
-                            a = a + dd * dd;
-                        }
+                    for (int c4 = 0; c4 < DP / 4; ++c4) {
+                        float dd;
+                        dd = rw[u][c4].x - qv[4 * c4 + 0]; a = a + dd * dd;
+                        dd = rw[u][c4].y - qv[4 * c4 + 1]; a = a + dd * dd;
+                        dd = rw[u][c4].z - qv[4 * c4 + 2]; a = a + dd * dd;
+                        dd = rw[u][c4].w - qv[4 * c4 + 3]; a = a + dd * dd;
                     }
                     const bool take = val[u] && a < bound;
                     if (__ballot(take) == 0ull) continue;
@@ -2332,20 +2307,12 @@ __global__ __launch_bounds__(kWave) void k_sweep(TreeView t, const float *__rest
     }
 }
 
-// head[i] = dimensions 0-3 of tree point i (16 bytes a point, contiguous)
-__global__ void k_point_heads(const float *__restrict__ pts, int n, int dpad, float4 *__restrict__ head) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) head[i] = *reinterpret_cast<const float4 *>(pts + (size_t)i * dpad);
-}
-
 hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, int K, const int *propleaf,
                      const float *p1d, const int *p1p, const int *p1l, const int *natleaf, int *fl_pos,
-                     const int *row0_l, float *head, hipStream_t st) {
+                     const int *row0_l, hipStream_t st) {
     if (n_rows < 2) return hipSuccess;
     // offsets of a leaf's chunks (below the leaf size) share a key field with the owner entry
     if (t.max_leaf >= (1 << 24)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_point_heads, dim3(blocks_for(t.n, 256)), dim3(256), 0, st, t.pts, t.n, t.dpad,
-                       reinterpret_cast<float4 *>(head));
     const size_t lds = (size_t)set_words(kSweepCap) * sizeof(uint32_t) + kSweepTab * sizeof(int4) +
                        (size_t)kRowBatch * kWave * sizeof(int);
     const int band = (n_cols + kXcds - 1) / kXcds;
@@ -2356,12 +2323,16 @@ hipError_t row_sweep(const TreeView &t, const float *q, int n_rows, int n_cols, 
         auto launch = [&](auto unr) {
             constexpr int UNR = decltype(unr)::value;
             hipLaunchKernelGGL((k_sweep<DP, UNR>), dim3((unsigned)(band * kXcds)), dim3(kWave), lds, st, t, q,
-                               n_rows, n_cols, K, propleaf, p1d, p1p, p1l, natleaf, fl_pos, row0_l, t.max_leaf, band,
-                               reinterpret_cast<const float4 *>(head));
+                               n_rows, n_cols, K, propleaf, p1d, p1p, p1l, natleaf, fl_pos, row0_l, t.max_leaf, band);
             return hipGetLastError();
         };
-        if (DP <= 16 && chunks > 4) return launch(std::integral_constant<int, 10>{});
-        return launch(std::integral_constant<int, 4>{});
+        // a window holds UNR chunks' rows in registers (UNR x DP floats
+        // per lane): six for DP <= 16 (a K = 5 row of 64-point leaves is one
+        // window), fewer for wider points
+        if (DP <= 16 && chunks > 4) return launch(std::integral_constant<int, 6>{});
+        if (DP <= 16) return launch(std::integral_constant<int, 4>{});
+        if (DP <= 32) return launch(std::integral_constant<int, 3>{});
+        return launch(std::integral_constant<int, 2>{});
     });
 }
 
