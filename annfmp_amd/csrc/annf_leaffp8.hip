@@ -788,9 +788,6 @@ hipError_t fp8_select(const uint64_t *keys, int S, int q0, int nq, const int *qo
         if (rw <= 7)
             hipLaunchKernelGGL((k_fp8_select2<NS>), g2, b, 0, st, keys, SP, S, q0, nq, qorder, nsrc, pairleaf, ga, gb,
                                norm_a, norm_b, base, K, keys_out, knn, nn, dist);
-        else if (rw + 1 <= 8)
-            hipLaunchKernelGGL((k_fp8_select<NS, 8, 9>), g, b, 0, st, keys, SP, S, q0, nq, qorder, nsrc,
-                               pairleaf, ga, gb, norm_a, norm_b, base, K, keys_out, knn, nn, dist);
         else
             hipLaunchKernelGGL((k_fp8_select<NS, 12, 6>), g, b, 0, st, keys, SP, S, q0, nq, qorder, nsrc,
                                pairleaf, ga, gb, norm_a, norm_b, base, K, keys_out, knn, nn, dist);
