@@ -1708,10 +1708,25 @@ __device__ __forceinline__ void wset_append(const BlockSet &S, int &n, bool take
     n += __popcll(m);
 }
 
+// Minimum of v over the 64 lanes (every lane gets it): DPP within each row
+// of 16 lanes, then the four rows' minima read as scalars.
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x141, 0xF, 0xF, false));   // row_half_mirror
+    v = min(v, (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x140, 0xF, 0xF, false));   // row_mirror
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)v, 0);
+    const uint32_t b = (uint32_t)__builtin_amdgcn_readlane((int)v, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)v, 32);
+    const uint32_t d = (uint32_t)__builtin_amdgcn_readlane((int)v, 48);
+    return min(min(a, b), min(c, d));
+}
+
 // bset_reduce for a one-wavefront block: with at most 64 items each lane
-// holds one item and ranks it against every key (LDS broadcast reads, eight
-// in flight); the items of rank < K move to slots 0..K-1, and lane j < K
-// returns item j's position and leaf (opos/opl; other lanes keep theirs).
+// holds one item, and min(n, K) rounds of a wave minimum over the 64-bit
+// keys (all distinct) take them in ascending order; the items of rank < K
+// move to slots 0..K-1, and lane j < K returns item j's position and leaf
+// (opos/opl; other lanes keep theirs).
 __device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, int lane, int &opos, int &opl) {
     wave_sync();
     if (n > kWave) {
@@ -1723,21 +1738,32 @@ __device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, in
         return;
     }
     const bool have = lane < n;
-    const uint64_t mk = have ? S.key[lane] : ~0ull;
+    uint64_t mk = have ? S.key[lane] : ~0ull;
     const int mp = have ? S.pos[lane] : 0, ml = have ? S.pl[lane] : 0;
-    const int r = n > 0 ? rank_below(S.key, n, mk) : 0;
     const int m = min(n, K);
-    wave_sync();
-    if (have && r < m) {
-        S.key[r] = mk;
-        S.pos[r] = mp;
-        S.pl[r] = ml;
+    uint64_t rk = 0;
+    int rp = 0, rl = 0;
+    for (int r = 0; r < m; ++r) {
+        const uint32_t hi = (uint32_t)(mk >> 32), lo = (uint32_t)mk;
+        const uint32_t mh = wave_min_u32(hi);
+        const uint32_t mlo = wave_min_u32(hi == mh ? lo : 0xFFFFFFFFu);
+        const int wl = __builtin_ctzll(__ballot(hi == mh && lo == mlo));
+        const int wp = __builtin_amdgcn_readlane(mp, wl), wq = __builtin_amdgcn_readlane(ml, wl);
+        if (lane == r) {
+            rk = ((uint64_t)mh << 32) | mlo;
+            rp = wp;
+            rl = wq;
+        }
+        if (lane == wl) mk = ~0ull;
     }
-    wave_sync();
     if (lane < m) {
-        opos = S.pos[lane];
-        opl = S.pl[lane];
+        S.key[lane] = rk;
+        S.pos[lane] = rp;
+        S.pl[lane] = rl;
+        opos = rp;
+        opl = rl;
     }
+    wave_sync();
     n = m;
 }
 
