@@ -2461,9 +2461,96 @@ __global__ __launch_bounds__(256) void k_best(PatchGeom ga, PatchGeom gb, const 
     if (fdist) fdist[i] = bd;
 }
 
+// k_best for image patches whose rows fit LW - 1 dwords and K <= kBestMax:
+// the patch is walked row by row, and each row of the query and of every
+// candidate is read as one LW-dword window (wide loads, all in flight
+// together) instead of two dword loads per word; the integer sums and the
+// selection are those of k_best.
+constexpr int kBestMax = 8;
+
+template <int LW>
+__global__ __launch_bounds__(256) void k_best_w(PatchGeom ga, PatchGeom gb, const int *__restrict__ fl_pos,
+                                                const int *__restrict__ orig, int nq, int K, int sel,
+                                                int *__restrict__ fin, float *__restrict__ fdist,
+                                                int *__restrict__ knn) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    const int *lp = fl_pos + (size_t)i * K;
+    int jb[kBestMax];
+#pragma unroll
+    for (int k = 0; k < kBestMax; ++k) jb[k] = k < K ? orig[lp[k]] : 0;
+    if (knn)
+#pragma unroll
+        for (int k = 0; k < kBestMax; ++k)
+            if (k < K) knn[(size_t)i * K + k] = jb[k];
+    const int kk = sel ? K : 1;
+    if (!sel && !fdist) {
+        fin[i] = jb[0];
+        return;
+    }
+    const int rw = (ga.rowlen + 3) >> 2;
+    const int tail = ga.rowlen & 3;
+    const uint32_t tmask = tail ? ((1u << (8 * tail)) - 1u) : 0xFFFFFFFFu;
+    const long long oa = patch_off(ga, i);
+    long long ob[kBestMax];
+#pragma unroll
+    for (int k = 0; k < kBestMax; ++k) ob[k] = patch_off(gb, jb[k]);
+    uint32_t saa = 0, sbb[kBestMax], sab[kBestMax];
+#pragma unroll
+    for (int k = 0; k < kBestMax; ++k) sbb[k] = sab[k] = 0;
+    for (int r = 0; r < ga.rows; ++r) {
+        uint32_t wa[1][LW], wb[kBestMax][1][LW];
+        fetch_rows<1, LW>(ga, oa, true, r, 1, wa);
+#pragma unroll
+        for (int k = 0; k < kBestMax; ++k) fetch_rows<1, LW>(gb, ob[k], k < kk, r, 1, wb[k]);
+        const int sha = (int)((oa + (long long)r * ga.rstride) & 3);
+#pragma unroll
+        for (int w = 0; w < LW - 1; ++w) {
+            if (w < rw) {
+                uint32_t va = __builtin_amdgcn_alignbyte(wa[0][w + 1], wa[0][w], sha);
+                if (w == rw - 1) va &= tmask;
+                saa = __builtin_amdgcn_udot4(va, va, saa, false);
+#pragma unroll
+                for (int k = 0; k < kBestMax; ++k) {
+                    if (k < kk) {
+                        const int shb = (int)((ob[k] + (long long)r * gb.rstride) & 3);
+                        uint32_t vb = __builtin_amdgcn_alignbyte(wb[k][0][w + 1], wb[k][0][w], shb);
+                        if (w == rw - 1) vb &= tmask;
+                        sbb[k] = __builtin_amdgcn_udot4(vb, vb, sbb[k], false);
+                        sab[k] = __builtin_amdgcn_udot4(va, vb, sab[k], false);
+                    }
+                }
+            }
+        }
+    }
+    int best = -1;
+    float bd = kFltMax;
+#pragma unroll
+    for (int k = 0; k < kBestMax; ++k) {
+        if (k >= kk) continue;
+        const uint32_t S = saa + sbb[k] - 2u * sab[k];
+        const float dd = (S < (1u << 24)) ? sqrt_rn((float)S) : patch_l2_fp32(ga, i, gb, jb[k]);
+        if (!sel) { best = jb[k]; bd = dd; continue; }
+        if (dd < bd) { bd = dd; best = jb[k]; }
+    }
+    fin[i] = best;
+    if (fdist) fdist[i] = bd;
+}
+
 hipError_t best_of_k(const PatchGeom &ga, const PatchGeom &gb, const int *fl_pos, const int *orig, int nq, int K,
                      int select_best, int *final_out, float *dist_out, int *knn_out, hipStream_t st) {
     if (nq == 0) return hipSuccess;
+    const int rw = (ga.rowlen + 3) >> 2;
+    auto wide = [&](auto lw) {
+        constexpr int LW = decltype(lw)::value;
+        hipLaunchKernelGGL((k_best_w<LW>), dim3(blocks_for(nq, 256)), dim3(256), 0, st, ga, gb, fl_pos, orig, nq, K,
+                           select_best, final_out, dist_out, knn_out);
+        return hipGetLastError();
+    };
+    if (ga.image && gb.image && K <= kBestMax && ga.rows == gb.rows && ga.rowlen == gb.rowlen) {
+        if (rw <= 7) return wide(std::integral_constant<int, 8>{});
+        if (rw <= 9) return wide(std::integral_constant<int, 10>{});
+    }
     hipLaunchKernelGGL(k_best, dim3(blocks_for(nq, 256)), dim3(256), 0, st, ga, gb, fl_pos, orig, nq, K, select_best,
                        final_out, dist_out, knn_out);
     return hipGetLastError();
