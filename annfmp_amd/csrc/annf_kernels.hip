@@ -2113,7 +2113,8 @@ hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hi
 // the work per row to about two leaves:
 //   * only candidates strictly closer than the current K-th distance can
 //     enter (an equal distance arrives later and loses), so they are filtered
-//     in registers and the survivors merged in the wavefront's LDS set;
+//     in registers and the survivors appended to the wavefront's LDS set,
+//     whose K smallest keys K rounds of a wave minimum then pick;
 //   * a leaf listed several times is scored once and every surviving point
 //     is appended once per occurrence, each with its own arrival (the
 //     reference's duplicates);
@@ -2124,8 +2125,9 @@ hipError_t prop_leaves(const TreeView &t, int n_a, int n_cols, int *propleaf, hi
 // A row's unique leaves are cut into chunks of 64 points, listed in an LDS
 // table (point range, owner entry, offset in the leaf); the loads of UNR
 // chunks (every valid point's whole row and leaf descriptor) are issued
-// together, so a row whose leaves fit UNR chunks (K = 5 with leaves of up to
-// 64 points: 6) costs one dependent memory round trip after its table.  (A
+// together, so a row whose unique leaves fit UNR chunks (six for d <= 16; a
+// 1080p row with K = 5 has five on average) costs one dependent memory round
+// trip after its table.  (A
 // first round of dimensions 0-3 that filtered which rows load the rest cut
 // the bytes but added a round trip per row: about 5 % slower.)  Leaf ranges
 // come from the leaf descriptors (propleaf), and the next
