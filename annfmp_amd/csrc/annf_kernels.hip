@@ -1747,7 +1747,9 @@ __device__ __forceinline__ void wset_reduce(const BlockSet &S, int &n, int K, in
         const uint32_t hi = (uint32_t)(mk >> 32), lo = (uint32_t)mk;
         const uint32_t mh = wave_min_u32(hi);
         const uint32_t mlo = wave_min_u32(hi == mh ? lo : 0xFFFFFFFFu);
-        const int wl = __builtin_ctzll(__ballot(hi == mh && lo == mlo));
+        const unsigned long long win = __ballot(hi == mh && lo == mlo && mk != ~0ull);
+        if (win == 0ull) break;   // no item left (not reached: r < n items hold real keys)
+        const int wl = __builtin_ctzll(win);
         const int wp = __builtin_amdgcn_readlane(mp, wl), wq = __builtin_amdgcn_readlane(ml, wl);
         if (lane == r) {
             rk = ((uint64_t)mh << 32) | mlo;
